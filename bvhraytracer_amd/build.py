@@ -1,0 +1,72 @@
+"""Builds librt.so (host C++ + HIP for gfx950) in-tree.
+
+Host sources are compiled with g++, HIP sources with hipcc, and the shared
+library is linked with hipcc.  Every float path is built with
+-ffp-contract=off: the reference kernels (ISPC, sse4-i32x4) have no FMA, and
+bit-exact parity depends on the same operation sequence.
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "librt.so")
+OBJ = os.path.join(HERE, "_build")
+
+HOST_SOURCES = ["scene.cpp", "builder.cpp", "host_api.cpp"]
+HIP_SOURCES = ["rt_kernels.hip", "rt_api.hip"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+CXXFLAGS = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-pthread",
+            "-Wall", "-Wextra", "-Wno-unused-parameter"]
+HIPFLAGS = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+            "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-gpu-flush-denormals-to-zero",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wno-unused-parameter", "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout)
+    return r.stdout
+
+
+def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(OBJ, src + ".o")
+        jobs.append(["g++", *CXXFLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    for src in HIP_SOURCES:
+        obj = os.path.join(OBJ, src + ".o")
+        jobs.append([HIPCC, *HIPFLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for out in ex.map(_run, jobs):
+            if verbose and out.strip():
+                print(out)
+    objs = [os.path.join(OBJ, s + ".o") for s in HOST_SOURCES + HIP_SOURCES]
+    tmp = OUT + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-pthread"])
+    os.replace(tmp, OUT)
+    return OUT
+
+
+def build_oracle():
+    """Builds the oracle (test infrastructure) with its own Makefile."""
+    _run(["make", "-s", "-C", os.path.join(os.path.dirname(HERE), "oracle")])
+
+
+def clean():
+    shutil.rmtree(OBJ, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    build_oracle()
+    print(OUT)

@@ -1,0 +1,363 @@
+// N-ary SAH BVH builder and FastNode flattener (host side of the path).
+//
+// Restates, step for step, the reference builder:
+//   Bvh::Bvh / Bvh::recursiveBvh        src/accelerationStructure/bvh.cpp:28-40
+//   chooseAxisAndSort                   src/accelerationStructure/aabb.h:40-69
+//   Aabb::calculateBounds / getCenter   aabb.h:113-146
+//   sweepRight / sweepLeft              aabb.h:205-222 (left sweep off by one)
+//   Node::sah (step-function leaf cost) src/accelerationStructure/node.h:66-100
+//   computerBestSplit(Sort)             src/accelerationStructure/aabb.cpp:4-69
+//   Aabb::recursiveBvh                  aabb.cpp:71-206
+//   calculateTraverseOrderEachAxis      src/accelerationStructure/node.cpp:254-275
+//   Bvh::bvhAnalysis (depth, counts)    bvh.cpp:338-389
+//   FastNodeManager ctor + customTreeOrder  fastNodeManager.h:1495-1688
+// and RayTracer::preparePrimitives (rayTracer.cpp:747-754).  All float
+// arithmetic is FP32 in the reference's evaluation order (build with
+// -ffp-contract=off).  The reference sorts with an unstable parallel
+// std::sort; this builder uses std::stable_sort so the topology is
+// deterministic (equal keys keep their current relative order).
+#include <algorithm>
+#include <atomic>
+#include <limits>
+#include <memory>
+#include <thread>
+
+#include "host_common.h"
+
+namespace rt {
+
+namespace {
+
+struct Prim {
+    float bmin[3], bmax[3], c[3];
+    uint32_t id;
+};
+
+// glm::min / glm::max component semantics: min(x,y) = (y < x) ? y : x,
+// max(x,y) = (x < y) ? y : x.
+inline float gmin(float x, float y) { return (y < x) ? y : x; }
+inline float gmax(float x, float y) { return (x < y) ? y : x; }
+
+struct Box {
+    float mn[3] = {222222.0f, 222222.0f, 222222.0f};
+    float mx[3] = {-222222.0f, -222222.0f, -222222.0f};
+    void add(const Prim& p) {
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = gmin(mn[a], p.bmin[a]);
+            mx[a] = gmax(mx[a], p.bmax[a]);
+        }
+    }
+    // Aabb::getSurfaceArea (aabb.h:130-134)
+    float area() const {
+        float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+        return 2.0f * (dx * dy + dx * dz + dy * dz);
+    }
+};
+
+// maxDimension (glmUtil.h:7-11): ties go to the later axis.
+inline int max_dimension(float x, float y, float z) {
+    return (x > y) ? ((x > z) ? 0 : 2) : ((y > z) ? 1 : 2);
+}
+
+// Node::sah(invArea, leafTarget) for primCount != 0 (node.h:78-91)
+inline float sah_leaf(size_t count, float area, float invArea, int leafTarget) {
+    size_t steps = ((count - 1) / (size_t)leafTarget) + 1;
+    return (float)steps * area * invArea * (float)leafTarget;
+}
+
+struct BNode {
+    Box box;
+    uint32_t begin = 0, end = 0;  // primitive range (empty for inner nodes)
+    uint8_t depth = 0;
+    std::vector<std::unique_ptr<BNode>> children;
+    std::vector<int8_t> order[3];
+    mutable uint32_t fid = 0;  // position in the flattened node array
+};
+
+struct Builder {
+    std::vector<Prim>& prims;
+    unsigned N, L;
+    bool sortEachSplit;
+    int leafSplitOption;
+    float sahFactor;
+    std::atomic<int> spareThreads;
+
+    Builder(std::vector<Prim>& p, unsigned n, unsigned l, bool s, int o, float f, int threads)
+        : prims(p), N(n), L(l), sortEachSplit(s), leafSplitOption(o), sahFactor(f),
+          spareThreads(threads - 1) {}
+
+    // chooseAxisAndSort (aabb.h:40-69)
+    int choose_axis_and_sort(uint32_t b, uint32_t e) {
+        float mn[3] = {222222.0f, 222222.0f, 222222.0f};
+        float mx[3] = {-222222.0f, -222222.0f, -222222.0f};
+        for (uint32_t i = b; i < e; ++i)
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = gmin(mn[a], prims[i].c[a]);
+                mx[a] = gmax(mx[a], prims[i].c[a]);
+            }
+        int axis = max_dimension(mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]);
+        auto less = [axis](const Prim& p1, const Prim& p2) { return p1.c[axis] < p2.c[axis]; };
+        if (!std::is_sorted(prims.begin() + b, prims.begin() + e, less))
+            std::stable_sort(prims.begin() + b, prims.begin() + e, less);
+        return axis;
+    }
+
+    // computerBestSplit(Sort) (aabb.cpp:4-69).  Returns the split position.
+    uint32_t best_split(uint32_t b, uint32_t e, float invArea, float& cost) {
+        if (sortEachSplit) choose_axis_and_sort(b, e);
+        const size_t size = e - b;
+        std::vector<float> metric(size - 1, 0.0f);
+        Box left;  // sweepRight reads *(++primitiveEnd): prims b+1 .. b+m+1
+        for (size_t m = 0; m + 1 < size; ++m) {
+            left.add(prims[b + m + 1]);
+            metric[m] += sah_leaf(m + 1, left.area(), invArea, (int)L);
+        }
+        Box right;  // sweepLeft reads *(--primitiveBegin): prims b+m+1 .. e-1
+        for (size_t m = size - 1; m-- > 0;) {
+            right.add(prims[b + m + 1]);
+            metric[m] += sah_leaf(size - 1 - m, right.area(), invArea, (int)L);
+        }
+        size_t best = 0;  // std::min_element: first minimum
+        for (size_t m = 1; m < metric.size(); ++m)
+            if (metric[m] < metric[best]) best = m;
+        cost = metric[best];
+        return b + (uint32_t)best + 1;
+    }
+
+    void bounds_of(BNode& n) {
+        n.box = Box();
+        for (uint32_t i = n.begin; i < n.end; ++i) n.box.add(prims[i]);
+    }
+
+    // Aabb::recursiveBvh (aabb.cpp:71-206) + Node::recursiveBvh (node.cpp:23-30)
+    void recurse(BNode& node) {
+        const size_t count = node.end - node.begin;
+        if (count <= L && (leafSplitOption == 0 || leafSplitOption == 2)) return;
+        choose_axis_and_sort(node.begin, node.end);
+        struct Iv {
+            uint32_t b, e;
+        };
+        std::vector<Iv> work{{node.begin, node.end}};
+        size_t bestI = 0;
+        const float invArea = 1.0f / node.box.area();
+        for (unsigned br = 0; br + 1 < N; ++br) {
+            size_t primCounter = 0;
+            for (size_t i = 0; i < work.size(); ++i)
+                if (primCounter < work[i].e - work[i].b) {
+                    primCounter = work[i].e - work[i].b;
+                    bestI = i;
+                }
+            if (primCounter <= L) break;
+            float cost;
+            uint32_t split = best_split(work[bestI].b, work[bestI].e, invArea, cost);
+            Iv p2{split, work[bestI].e};
+            work[bestI].e = split;
+            work.insert(work.begin() + bestI + 1, p2);
+        }
+        if (leafSplitOption == 1 || leafSplitOption == 2) {
+            for (size_t br = work.size() - 1; br + 1 < N; ++br) {
+                float bestImp = 0.0f;
+                long bestId = -1;
+                uint32_t bestSplit = 0;
+                for (size_t i = 0; i < work.size(); ++i) {
+                    int cnt = (int)(work[i].e - work[i].b);
+                    if (cnt == 1) continue;
+                    float leafCost = (float)cnt;
+                    float splitCost = 0.0f;
+                    uint32_t split = best_split(work[i].b, work[i].e, invArea, splitCost);
+                    splitCost += sahFactor;
+                    if (leafCost - splitCost > bestImp) {
+                        bestImp = leafCost - splitCost;
+                        bestId = (long)i;
+                        bestSplit = split;
+                    }
+                }
+                if (bestImp == 0.0f) break;
+                Iv p2{bestSplit, work[bestId].e};
+                work[bestId].e = bestSplit;
+                work.insert(work.begin() + bestId + 1, p2);
+            }
+        }
+        if (work.size() == 1) return;
+        for (auto& iv : work) {
+            auto c = std::make_unique<BNode>();
+            c->begin = iv.b;
+            c->end = iv.e;
+            c->depth = (uint8_t)(node.depth + 1);
+            bounds_of(*c);
+            node.children.push_back(std::move(c));
+        }
+        if (sortEachSplit) traverse_order(node);
+        node.begin = node.end;  // primitiveBegin = primitiveEnd: inner node
+        // children are independent: build the big ones on spare threads
+        std::vector<std::thread> ts;
+        for (auto& c : node.children) {
+            BNode* cp = c.get();
+            if (cp->end - cp->begin > 20000 && spareThreads.fetch_sub(1) > 0) {
+                ts.emplace_back([this, cp] {
+                    recurse(*cp);
+                    spareThreads.fetch_add(1);
+                });
+            } else {
+                if (cp->end - cp->begin > 20000) spareThreads.fetch_add(1);
+                recurse(*cp);
+            }
+        }
+        for (auto& t : ts) t.join();
+    }
+
+    // Node::calculateTraverseOrderEachAxis (node.cpp:254-275); the key is
+    // Aabb::getCenter = (max + min) + 0.5f (aabb.h:142-146), ties by index.
+    void traverse_order(BNode& node) {
+        const size_t cc = node.children.size();
+        for (int face = 0; face < 3; ++face) {
+            std::vector<std::pair<float, int>> t(cc);
+            for (size_t i = 0; i < cc; ++i) {
+                const Box& bx = node.children[i]->box;
+                t[i] = {(bx.mx[face] + bx.mn[face]) + 0.5f, (int)i};
+            }
+            std::sort(t.begin(), t.end());
+            node.order[face].clear();
+            for (auto& p : t) node.order[face].push_back((int8_t)p.second);
+        }
+    }
+};
+
+void flatten(const BNode& root, const std::vector<Prim>& prims, const float* tris, unsigned G,
+             HostBvh& out) {
+    // customTreeOrder (fastNodeManager.h:1675-1688): root, then every node's
+    // children as one block, recursively in child order.  Entering a node
+    // appends its children; the iteration below is that recursion unrolled.
+    std::vector<const BNode*> nodes{&root};
+    struct Frame {
+        const BNode* n;
+        size_t next;
+    };
+    root.fid = 0;
+    auto append_children = [&nodes](const BNode* n) {
+        for (auto& c : n->children) {
+            c->fid = (uint32_t)nodes.size();
+            nodes.push_back(c.get());
+        }
+    };
+    append_children(&root);
+    std::vector<Frame> fs{{&root, 0}};
+    while (!fs.empty()) {
+        Frame& f = fs.back();
+        if (f.next < f.n->children.size()) {
+            const BNode* c = f.n->children[f.next++].get();
+            append_children(c);
+            fs.push_back({c, 0});
+        } else {
+            fs.pop_back();
+        }
+    }
+    const uint32_t M = out.M, Lm = out.Lm;
+    const size_t nn = nodes.size();
+    auto id_of = [](const BNode* p) { return p->fid; };
+    out.bounds.assign(nn * 6 * M, 0.0f);
+    out.base.assign(nn, 0);
+    out.order.assign(nn * 3 * M, 0);
+    out.childType.assign(nn, 0);
+    out.tri.clear();
+    out.triIds.clear();
+    const float qnan = std::numeric_limits<float>::quiet_NaN();
+    double depthSum = 0;
+    uint32_t leaves = 0, triCount = 0, maxDepth = 0;
+    for (size_t i = 0; i < nn; ++i) {
+        const BNode& n = *nodes[i];
+        float* bd = &out.bounds[i * 6 * M];
+        const size_t cc = n.children.size();
+        if (cc > 0) {
+            out.base[i] = id_of(n.children[0].get());
+            const uint32_t nrs = (uint32_t)(((cc - 1) / G + 1) * G);
+            for (int a = 0; a < 3; ++a)
+                for (size_t j = 0; j < cc; ++j) {
+                    bd[a * nrs + j] = n.children[j]->box.mn[a];
+                    bd[nrs * 3 + a * nrs + j] = n.children[j]->box.mx[a];
+                }
+            if (nrs != M) {
+                bd[6 * M - 2] = (float)nrs;
+                bd[6 * M - 1] = qnan;
+            }
+            uint16_t ct = 0;
+            for (size_t j = 0; j < cc; ++j)
+                if (!n.children[j]->children.empty()) ct |= (uint16_t)(1u << j);
+            out.childType[i] = ct;
+            for (int a = 0; a < 3; ++a)
+                for (uint32_t p = 0; p < M; ++p)
+                    out.order[i * 3 * M + a * M + p] =
+                        (p < n.order[0].size()) ? n.order[a][p] : (int8_t)0;
+        }
+        const uint32_t pc = n.end - n.begin;
+        if (pc > 0) {
+            const uint32_t pBegin = (uint32_t)out.tri.size();
+            const uint32_t lrs = ((pc - 1) / G + 1) * G;
+            out.tri.resize(pBegin + (size_t)9 * Lm, 0.0f);
+            for (int v = 0; v < 3; ++v)
+                for (int a = 0; a < 3; ++a)
+                    for (uint32_t j = 0; j < pc; ++j)
+                        out.tri[pBegin + (v * 3 + a) * lrs + j] =
+                            tris[(size_t)prims[n.begin + j].id * 9 + v * 3 + a];
+            if (lrs != Lm) {
+                out.tri[pBegin + Lm * 9 - 2] = (float)lrs;
+                out.tri[pBegin + Lm * 9 - 1] = qnan;
+            }
+            bd[0] = qnan;
+            out.base[i] = pBegin;
+            for (uint32_t j = 0; j < Lm; ++j)
+                out.triIds.push_back(j < pc ? (int32_t)prims[n.begin + j].id : -1);
+            depthSum += n.depth;
+            ++leaves;
+            triCount += pc;
+            maxDepth = std::max<uint32_t>(maxDepth, n.depth);
+        }
+    }
+    out.averageDepth = leaves ? (float)(depthSum / leaves) : 0.0f;
+    out.triangleCount = triCount;
+    out.maxDepth = maxDepth;
+}
+
+}  // namespace
+
+int build_bvh(const float* tris, uint32_t n, const rt_build_params& p, HostBvh& out) {
+    if (!tris || n == 0) return set_error(RT_ERR_ARG, "rt_build: empty triangle list");
+    if (p.gang != 4 && p.gang != 8 && p.gang != 16)
+        return set_error(RT_ERR_UNSUPPORTED, "rt_build: gang size must be 4, 8 or 16");
+    if (p.branching < 2 || p.leaf_size < 1)
+        return set_error(RT_ERR_ARG, "rt_build: branching factor >= 2 and leaf size >= 1 required");
+    const uint32_t G = p.gang;
+    out.N = p.branching;
+    out.L = p.leaf_size;
+    out.G = G;
+    out.M = (p.branching + G - 1) / G * G;
+    out.Lm = (p.leaf_size + G - 1) / G * G;
+    if (out.M > 16 || out.Lm > 16)
+        return set_error(RT_ERR_UNSUPPORTED, "rt_build: node and leaf memory are limited to 16");
+    std::vector<Prim> prims(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* t = tris + (size_t)i * 9;
+        Prim& q = prims[i];
+        // Triangle::updateBounds (triangle.cpp:41-49)
+        for (int a = 0; a < 3; ++a) {
+            float lo = gmin(t[a], t[3 + a]), hi = gmax(t[a], t[3 + a]);
+            q.bmin[a] = gmin(lo, t[6 + a]);
+            q.bmax[a] = gmax(hi, t[6 + a]);
+            q.c[a] = q.bmax[a] * 0.5f + q.bmin[a] * 0.5f;
+        }
+        q.id = i;
+    }
+    int threads = p.threads ? (int)p.threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    Builder b(prims, p.branching, p.leaf_size, p.sort_each_split != 0, p.leaf_split_option,
+              p.sah_factor, threads);
+    b.choose_axis_and_sort(0, n);  // RayTracer::preparePrimitives
+    BNode root;
+    root.begin = 0;
+    root.end = n;
+    b.bounds_of(root);
+    b.recurse(root);
+    flatten(root, prims, tris, G, out);
+    return RT_OK;
+}
+
+}  // namespace rt

@@ -1,0 +1,462 @@
+// Device-side C ABI: rt_upload_bvh / rt_trace / rt_stats / rt_release.
+//
+// rt_upload_bvh replaces FastNodeManager's role as the owner of the
+// flattened BVH (fastNodeManager.h:158-215): it reads the reference FastNode
+// arrays once, decodes the loop counts the reference decodes on every visit
+// (getNodeLoopCount / getLeafLoopCount + callIspcTemplateNotConst,
+// fastNodeManager.h:1728-1763,271-275), and writes the device layout of
+// rt_layout.h into HBM.  rt_trace replaces the per-visit ISPC calls and the
+// per-ray traversal loops with one launch per ray batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "host_common.h"
+#include "rt_kernels.h"
+#include "rt_layout.h"
+
+using namespace rt;
+
+namespace {
+
+int hip_fail(hipError_t e, const char* what) {
+    return set_error(RT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr, what)                       \
+    do {                                          \
+        hipError_t _e = (expr);                   \
+        if (_e != hipSuccess) return hip_fail(_e, what); \
+    } while (0)
+
+// Per-stream scratch: work counters, stack spill and host staging buffers.
+struct Workspace {
+    uint32_t* work = nullptr;  // 4 words
+    uint32_t* spill = nullptr;
+    size_t spillBytes = 0;
+    void* stage = nullptr;  // rays | out | counts for host-pointer calls
+    size_t stageBytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+};
+
+struct LayoutInfo {
+    uint32_t inner = 0, leaves = 0, maxDepth = 0;
+};
+
+uint32_t decode_node_k(const rt_bvh_desc* d, uint32_t i) {
+    const uint32_t M = d->node_memory;
+    int k = (int)M;
+    if (M != d->gang) {
+        const float* bd = d->bounds + (size_t)i * 6 * M;
+        if (std::isnan(bd[M * 6 - 1])) k = (int)bd[M * 6 - 2];
+    }
+    return (k == 4 || k == 8 || k == 12) ? (uint32_t)k : 16u;
+}
+
+uint32_t decode_leaf_k(const rt_bvh_desc* d, uint32_t pib) {
+    const uint32_t Lm = d->leaf_memory;
+    int k = (int)Lm;
+    if (Lm != d->gang) {
+        const uint64_t end = (uint64_t)pib + (uint64_t)Lm * 9;
+        if (std::isnan(d->tri_points[end - 1])) k = (int)d->tri_points[end - 2];
+    }
+    return (k == 4 || k == 8 || k == 12) ? (uint32_t)k : 16u;
+}
+
+}  // namespace
+
+struct rt_scene {
+    std::mutex mu;
+    int device = 0;
+    uint32_t M = 0, Lm = 0, G = 0, nodeCount = 0;
+    uint32_t inner = 0, leaves = 0, maxDepth = 0;
+    uint32_t rootEntry = 0;
+    uint32_t* nodes = nullptr;
+    float* tris = nullptr;
+    int32_t* triIds = nullptr;
+    uint32_t* nodeEntry = nullptr;
+    uint64_t bytes = 0;
+    uint32_t blockSize = 256, stackWindow = 16;
+    uint32_t cus = 0;
+    std::unordered_map<hipStream_t, Workspace> ws;
+    hipStream_t ownStream = nullptr;
+    float lastMs = 0;
+    uint64_t raysTraced = 0;
+    uint32_t errorRays = 0;
+};
+
+namespace {
+
+// Builds the device layout on the host.  Validates the layout contract the
+// reference's traversal relies on (contiguous children, loop counts within
+// the node/leaf memory).
+int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<float>& tris,
+            std::vector<int32_t>& triIds, std::vector<uint32_t>& entry, LayoutInfo& info) {
+    const uint32_t M = d->node_memory, Lm = d->leaf_memory, G = d->gang, n = d->node_count;
+    if (!(M == 4 || M == 8 || M == 12 || M == 16) || !(Lm == 4 || Lm == 8 || Lm == 12 || Lm == 16))
+        return set_error(RT_ERR_UNSUPPORTED, "rt_upload_bvh: node/leaf memory must be 4, 8, 12 or 16");
+    if (!(G == 4 || G == 8 || G == 16) || M % G || Lm % G)
+        return set_error(RT_ERR_UNSUPPORTED, "rt_upload_bvh: gang must be 4, 8 or 16 and divide M and Lm");
+    if (n == 0 || !d->bounds || !d->base || !d->order || !d->tri_points || !d->tri_ids)
+        return set_error(RT_ERR_ARG, "rt_upload_bvh: missing arrays");
+    const uint64_t block = 9ull * Lm;
+    if (d->tri_float_count % block)
+        return set_error(RT_ERR_LAYOUT, "rt_upload_bvh: trianglePoints is not a whole number of leaf blocks");
+    const uint64_t nLeafBlocks = d->tri_float_count / block;
+    std::vector<uint8_t> isLeaf(n);
+    std::vector<uint32_t> index(n), kOf(n);
+    uint32_t nInner = 0, nLeaves = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        isLeaf[i] = std::isnan(d->bounds[(size_t)i * 6 * M]) ? 1 : 0;
+        if (isLeaf[i]) {
+            const uint32_t pib = d->base[i];
+            if (pib % block || pib / block >= nLeafBlocks)
+                return set_error(RT_ERR_LAYOUT, "rt_upload_bvh: leaf primIdBegin is not a leaf block start");
+            index[i] = (uint32_t)(pib / block);
+            kOf[i] = decode_leaf_k(d, pib);
+            if (kOf[i] > Lm) return set_error(RT_ERR_LAYOUT, "rt_upload_bvh: leaf loop count exceeds leaf memory");
+            ++nLeaves;
+        } else {
+            index[i] = nInner++;
+            kOf[i] = decode_node_k(d, i);
+            if (kOf[i] > M) return set_error(RT_ERR_LAYOUT, "rt_upload_bvh: node loop count exceeds node memory");
+        }
+    }
+    if (nInner >= (1u << 28) || nLeafBlocks >= (1u << 28))
+        return set_error(RT_ERR_UNSUPPORTED, "rt_upload_bvh: more than 2^28 inner nodes or leaves");
+    entry.resize(n);
+    for (uint32_t i = 0; i < n; ++i)
+        entry[i] = (isLeaf[i] ? kEntryLeaf : 0u) | (((kOf[i] / 4) - 1) << kEntryKShift) | index[i];
+    const uint32_t RECW = record_words(M), OW = order_words(M);
+    nodes.assign((size_t)nInner * RECW, 0u);
+    std::vector<uint32_t> depth(n, 0);
+    uint32_t maxDepth = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (isLeaf[i]) continue;
+        uint32_t* rec = &nodes[(size_t)index[i] * RECW];
+        const float* bd = d->bounds + (size_t)i * 6 * M;
+        const uint32_t k = kOf[i];
+        for (uint32_t p = 0; p < 6; ++p)
+            for (uint32_t s = 0; s < k; ++s) {
+                float v = bd[p * k + s];
+                std::memcpy(&rec[p * M + s], &v, 4);
+            }
+        const int8_t* ord = d->order + (size_t)i * 3 * M;
+        int mx = 0;
+        for (uint32_t q = 0; q < M; ++q) mx = std::max<int>(mx, ord[q]);
+        const uint32_t c = (uint32_t)mx + 1;
+        const uint32_t cb = d->base[i];
+        if ((uint64_t)cb + c > n || cb <= i)
+            return set_error(RT_ERR_LAYOUT, "rt_upload_bvh: child block outside the node array");
+        for (uint32_t a = 0; a < 3; ++a) {
+            std::vector<int> seen(c, 0);
+            for (uint32_t q = 0; q < M; ++q) {
+                const int v = ord[a * M + q];
+                if (v < 0 || v >= (int)c || (q >= c && v != 0))
+                    return set_error(RT_ERR_LAYOUT, "rt_upload_bvh: traverse order is not a padded permutation");
+                if (q < c) seen[v]++;
+            }
+            for (uint32_t s = 0; s < c; ++s)
+                if (seen[s] != 1) return set_error(RT_ERR_LAYOUT, "rt_upload_bvh: traverse order is not a permutation");
+        }
+        uint32_t leafMask = 0, kcodes = 0, innerBase = 0xFFFFFFFFu, leafBase = 0xFFFFFFFFu;
+        uint32_t ni = 0, nl = 0;
+        for (uint32_t s = 0; s < c; ++s) {
+            const uint32_t ch = cb + s;
+            kcodes |= ((kOf[ch] / 4) - 1) << (2 * s);
+            depth[ch] = depth[i] + 1;
+            maxDepth = std::max(maxDepth, depth[ch]);
+            if (isLeaf[ch]) {
+                leafMask |= 1u << s;
+                if (leafBase == 0xFFFFFFFFu) leafBase = index[ch];
+                if (index[ch] != leafBase + nl++)
+                    return set_error(RT_ERR_LAYOUT, "rt_upload_bvh: leaf children are not contiguous");
+            } else {
+                if (innerBase == 0xFFFFFFFFu) innerBase = index[ch];
+                if (index[ch] != innerBase + ni++)
+                    return set_error(RT_ERR_LAYOUT, "rt_upload_bvh: inner children are not contiguous");
+            }
+        }
+        uint32_t* h = rec + 6 * M;
+        h[0] = innerBase == 0xFFFFFFFFu ? 0 : innerBase;
+        h[1] = leafBase == 0xFFFFFFFFu ? 0 : leafBase;
+        h[2] = leafMask | (c << 16) | (k << 24);
+        h[3] = kcodes;
+        h[4] = cb;
+        for (uint32_t a = 0; a < 3; ++a)
+            for (uint32_t q = 0; q < M; ++q)
+                h[5 + a * OW + q / 8] |= ((uint32_t)ord[a * M + q] & 15u) << (4 * (q % 8));
+    }
+    const uint32_t LS = leaf_stride(Lm);
+    tris.assign((size_t)nLeafBlocks * LS, 0.0f);
+    for (uint64_t l = 0; l < nLeafBlocks; ++l)
+        std::memcpy(&tris[l * LS], d->tri_points + l * block, block * sizeof(float));
+    triIds.assign(d->tri_ids, d->tri_ids + nLeafBlocks * Lm);
+    info.inner = nInner;
+    info.leaves = (uint32_t)nLeafBlocks;
+    info.maxDepth = maxDepth;
+    return RT_OK;
+}
+
+template <class T>
+int dev_upload(T** dst, const std::vector<T>& src, uint64_t& bytes) {
+    const size_t b = std::max<size_t>(src.size() * sizeof(T), 64);
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(dst), b), "hipMalloc");
+    bytes += b;
+    if (!src.empty()) HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+    return RT_OK;
+}
+
+void free_scene(rt_scene* s) {
+    if (!s) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(s->device);
+    for (auto& kv : s->ws) {
+        Workspace& w = kv.second;
+        if (w.work) (void)hipFree(w.work);
+        if (w.spill) (void)hipFree(w.spill);
+        if (w.stage) (void)hipFree(w.stage);
+        if (w.ev0) (void)hipEventDestroy(w.ev0);
+        if (w.ev1) (void)hipEventDestroy(w.ev1);
+    }
+    if (s->nodes) (void)hipFree(s->nodes);
+    if (s->tris) (void)hipFree(s->tris);
+    if (s->triIds) (void)hipFree(s->triIds);
+    if (s->nodeEntry) (void)hipFree(s->nodeEntry);
+    if (s->ownStream) (void)hipStreamDestroy(s->ownStream);
+    (void)hipSetDevice(cur);
+    delete s;
+}
+
+// Stack capacity per ray: every level keeps at most M-1 pending siblings
+// plus the children of the node being expanded.
+uint32_t stack_capacity(const rt_scene* s) { return ((1 + (s->maxDepth + 1) * s->M + 8) + 7) / 8 * 8; }
+
+int get_workspace(rt_scene* s, hipStream_t st, Workspace*& out) {
+    Workspace& w = s->ws[st];
+    if (!w.work) {
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.work), 64), "hipMalloc(work)");
+        HIP_TRY(hipEventCreate(&w.ev0), "hipEventCreate");
+        HIP_TRY(hipEventCreate(&w.ev1), "hipEventCreate");
+    }
+    out = &w;
+    return RT_OK;
+}
+
+int ensure(void** p, size_t& have, size_t need, const char* what) {
+    if (have >= need) return RT_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    have = 0;
+    const size_t grow = std::max(need, have + have / 2);
+    HIP_TRY(hipMalloc(p, grow), what);
+    have = grow;
+    return RT_OK;
+}
+
+}  // namespace
+
+extern "C" int rt_scene_footprint(const rt_bvh_desc* d, uint64_t* bytes) {
+    if (!d || !bytes) return set_error(RT_ERR_ARG, "rt_scene_footprint: null argument");
+    const uint32_t M = d->node_memory, Lm = d->leaf_memory;
+    if (!(M == 4 || M == 8 || M == 12 || M == 16) || !(Lm == 4 || Lm == 8 || Lm == 12 || Lm == 16))
+        return set_error(RT_ERR_UNSUPPORTED, "rt_scene_footprint: node/leaf memory must be 4, 8, 12 or 16");
+    uint64_t inner = 0;
+    for (uint32_t i = 0; i < d->node_count; ++i)
+        if (!std::isnan(d->bounds[(size_t)i * 6 * M])) ++inner;
+    const uint64_t leaves = d->tri_float_count / (9ull * Lm);
+    *bytes = inner * record_words(M) * 4ull + leaves * leaf_stride(Lm) * 4ull + leaves * Lm * 4ull +
+             (uint64_t)d->node_count * 4ull;
+    return RT_OK;
+}
+
+extern "C" int rt_upload_bvh(const rt_bvh_desc* d, rt_scene_t* out) {
+    if (!d || !out) return set_error(RT_ERR_ARG, "rt_upload_bvh: null argument");
+    std::vector<uint32_t> nodes, entry;
+    std::vector<float> tris;
+    std::vector<int32_t> triIds;
+    LayoutInfo info;
+    int rc;
+    try {
+        rc = convert(d, nodes, tris, triIds, entry, info);
+    } catch (const std::bad_alloc&) {
+        rc = set_error(RT_ERR_NOMEM, "rt_upload_bvh: out of host memory");
+    }
+    if (rc != RT_OK) return rc;
+    rt_scene* s = new (std::nothrow) rt_scene;
+    if (!s) return set_error(RT_ERR_NOMEM, "rt_upload_bvh: out of host memory");
+    (void)hipGetDevice(&s->device);
+    s->M = d->node_memory;
+    s->Lm = d->leaf_memory;
+    s->G = d->gang;
+    s->nodeCount = d->node_count;
+    s->inner = info.inner;
+    s->leaves = info.leaves;
+    s->maxDepth = info.maxDepth;
+    s->rootEntry = entry[0];
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, s->device) == hipSuccess) s->cus = (uint32_t)prop.multiProcessorCount;
+    if (!s->cus) s->cus = 256;
+    if ((rc = dev_upload(&s->nodes, nodes, s->bytes)) != RT_OK || (rc = dev_upload(&s->tris, tris, s->bytes)) != RT_OK ||
+        (rc = dev_upload(&s->triIds, triIds, s->bytes)) != RT_OK ||
+        (rc = dev_upload(&s->nodeEntry, entry, s->bytes)) != RT_OK) {
+        free_scene(s);
+        return rc;
+    }
+    *out = s;
+    return RT_OK;
+}
+
+extern "C" int rt_set_launch(rt_scene_t s, uint32_t block_size, uint32_t stack_window) {
+    if (!s) return set_error(RT_ERR_ARG, "rt_set_launch: null scene");
+    if (block_size && (block_size % 64 || block_size > 256))
+        return set_error(RT_ERR_ARG, "rt_set_launch: block size must be 64, 128, 192 or 256");
+    if (stack_window && stack_window != 8 && stack_window != 16 && stack_window != 32)
+        return set_error(RT_ERR_ARG, "rt_set_launch: stack window must be 8, 16 or 32");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (block_size) s->blockSize = block_size;
+    if (stack_window) s->stackWindow = stack_window;
+    return RT_OK;
+}
+
+extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, void* out,
+                        rt_counts* counts, void* hip_stream) {
+    if (!s) return set_error(RT_ERR_ARG, "rt_trace: null scene");
+    if (n == 0) return RT_OK;
+    if (!rays || !out) return set_error(RT_ERR_ARG, "rt_trace: null rays or output");
+    if ((flags & RT_COUNTS) && !counts) return set_error(RT_ERR_ARG, "rt_trace: RT_COUNTS without a counts buffer");
+    if (n >= 0x80000000u) return set_error(RT_ERR_ARG, "rt_trace: at most 2^31-1 rays per call");
+    if ((flags & RT_ANY) && (flags & RT_SAVE_DIST))
+        return set_error(RT_ERR_ARG, "rt_trace: RT_ANY and RT_SAVE_DIST are exclusive");
+    if (flags & (RT_STREAM | RT_COOP))
+        return set_error(RT_ERR_UNSUPPORTED, "rt_trace: stream and cooperative scheduling are not built yet");
+    const int mode = (flags & RT_ANY) ? kAny : ((flags & RT_SAVE_DIST) ? kSaveDist : kClosest);
+    std::lock_guard<std::mutex> lk(s->mu);
+    int prevDev = 0;
+    (void)hipGetDevice(&prevDev);
+    if (prevDev != s->device) HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    struct Restore {
+        int dev, prev;
+        ~Restore() {
+            if (dev != prev) (void)hipSetDevice(prev);
+        }
+    } restore{s->device, prevDev};
+    hipStream_t st = reinterpret_cast<hipStream_t>(hip_stream);
+    if (!st) {
+        if (!s->ownStream) HIP_TRY(hipStreamCreateWithFlags(&s->ownStream, hipStreamNonBlocking), "hipStreamCreate");
+        st = s->ownStream;
+    }
+    Workspace* w = nullptr;
+    int rc = get_workspace(s, st, w);
+    if (rc != RT_OK) return rc;
+
+    const uint32_t S = s->stackWindow;
+    uint32_t bd = s->blockSize;
+    while (bd > 64 && trace_lds_bytes(s->M, mode, S, bd) > 65536) bd -= 64;
+    const int occ = std::max(1, trace_occupancy(s->M, mode, S, bd));
+    const uint64_t wantBlocks = (uint64_t)s->cus * (uint64_t)occ;
+    const uint64_t needBlocks = ((uint64_t)n + bd - 1) / bd;
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min(wantBlocks, needBlocks));
+    const uint32_t cap = stack_capacity(s);
+    const uint32_t spillCap = cap > S ? cap - S : 0;
+    const size_t slots = (size_t)grid * bd;
+    const size_t spillNeed = std::max<size_t>(64, slots * spillCap * 4 * (mode == kSaveDist ? 2 : 1));
+    if ((rc = ensure(reinterpret_cast<void**>(&w->spill), w->spillBytes, spillNeed, "hipMalloc(spill)")) != RT_OK)
+        return rc;
+
+    const bool dev = (flags & RT_DEVICE_PTRS) != 0;
+    const size_t outBytes = (mode == kAny) ? (size_t)n : (size_t)n * sizeof(rt_hit);
+    const size_t rayBytes = (size_t)n * sizeof(rt_ray);
+    const size_t cntBytes = (flags & RT_COUNTS) ? (size_t)n * sizeof(rt_counts) : 0;
+    const rt_ray* dRays = rays;
+    void* dOut = out;
+    rt_counts* dCounts = (flags & RT_COUNTS) ? counts : nullptr;
+    if (!dev) {
+        const size_t outOff = (rayBytes + 255) / 256 * 256;
+        const size_t cntOff = outOff + (outBytes + 255) / 256 * 256;
+        if ((rc = ensure(&w->stage, w->stageBytes, cntOff + cntBytes + 256, "hipMalloc(stage)")) != RT_OK) return rc;
+        char* base = static_cast<char*>(w->stage);
+        dRays = reinterpret_cast<const rt_ray*>(base);
+        dOut = base + outOff;
+        dCounts = cntBytes ? reinterpret_cast<rt_counts*>(base + cntOff) : nullptr;
+        HIP_TRY(hipMemcpyAsync(const_cast<rt_ray*>(dRays), rays, rayBytes, hipMemcpyHostToDevice, st), "hipMemcpy(rays)");
+    }
+    KParams p{};
+    p.nodes = s->nodes;
+    p.tris = s->tris;
+    p.triIds = s->triIds;
+    p.nodeEntry = s->nodeEntry;
+    p.rays = dRays;
+    p.hits = mode == kAny ? nullptr : static_cast<rt_hit*>(dOut);
+    p.occluded = mode == kAny ? static_cast<uint8_t*>(dOut) : nullptr;
+    p.counts = dCounts;
+    p.spill = w->spill;
+    p.work = w->work;
+    p.n = n;
+    p.rootEntry = s->rootEntry;
+    p.spillCap = spillCap;
+    p.gang = s->G;
+    p.Lm = s->Lm;
+    p.nodeCount = s->nodeCount;
+    HIP_TRY(launch_reset(w->work, st), "reset kernel");
+    HIP_TRY(hipEventRecord(w->ev0, st), "hipEventRecord");
+    HIP_TRY(launch_trace(s->M, mode, S, bd, grid, p, st), "trace kernel launch");
+    HIP_TRY(hipEventRecord(w->ev1, st), "hipEventRecord");
+    w->timed = true;
+    s->raysTraced += n;
+    if (!dev) {
+        HIP_TRY(hipMemcpyAsync(out, dOut, outBytes, hipMemcpyDeviceToHost, st), "hipMemcpy(out)");
+        if (cntBytes) HIP_TRY(hipMemcpyAsync(counts, dCounts, cntBytes, hipMemcpyDeviceToHost, st), "hipMemcpy(counts)");
+    }
+    if (!dev || !(flags & RT_NO_SYNC)) {
+        HIP_TRY(hipStreamSynchronize(st), "trace kernel");
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, w->ev0, w->ev1) == hipSuccess) s->lastMs = ms;
+        uint32_t errs = 0;
+        HIP_TRY(hipMemcpy(&errs, w->work + 1, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
+        s->errorRays += errs;
+    }
+    return RT_OK;
+}
+
+extern "C" int rt_stats(rt_scene_t s, rt_stats_t* o) {
+    if (!s || !o) return set_error(RT_ERR_ARG, "rt_stats: null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto& kv : s->ws) {
+        Workspace& w = kv.second;
+        if (w.timed && hipEventQuery(w.ev1) == hipSuccess) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, w.ev0, w.ev1) == hipSuccess) s->lastMs = ms;
+        }
+    }
+    std::memset(o, 0, sizeof(*o));
+    uint64_t spill = 0;
+    for (auto& kv : s->ws) spill += kv.second.spillBytes + kv.second.stageBytes;
+    o->bytes_resident = s->bytes + spill;
+    o->inner_nodes = s->inner;
+    o->leaves = s->leaves;
+    o->node_record_bytes = record_words(s->M) * 4;
+    o->leaf_block_bytes = leaf_stride(s->Lm) * 4;
+    o->last_kernel_ms = s->lastMs;
+    o->rays_traced = s->raysTraced;
+    o->block_size = s->blockSize;
+    o->stack_window = s->stackWindow;
+    o->max_stack = stack_capacity(s);
+    o->error_rays = s->errorRays;
+    return RT_OK;
+}
+
+extern "C" int rt_release(rt_scene_t s) {
+    if (!s) return RT_OK;
+    free_scene(s);
+    return RT_OK;
+}

@@ -1,0 +1,56 @@
+// Traversal kernels: per-ray wide-BVH traversal on CDNA4, one ray per lane,
+// persistent waves refilling lanes from a global ray counter.
+//
+// Semantics follow the reference traversals exactly (bit-exact hit identity
+// and node/leaf counts), with the FastNode arrays already converted to the
+// device layout of rt_layout.h:
+//   closest hit  FastNodeManager::intersect            fastNodeManager.h:1328-1418
+//   save dist    FastNodeManager::intersectSaveDistance fastNodeManager.h:1232-1326
+//   any hit      FastNodeManager::intersectSecondary    fastNodeManager.h:1420-1493
+// with the ISPC kernels inlined:
+//   slab test    aabbIntersect{k}   rayTracerTemplate.ispc:170-234
+//   closest MT   triIntersect{k}    rayTracerTemplate.ispc:25-108
+//   any-hit MT   triAnyHit{k}       rayTracerTemplate.ispc:111-168
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/rt.h"
+#include "rt_layout.h"
+
+namespace rt {
+
+enum TraceMode { kClosest = 0, kAny = 1, kSaveDist = 2 };
+
+struct KParams {
+    const uint32_t* __restrict__ nodes;  // inner records
+    const float* __restrict__ tris;      // leaf blocks
+    const int32_t* __restrict__ triIds;  // leaf ordinal * Lm + slot -> triangle id
+    const uint32_t* __restrict__ nodeEntry;  // FastNode id -> stack entry (rare path)
+    const rt_ray* __restrict__ rays;
+    rt_hit* hits;
+    uint8_t* occluded;
+    rt_counts* counts;
+    uint32_t* spill;       // per thread slot: spillCap entries (+ spillCap floats)
+    uint32_t* work;        // [0] ray counter, [1] guarded-condition counter
+    uint32_t n;
+    uint32_t rootEntry;
+    uint32_t spillCap;
+    uint32_t gang;
+    uint32_t Lm;
+    uint32_t nodeCount;
+};
+
+// Launches the per-ray kernel for node memory M on `stream`.
+hipError_t launch_trace(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize,
+                        uint32_t grid, const KParams& p, hipStream_t stream);
+
+// Blocks per CU the per-ray kernel reaches with this configuration.
+int trace_occupancy(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize);
+
+// Dynamic LDS bytes of the per-ray kernel (stack window, and for the
+// save-distance mode the stack distances and aabbDistances).
+size_t trace_lds_bytes(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize);
+
+hipError_t launch_reset(uint32_t* work, hipStream_t stream);
+
+}  // namespace rt

@@ -1,0 +1,50 @@
+// Device layout of an uploaded scene (shared by the upload code and the
+// kernels).
+//
+// Inner node record (one per FastNode with children), RECW(M) 32-bit words,
+// 128/256/384/512 bytes for M = 4/8/12/16, 64-byte aligned:
+//   [0, 6M)        child boxes, SoA planes of M floats:
+//                  minX minY minZ maxX maxY maxZ (slot s of plane p at p*M+s).
+//                  Slots [childCount, k) are the reference's zero boxes,
+//                  slots >= k are never read.
+//   [6M, 6M+HW)    header block:
+//                  w0 innerBase   record index of the first inner child
+//                  w1 leafBase    leaf ordinal of the first leaf child
+//                  w2 leafMask (bits 0-15) | childCount << 16 | k << 24
+//                  w3 kcodes      2 bits per child slot: (k_child / 4) - 1
+//                  w4 childBase   FastNode id of child slot 0 (childIdBegin)
+//                  w5..           traverseOrderEachAxis, 4-bit slots,
+//                                 OW = ceil(M/8) words per axis
+// Children of one FastNode are contiguous (customTreeOrder), so the inner
+// children get consecutive record indices and the leaf children consecutive
+// leaf ordinals: the stack entry of child slot s is computed from
+// innerBase/leafBase and a popcount of leafMask, without a table lookup.
+//
+// Leaf block (one per leaf, in FastNode order == trianglePoints order):
+//   LSTRIDE(Lm) floats (9*Lm rounded up to 16 floats = 64 bytes) holding the
+//   reference's 9*Lm-float block unchanged: planes p0.x p0.y p0.z p1.x ...
+//   p2.z at stride k = leafRealSize.
+//
+// Stack entry (32 bits):
+//   bit 31        leaf
+//   bit 30        invalid (FastNode id outside the node array)
+//   bits 28-29    kcode: loop count k = (kcode + 1) * 4 of that node/leaf
+//   bits 0-27     inner record index / leaf ordinal
+#pragma once
+#include <cstdint>
+
+namespace rt {
+
+constexpr uint32_t kEntryLeaf = 0x80000000u;
+constexpr uint32_t kEntryInvalid = 0x40000000u;
+constexpr uint32_t kEntryIndexMask = 0x0FFFFFFFu;
+constexpr int kEntryKShift = 28;
+
+__host__ __device__ constexpr uint32_t order_words(uint32_t M) { return (M + 7) / 8; }
+__host__ __device__ constexpr uint32_t header_words(uint32_t M) { return 5 + 3 * order_words(M); }
+__host__ __device__ constexpr uint32_t record_words(uint32_t M) {
+    return ((6 * M + header_words(M)) + 15) / 16 * 16;
+}
+__host__ __device__ constexpr uint32_t leaf_stride(uint32_t Lm) { return (9 * Lm + 15) / 16 * 16; }
+
+}  // namespace rt

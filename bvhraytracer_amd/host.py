@@ -1,0 +1,245 @@
+"""Host-side mirror of the reference's interface for the traversal path.
+
+Reference counterparts:
+  Bvh(prims, N, L, sortEachSplit, leafSplitOption) + recursiveBvh(sahFactor)
+      src/accelerationStructure/bvh.h:28-54, bvh.cpp:28-40
+  FastNodeManager<G, M, WG>(bvh, leafMemory, cache)   (the flattened arrays)
+      src/accelerationStructure/fastNodeManager.h:158-215, 1495-1673
+  FastNodeManager::intersect / intersectSaveDistance / intersectSecondary
+      fastNodeManager.h:1232-1493  -> Scene.intersect(..., mode=...)
+  intersectWideAlternative / intersectSecondaryWideAlternative
+      fastNodeManager.h:835-1230   -> Scene.intersect(..., stream=True)
+  getSurfaceNormalPosition      fastNodeManager.h:1699-1726 -> FastNodeManager.surface()
+  Camera::getRayTargetPosition  src/camera.h:91-98 -> camera_rays()
+  RayTracer::loadSahFactor      src/rayTracer.cpp:880-925 -> load_sah_factor()
+
+The Scene methods run on the GPU through librt.so; nothing here computes a
+traversal result on the CPU.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import (COUNTS_DTYPE, HIT_DTYPE, RAY_DTYPE, RtBuildParams, RtBvhDesc, RtCamera,
+                   RtStats, check, lib, ptr)
+
+
+def roundup(x, g):
+    return int(math.ceil(x / float(g)) * g)
+
+
+def load_sah_factor(leaf_size, branching, gang=4, table_path=None):
+    """RayTracer::loadSahFactor (rayTracer.cpp:880-925)."""
+    p = table_path.encode() if table_path else None
+    return float(lib().rt_sah_factor(int(leaf_size), int(branching), int(gang), p))
+
+
+def generate_scene(kind, seed, count, param=0):
+    """Procedural triangles as float32 (n, 9): p0, p1, p2."""
+    L = lib()
+    out = ctypes.POINTER(ctypes.c_float)()
+    n = ctypes.c_uint32()
+    k = {"spheres": _lib.RT_SCENE_SPHERES, "interior": _lib.RT_SCENE_INTERIOR}.get(kind, kind)
+    check(L.rt_scene_generate(int(k), int(seed), int(count), int(param), ctypes.byref(out),
+                              ctypes.byref(n)))
+    try:
+        arr = np.ctypeslib.as_array(out, shape=(n.value * 9,)).copy().reshape(n.value, 9)
+    finally:
+        L.rt_free(ctypes.cast(out, ctypes.c_void_p))
+    return arr
+
+
+def spheres_scene(seed=0, spheres=50, frequency=10):
+    """100,002 triangles for the defaults: 50 frequency-10 icospheres + ground."""
+    return generate_scene("spheres", seed, spheres, frequency)
+
+
+def interior_scene(seed, triangles=1_000_000, clusters=0):
+    return generate_scene("interior", seed, triangles, clusters)
+
+
+class Bvh:
+    """Bvh + recursiveBvh + FastNodeManager flattening, done by librt's host code.
+
+    Holds the reference FastNode arrays (numpy views) that both the device
+    upload and the oracle consume.
+    """
+
+    def __init__(self, tris, branching, leaf_size, gang=4, sort_each_split=True,
+                 leaf_split_option=1, sah_factor=None, threads=0):
+        tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+        if sah_factor is None:
+            sah_factor = load_sah_factor(leaf_size, branching, gang)
+        self.branching, self.leaf_size, self.gang = int(branching), int(leaf_size), int(gang)
+        self.sah_factor = float(sah_factor)
+        self.tris = tris
+        p = RtBuildParams(self.branching, self.leaf_size, self.gang, int(bool(sort_each_split)),
+                          int(leaf_split_option), ctypes.c_float(self.sah_factor).value, int(threads))
+        h = ctypes.c_void_p()
+        check(lib().rt_build(ptr(tris), tris.shape[0], ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+        d = RtBvhDesc()
+        check(lib().rt_host_bvh_desc(h, ctypes.byref(d)))
+        self.desc = d
+        M, Lm, n = d.node_memory, d.leaf_memory, d.node_count
+        self.node_memory, self.leaf_memory, self.node_count = M, Lm, n
+
+        def view(addr, ctype, count, dtype):
+            if count == 0:
+                return np.zeros(0, dtype=dtype)
+            buf = (ctype * count).from_address(addr)
+            return np.frombuffer(buf, dtype=dtype, count=count)
+
+        self.bounds = view(d.bounds, ctypes.c_float, n * 6 * M, np.float32).reshape(n, 6 * M)
+        self.base = view(d.base, ctypes.c_uint32, n, np.uint32)
+        self.order = view(d.order, ctypes.c_int8, n * 3 * M, np.int8).reshape(n, 3, M)
+        self.child_type = view(d.child_type, ctypes.c_uint16, n, np.uint16)
+        self.tri_points = view(d.tri_points, ctypes.c_float, d.tri_float_count, np.float32)
+        nl = d.tri_float_count // (9 * Lm)
+        self.tri_ids = view(d.tri_ids, ctypes.c_int32, nl * Lm, np.int32)
+        self.average_depth = d.average_depth
+        self.triangle_count = d.triangle_count
+        self.max_depth = d.max_depth
+
+    def is_leaf(self):
+        return np.isnan(self.bounds[:, 0])
+
+    def surface(self, rays, hits):
+        """getSurfaceNormalPosition for every hit: (normal (n,3), position (n,3))."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)
+        n = len(rays)
+        nrm = np.empty((n, 3), np.float32)
+        pos = np.empty((n, 3), np.float32)
+        check(lib().rt_hit_surface(self._h, ptr(rays), ptr(hits), n, ptr(nrm), ptr(pos)))
+        return nrm, pos
+
+    def footprint(self):
+        b = ctypes.c_uint64()
+        check(lib().rt_scene_footprint(ctypes.byref(self.desc), ctypes.byref(b)))
+        return b.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rt_host_bvh_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+MODES = {"closest": _lib.RT_CLOSEST, "any": _lib.RT_ANY, "save_distance": _lib.RT_SAVE_DIST}
+
+
+class Scene:
+    """A BVH resident in HBM (rt_upload_bvh) with the traversal entry points."""
+
+    def __init__(self, bvh):
+        self.bvh = bvh
+        h = ctypes.c_void_p()
+        check(lib().rt_upload_bvh(ctypes.byref(bvh.desc), ctypes.byref(h)))
+        self._h = h
+
+    def set_launch(self, block_size=0, stack_window=0):
+        check(lib().rt_set_launch(self._h, int(block_size), int(stack_window)))
+
+    def intersect(self, rays, mode="closest", counts=False, stream=False, coop=False):
+        """Host-array form: rays (n,) RAY_DTYPE -> hits (n,) HIT_DTYPE, or
+        occluded (n,) uint8 for mode='any'; plus counts when asked."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        n = len(rays)
+        flags = MODES[mode] | (_lib.RT_STREAM if stream else 0) | (_lib.RT_COOP if coop else 0)
+        out = np.empty(n, np.uint8) if mode == "any" else np.empty(n, HIT_DTYPE)
+        cnt = np.zeros(n, COUNTS_DTYPE) if counts else None
+        if counts:
+            flags |= _lib.RT_COUNTS
+        check(lib().rt_trace(self._h, ptr(rays), n, flags, ptr(out), ptr(cnt), None))
+        return (out, cnt) if counts else out
+
+    def intersect_device(self, rays, out, mode="closest", counts=None, stream=None, sync=True,
+                         stream_sched=False, coop=False):
+        """Device-buffer form: rays/out/counts are torch CUDA tensors (or
+        device addresses); stream is a torch.cuda.Stream or None."""
+        n = rays.shape[0] if hasattr(rays, "shape") else int(rays[1])
+        flags = MODES[mode] | _lib.RT_DEVICE_PTRS
+        if counts is not None:
+            flags |= _lib.RT_COUNTS
+        if not sync:
+            flags |= _lib.RT_NO_SYNC
+        if stream_sched:
+            flags |= _lib.RT_STREAM
+        if coop:
+            flags |= _lib.RT_COOP
+        s = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        check(lib().rt_trace(self._h, ptr(rays), n, flags, ptr(out),
+                             ptr(counts) if counts is not None else None, s))
+
+    def stats(self):
+        s = RtStats()
+        check(lib().rt_stats(self._h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in RtStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rt_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def camera_rays(pos, target, width, height, wg=16, up=(0.0, 1.0, 0.0), focal=0.866):
+    """Primary rays in CameraFast tile order (cameraFast.h:318-342)."""
+    cam = RtCamera((ctypes.c_float * 3)(*pos), (ctypes.c_float * 3)(*target),
+                   (ctypes.c_float * 3)(*up), float(focal))
+    out = np.empty(width * height, RAY_DTYPE)
+    check(lib().rt_camera_rays(ctypes.byref(cam), width, height, wg, ptr(out)))
+    return out
+
+
+def spawn_shadow(normal, position, hits, light, tmax=1.0):
+    hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)
+    n = len(hits)
+    out = np.empty(n, RAY_DTYPE)
+    src = np.empty(n, np.uint32)
+    k = ctypes.c_uint32()
+    lt = (ctypes.c_float * 3)(*light)
+    check(lib().rt_spawn_shadow(ptr(np.ascontiguousarray(normal, np.float32)),
+                                ptr(np.ascontiguousarray(position, np.float32)), ptr(hits), n,
+                                lt, float(tmax), ptr(out), ptr(src), ctypes.byref(k)))
+    return out[:k.value].copy(), src[:k.value].copy()
+
+
+def spawn_diffuse(primary, normal, position, hits, count, seed=3, tmax=222222.0):
+    primary = np.ascontiguousarray(primary, dtype=RAY_DTYPE)
+    hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)
+    out = np.empty(count, RAY_DTYPE)
+    check(lib().rt_spawn_diffuse(ptr(primary), ptr(np.ascontiguousarray(normal, np.float32)),
+                                 ptr(np.ascontiguousarray(position, np.float32)), ptr(hits),
+                                 len(hits), int(seed), int(count), float(tmax), ptr(out)))
+    return out
+
+
+def make_rays(origins, targets, tmax=222222.0):
+    """FastRay(pos, target, maxDistance) (fastRay.cpp:3-20) for arrays of
+    points; direction = target - pos, inverse with FLT_MAX for 0."""
+    o = np.asarray(origins, np.float32).reshape(-1, 3)
+    t = np.asarray(targets, np.float32).reshape(-1, 3)
+    d = (t - o).astype(np.float32)
+    with np.errstate(divide="ignore"):
+        inv = np.where(d == 0, np.float32(np.finfo(np.float32).max),
+                       np.float32(1.0) / d).astype(np.float32)
+    r = np.empty(len(o), RAY_DTYPE)
+    r["tmax"] = np.float32(tmax)
+    r["ox"], r["oy"], r["oz"] = o[:, 0], o[:, 1], o[:, 2]
+    r["ix"], r["iy"], r["iz"] = inv[:, 0], inv[:, 1], inv[:, 2]
+    r["dx"], r["dy"], r["dz"] = d[:, 0], d[:, 1], d[:, 2]
+    return r

@@ -1,0 +1,225 @@
+/*
+ * rt.h — C ABI of the MI355X wide-BVH ray-query engine (librt.so).
+ *
+ * The reference (Hengoo/BVHRaytracer) crosses a C ABI once per node visit and
+ * once per leaf visit: the ISPC exports declared in the generated header
+ * src/ISPC/ISPCBuild/rayTracer_ISPC.h (included at
+ * src/accelerationStructure/fastNodeManager.h:256), i.e.
+ *   bool    aabbIntersect{4,8,12,16}(const float* bounds, float* dist, float* rayInfo)
+ *   int32_t triIntersect{4,8,12,16}(const float* tri, uint32_t triId, float* rayInfo)
+ *   bool    triAnyHit{4,8,12,16}(const float* tri, uint32_t triId, float* rayInfo)
+ * (src/ISPC/rayTracerTemplate.ispc:26,112,171), driven by the header-only
+ * FastNodeManager traversals (fastNodeManager.h:1232-1493, 835-1230).
+ * This library moves that seam to one call per ray batch: the caller hands
+ * over the flattened FastNode arrays once (rt_upload_bvh) and then whole ray
+ * buffers (rt_trace).  All functions return RT_OK (0) or a negative RT_ERR_*
+ * code; rt_last_error() returns the message of the calling thread's last
+ * failure.  No C++ exception crosses this boundary.
+ *
+ * Host-side entry points (rt_scene_*, rt_build*, rt_camera_*, rt_spawn_*,
+ * rt_config_*) restate the reference's host code that produces the path's
+ * inputs: the SAH builder, the FastNode flattener, the camera and the
+ * config.txt / sahFactorConfigSse.txt readers.
+ */
+#ifndef BVHRT_RT_H
+#define BVHRT_RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define RT_OK 0
+#define RT_ERR_ARG (-1)         /* bad argument / inconsistent sizes          */
+#define RT_ERR_HIP (-2)         /* HIP runtime call failed                    */
+#define RT_ERR_NOMEM (-3)       /* device or host allocation failed           */
+#define RT_ERR_UNSUPPORTED (-4) /* configuration outside the supported set    */
+#define RT_ERR_LAYOUT (-5)      /* FastNode arrays violate the layout contract */
+#define RT_ERR_BUSY (-6)        /* serving queue full / admission refused     */
+
+/* ---- ray / hit records ---------------------------------------------------- */
+
+/* == FastRay (src/fastRay.h:18-23): tMax, pos, invDirection, direction.
+ * The ISPC kernels read exactly this memory as rayInfo[0..9]
+ * (rayTracerTemplate.ispc:46-47,190-192).  dir is NOT normalised. */
+typedef struct rt_ray {
+    float tmax;
+    float ox, oy, oz;
+    float ix, iy, iz;
+    float dx, dy, dz;
+} rt_ray;
+
+/* Closest-hit result.  leaf/slot are the reference's hit identity
+ * (leafIndex = FastNode::primIdBegin, triIndex = in-leaf index,
+ * fastNodeManager.h:1369-1374); tri maps it back to the input triangle. */
+typedef struct rt_hit {
+    int32_t tri;   /* input triangle id, -1 = miss                          */
+    uint32_t leaf; /* primIdBegin of the hit leaf (float offset), 0 on miss  */
+    int32_t slot;  /* in-leaf index, -1 = miss                               */
+    float t;       /* final ray.tMax (the closest t)                         */
+    float b1, b2;  /* barycentrics of the hit triangle, same FP32 arithmetic */
+} rt_hit;
+
+/* Per-ray work counters.  nodes = number of aabbIntersect calls (inner node
+ * pops), leaves = number of triIntersect / triAnyHit calls (leaf pops), the
+ * count definitions of compactNode.cpp:651-717 applied to the FastNode
+ * traversal.  node_slots / leaf_slots = sum of the loop counts k passed to
+ * those calls (used for the bytes-per-ray roofline). */
+typedef struct rt_counts {
+    uint32_t nodes, leaves;
+    uint32_t node_slots, leaf_slots;
+} rt_counts;
+
+/* ---- the flattened BVH (reference FastNode layout) ------------------------ */
+
+/* Borrowed views of FastNodeManager's arrays (fastNodeManager.h:29-90,160-164,
+ * built at :1495-1673): per node i
+ *   bounds[i*6*M .. +6M)      SoA child boxes, plane stride nodeRealSize,
+ *                             NaN marker at [6M-1] / count at [6M-2];
+ *                             bounds[i*6*M] is NaN for a leaf
+ *   base[i]                   childIdBegin (inner) / primIdBegin (leaf)
+ *   order[i*3*M + axis*M + p] traverseOrderEachAxis, zero padded to M
+ *   child_type[i]             childType bitset (bit j = child j is inner)
+ * and tri_points = trianglePoints (blocks of 9*leaf_memory floats per leaf,
+ * plane stride leafRealSize, NaN marker at [9Lm-1]).  tri_ids maps
+ * (leaf ordinal * leaf_memory + slot) to the input triangle id, -1 = pad. */
+typedef struct rt_bvh_desc {
+    uint32_t branching;   /* N  */
+    uint32_t leaf_size;   /* L  */
+    uint32_t gang;        /* G  (4 = sse4-i32x4, 8 = avx2-i32x8, 16)          */
+    uint32_t node_memory; /* M  = roundup(N, G) */
+    uint32_t leaf_memory; /* Lm = roundup(L, G) */
+    uint32_t node_count;
+    const float* bounds;
+    const uint32_t* base;
+    const int8_t* order;
+    const uint16_t* child_type;
+    uint64_t tri_float_count;
+    const float* tri_points;
+    const int32_t* tri_ids;
+    /* informational (FastNodeManager::averageBvhDepth / triangleCount) */
+    float average_depth;
+    uint32_t triangle_count;
+    uint32_t max_depth;
+} rt_bvh_desc;
+
+typedef struct rt_scene* rt_scene_t;
+
+/* ---- trace flags ------------------------------------------------------------ */
+#define RT_CLOSEST 0u      /* FastNodeManager::intersect            (:1328-1418) */
+#define RT_ANY 1u          /* FastNodeManager::intersectSecondary   (:1420-1493) */
+#define RT_SAVE_DIST 2u    /* FastNodeManager::intersectSaveDistance(:1232-1326) */
+#define RT_STREAM 4u       /* ray-stream scheduling (intersectWideAlternative
+                              :835-1049), per-ray results identical to the
+                              per-ray traversal                                   */
+#define RT_COUNTS 8u       /* write rt_counts per ray                             */
+#define RT_DEVICE_PTRS 16u /* rays/out/counts are device pointers                 */
+#define RT_NO_SYNC 32u     /* with RT_DEVICE_PTRS: return without waiting         */
+#define RT_COOP 64u        /* wavefront-cooperative child tests (lanes = slots)   */
+
+/* ---- scene lifetime ----------------------------------------------------------- */
+
+/* Converts the FastNode arrays into the device layout and copies them into
+ * HBM on the current device.  The desc arrays are only read during the call. */
+int rt_upload_bvh(const rt_bvh_desc* desc, rt_scene_t* out);
+
+/* Traces n rays.  out is rt_hit[n] for RT_CLOSEST / RT_SAVE_DIST and
+ * uint8_t[n] (1 = occluded) for RT_ANY.  counts may be NULL unless RT_COUNTS.
+ * hip_stream is a hipStream_t (NULL = the library's stream).  With
+ * RT_DEVICE_PTRS the buffers stay on the device; otherwise they are staged. */
+int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags,
+             void* out, rt_counts* counts, void* hip_stream);
+
+typedef struct rt_stats_t {
+    uint64_t bytes_resident;  /* device bytes owned by the scene               */
+    uint32_t inner_nodes, leaves;
+    uint32_t node_record_bytes, leaf_block_bytes;
+    float last_kernel_ms;     /* duration of the last trace kernel (HIP events) */
+    uint64_t rays_traced;     /* rays traced through this scene so far          */
+    uint32_t block_size;      /* launch configuration chosen for the scene     */
+    uint32_t stack_window;    /* LDS stack entries per lane                    */
+    uint32_t max_stack;       /* stack capacity per ray (LDS + spill)          */
+    uint32_t error_rays;      /* rays that hit a guarded condition (see DESIGN) */
+} rt_stats_t;
+
+int rt_stats(rt_scene_t s, rt_stats_t* out);
+
+/* Overrides the launch configuration (block size 64..1024 in multiples of 64,
+ * LDS stack window 8/16/32 entries).  0 keeps the current value. */
+int rt_set_launch(rt_scene_t s, uint32_t block_size, uint32_t stack_window);
+
+int rt_release(rt_scene_t s);
+
+const char* rt_last_error(void);
+
+/* Number of device bytes a scene built from desc would occupy (admission). */
+int rt_scene_footprint(const rt_bvh_desc* desc, uint64_t* bytes);
+
+/* ---- host side: scenes, builder, camera, config ------------------------------- */
+
+/* Procedural scenes (PCG32 seeded).  kind 1: spheres + ground plane
+ * (icospheres of frequency `param`, `count` of them); kind 2: clustered
+ * "interior" room (count = total triangles, param = clusters, 0 = auto).
+ * Triangles are returned as float[n*9] (p0,p1,p2) owned by the library
+ * (free with rt_free). */
+#define RT_SCENE_SPHERES 1u
+#define RT_SCENE_INTERIOR 2u
+int rt_scene_generate(uint32_t kind, uint64_t seed, uint32_t count, uint32_t param,
+                      float** tris, uint32_t* n_tris);
+void rt_free(void* p);
+
+/* Bvh(prims, N, L, sortEachSplit, leafSplitOption) + recursiveBvh(sahFactor)
+ * (bvh.cpp:28-40, aabb.cpp:71-206) followed by the FastNodeManager flattener
+ * (fastNodeManager.h:1495-1688), with primitives pre-sorted as
+ * RayTracer::preparePrimitives does (rayTracer.cpp:747-754). */
+typedef struct rt_build_params {
+    uint32_t branching, leaf_size, gang;
+    uint32_t sort_each_split;   /* config.txt sortEachSplit (1)   */
+    int32_t leaf_split_option;  /* config.txt leafSplitOption (1) */
+    float sah_factor;           /* loadSahFactor(L, N, G)         */
+    uint32_t threads;           /* 0 = all cores                  */
+} rt_build_params;
+
+typedef struct rt_host_bvh* rt_host_bvh_t;
+int rt_build(const float* tris, uint32_t n_tris, const rt_build_params* p, rt_host_bvh_t* out);
+/* Fills desc with views into h (valid until rt_host_bvh_free). */
+int rt_host_bvh_desc(rt_host_bvh_t h, rt_bvh_desc* desc);
+int rt_host_bvh_free(rt_host_bvh_t h);
+
+/* Pinhole camera (camera.h:52-58,91-98): rays in CameraFast tile order
+ * (tile i, lane j -> ray i*wg*wg + j, cameraFast.h:324-342); rows of a
+ * partial last tile row are skipped.  Writes width*height rays. */
+typedef struct rt_camera {
+    float pos[3], target[3], up[3];
+    float focal; /* 0.866 = 60 degree horizontal FOV */
+} rt_camera;
+int rt_camera_rays(const rt_camera* cam, uint32_t width, uint32_t height, uint32_t wg,
+                   rt_ray* out);
+
+/* Hit reconstruction (getSurfaceNormalPosition, fastNodeManager.h:1699-1726):
+ * for each hit writes normal[3] and position[3]; misses get NaN. */
+int rt_hit_surface(rt_host_bvh_t h, const rt_ray* rays, const rt_hit* hits, uint32_t n,
+                   float* normal_out, float* position_out);
+
+/* Shadow rays toward a point light: FastRay(P + n*0.001, light, tmax).
+ * Returns the number of rays written (one per hit); src_index[i] = ray id. */
+int rt_spawn_shadow(const float* normal, const float* position, const rt_hit* hits,
+                    uint32_t n, const float light[3], float tmax, rt_ray* out,
+                    uint32_t* src_index, uint32_t* n_out);
+
+/* n_out cosine-distributed bounce rays cycling over the hits (PCG32 stream
+ * per ray index); the normal is face-forwarded against the primary ray. */
+int rt_spawn_diffuse(const rt_ray* primary, const float* normal, const float* position,
+                     const rt_hit* hits, uint32_t n, uint64_t seed, uint32_t n_out,
+                     float tmax, rt_ray* out);
+
+/* sahFactorConfigSse.txt lookup (RayTracer::loadSahFactor, rayTracer.cpp:880-925). */
+float rt_sah_factor(uint32_t leaf_size, uint32_t branching, uint32_t gang, const char* table_path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BVHRT_RT_H */

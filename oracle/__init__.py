@@ -1,0 +1,152 @@
+"""ORACLE — test infrastructure, not product code.
+
+ctypes binding of liboracle.so (cpu_ref.c), the CPU restatement of the
+reference traversal.  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use it, and only as the checker or the timed CPU baseline.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+ORC_CLOSEST, ORC_ANY, ORC_SAVE_DIST = 0, 1, 2
+ORC_WIDE_AXIS_BUG, ORC_WIDE_SHARED_DIST, ORC_WIDE_ROOT_AS_NODE = 1, 2, 4
+DIAG_STACK48, DIAG_BOGUS, DIAG_STACKCAP = 1, 2, 4
+
+RAY_DTYPE = np.dtype([("tmax", "<f4"), ("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"),
+                      ("ix", "<f4"), ("iy", "<f4"), ("iz", "<f4"),
+                      ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4")])
+HIT_DTYPE = np.dtype([("slot", "<i4"), ("leaf", "<u4"), ("t", "<f4"), ("b1", "<f4"), ("b2", "<f4")])
+COUNTS_DTYPE = np.dtype([("nodes", "<u4"), ("leaves", "<u4"),
+                         ("node_slots", "<u4"), ("leaf_slots", "<u4")])
+
+
+class OrcBvh(ctypes.Structure):
+    _fields_ = [("M", ctypes.c_uint32), ("Lm", ctypes.c_uint32), ("G", ctypes.c_uint32),
+                ("node_count", ctypes.c_uint32), ("bounds", ctypes.c_void_p),
+                ("base", ctypes.c_void_p), ("order", ctypes.c_void_p),
+                ("child_type", ctypes.c_void_p), ("tri_float_count", ctypes.c_uint64),
+                ("tri", ctypes.c_void_p)]
+
+
+_P = ctypes.c_void_p
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run make -C oracle")
+        L = ctypes.CDLL(LIB_PATH)
+        sig = {
+            "orc_aabb_intersect": (ctypes.c_int, [_P, ctypes.c_int, _P, _P]),
+            "orc_tri_intersect": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int, _P, ctypes.c_int]),
+            "orc_tri_anyhit": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int, _P]),
+            "orc_node_loop_count": (ctypes.c_int, [_P, ctypes.c_uint32]),
+            "orc_leaf_loop_count": (ctypes.c_int, [_P, ctypes.c_uint32]),
+            "orc_dispatch_k": (ctypes.c_int, [ctypes.c_int]),
+            "orc_intersect": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+            "orc_intersect_save_distance": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+            "orc_intersect_secondary": (ctypes.c_int, [_P, _P, _P, _P]),
+            "orc_intersect_wide": (None, [_P, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P]),
+            "orc_intersect_secondary_wide": (None, [_P, _P, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P]),
+            "orc_trace": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_uint32, ctypes.c_int, _P, _P, _P, _P]),
+            "orc_mt_f64": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+            "orc_bruteforce_closest": (ctypes.c_int32, [_P, ctypes.c_uint32, _P, _P]),
+            "orc_bruteforce_any": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class Oracle:
+    """Oracle view of a set of FastNode arrays (keeps the arrays alive)."""
+
+    def __init__(self, M, Lm, G, bounds, base, order, child_type, tri_points):
+        self.arrays = [np.ascontiguousarray(bounds, np.float32), np.ascontiguousarray(base, np.uint32),
+                       np.ascontiguousarray(order, np.int8), np.ascontiguousarray(child_type, np.uint16),
+                       np.ascontiguousarray(tri_points, np.float32)]
+        b, ba, o, ct, t = self.arrays
+        self.M, self.Lm, self.G = int(M), int(Lm), int(G)
+        self.node_count = len(ba)
+        self.s = OrcBvh(self.M, self.Lm, self.G, self.node_count, _p(b), _p(ba), _p(o), _p(ct),
+                        t.size, _p(t))
+
+    @classmethod
+    def from_bvh(cls, bvh):
+        return cls(bvh.node_memory, bvh.leaf_memory, bvh.gang, bvh.bounds, bvh.base, bvh.order,
+                   bvh.child_type, bvh.tri_points)
+
+    def trace(self, rays, mode=ORC_CLOSEST, stream=False, tile=256, threads=0, counts=True):
+        rays = np.ascontiguousarray(rays).view(RAY_DTYPE)
+        n = len(rays)
+        hits = np.zeros(n, HIT_DTYPE) if mode != ORC_ANY else None
+        occ = np.zeros(n, np.uint8) if mode == ORC_ANY else None
+        cnt = np.zeros(n, COUNTS_DTYPE) if counts else None
+        diag = np.zeros(n, np.uint32)
+        used = lib().orc_trace(ctypes.byref(self.s), _p(rays), n, mode, int(stream), tile, threads,
+                               _p(hits), _p(occ), _p(cnt), _p(diag))
+        return {"hits": hits, "occluded": occ, "counts": cnt, "diag": diag, "threads": used}
+
+    def wide(self, rays, flags=0):
+        rays = np.array(rays, copy=True).view(RAY_DTYPE)
+        n = len(rays)
+        li = np.zeros(n, np.uint32)
+        ti = np.zeros(n, np.int32)
+        cnt = np.zeros(n, COUNTS_DTYPE)
+        diag = np.zeros(n, np.uint32)
+        lib().orc_intersect_wide(ctypes.byref(self.s), _p(rays), n, flags, _p(li), _p(ti), _p(cnt), _p(diag))
+        return li, ti, cnt, rays["tmax"].copy()
+
+    def secondary_wide(self, rays, flags=0):
+        rays = np.array(rays, copy=True).view(RAY_DTYPE)
+        n = len(rays)
+        res = np.zeros(n, np.uint16)
+        cnt = np.zeros(n, COUNTS_DTYPE)
+        diag = np.zeros(n, np.uint32)
+        lib().orc_intersect_secondary_wide(ctypes.byref(self.s), _p(rays), n, flags, _p(res), _p(cnt), _p(diag))
+        return res, cnt
+
+
+def aabb_intersect(bounds, k, dist, ray):
+    return lib().orc_aabb_intersect(_p(bounds), k, _p(dist), _p(ray))
+
+
+def tri_intersect(tri, tri_id, k, ray, gang=4):
+    return lib().orc_tri_intersect(_p(tri), tri_id, k, _p(ray), gang)
+
+
+def tri_anyhit(tri, tri_id, k, ray):
+    return lib().orc_tri_anyhit(_p(tri), tri_id, k, _p(ray))
+
+
+def mt_f64(ray, tri9):
+    t, b1, b2 = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    ok = lib().orc_mt_f64(_p(ray), _p(np.ascontiguousarray(tri9, np.float32)), ctypes.byref(t),
+                          ctypes.byref(b1), ctypes.byref(b2))
+    return ok, t.value, b1.value, b2.value
+
+
+def bruteforce_closest(tris, ray):
+    tris = np.ascontiguousarray(tris, np.float32)
+    t = ctypes.c_float()
+    i = lib().orc_bruteforce_closest(_p(tris), tris.shape[0], _p(ray), ctypes.byref(t))
+    return i, t.value
+
+
+def bruteforce_any(tris, ray):
+    tris = np.ascontiguousarray(tris, np.float32)
+    return lib().orc_bruteforce_any(_p(tris), tris.shape[0], _p(ray))

@@ -1,0 +1,122 @@
+"""GPU parity: librt.so traversal vs the oracle on the same host-built BVH.
+
+Bar: hit id / leaf / slot / miss flag / node and leaf counts bit-exact; t,
+b1 and b2 are bit-exact too (same FP32 operation sequence), which is
+stricter than the 1e-5 relative tolerance the path allows.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import assert_closest_equal, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cfg1(rt):
+    tris = rt.spheres_scene(0)
+    bvh = rt.Bvh(tris, 4, 4)
+    rays = rt.camera_rays((0.0, 4.0, 32.0), (0.0, 0.0, 0.0), 256, 256, 16)
+    return tris, bvh, rays
+
+
+def test_config1_closest(rt, cfg1):
+    tris, bvh, rays = cfg1
+    print(f"config1 N4L4 closest: {len(rays)} rays", flush=True)
+    scene = rt.Scene(bvh)
+    hits, cnt = scene.intersect(rays, counts=True)
+    ref = oracle_for(bvh).trace(rays)
+    assert (ref["diag"] & oracle.DIAG_BOGUS).sum() == 0
+    assert_closest_equal(hits, ref["hits"], bvh, cnt, ref["counts"], "cfg1")
+    assert (hits["slot"] >= 0).sum() > 10000
+
+
+def test_config1_shadow_anyhit(rt, cfg1):
+    tris, bvh, rays = cfg1
+    scene = rt.Scene(bvh)
+    hits = scene.intersect(rays)
+    nrm, pos = bvh.surface(rays, hits)
+    srays, _ = rt.spawn_shadow(nrm, pos, hits, (0.0, 30.0, 0.0), 1.0)
+    print(f"config1 shadow: {len(srays)} rays", flush=True)
+    occ, cnt = scene.intersect(srays, mode="any", counts=True)
+    ref = oracle_for(bvh).trace(srays, mode=oracle.ORC_ANY)
+    assert np.array_equal(occ, ref["occluded"])
+    for f in ("nodes", "leaves", "node_slots", "leaf_slots"):
+        assert np.array_equal(cnt[f], ref["counts"][f]), f
+    assert 0 < occ.sum() < len(srays)
+
+
+def test_config1_save_distance(rt, cfg1):
+    tris, bvh, rays = cfg1
+    scene = rt.Scene(bvh)
+    hits, cnt = scene.intersect(rays, mode="save_distance", counts=True)
+    ref = oracle_for(bvh).trace(rays, mode=oracle.ORC_SAVE_DIST)
+    assert_closest_equal(hits, ref["hits"], bvh, cnt, ref["counts"], "cfg1 save-distance")
+
+
+@pytest.mark.parametrize("N", [4, 8, 12, 16])
+@pytest.mark.parametrize("L", [4, 8, 12, 16])
+def test_interior_sweep(rt, N, L):
+    tris = rt.interior_scene(7, 60_000, 30)
+    bvh = rt.Bvh(tris, N, L)
+    prim = rt.camera_rays((0.0, 5.0, -14.0), (0.0, 3.0, 0.0), 128, 96, 16)
+    scene = rt.Scene(bvh)
+    hits = scene.intersect(prim)
+    nrm, pos = bvh.surface(prim, hits)
+    diff = rt.spawn_diffuse(prim, nrm, pos, hits, 20000, seed=3)
+    print(f"interior N{N}L{L}: {len(prim)} primary + {len(diff)} diffuse", flush=True)
+    rays = np.concatenate([prim, diff])
+    g, gc = scene.intersect(rays, counts=True)
+    ref = oracle_for(bvh).trace(rays)
+    assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], f"N{N}L{L}")
+    srays, _ = rt.spawn_shadow(nrm, pos, hits, (0.0, 9.5, 0.0), 1.0)
+    occ, oc = scene.intersect(srays, mode="any", counts=True)
+    ro = oracle_for(bvh).trace(srays, mode=oracle.ORC_ANY)
+    assert np.array_equal(occ, ro["occluded"])
+    assert np.array_equal(oc["leaves"], ro["counts"]["leaves"])
+    assert np.array_equal(oc["nodes"], ro["counts"]["nodes"])
+
+
+@pytest.mark.parametrize("gang,N,L", [(8, 8, 8), (8, 16, 16), (16, 16, 16), (4, 5, 3)])
+def test_gang_sizes(rt, gang, N, L):
+    tris = rt.interior_scene(11, 30_000, 20)
+    bvh = rt.Bvh(tris, N, L, gang=gang)
+    rays = rt.camera_rays((0.0, 5.0, -14.0), (0.0, 3.0, 0.0), 128, 64, 16)
+    print(f"gang {gang} N{N}L{L}", flush=True)
+    g, gc = rt.Scene(bvh).intersect(rays, counts=True)
+    ref = oracle_for(bvh).trace(rays)
+    assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], f"G{gang}")
+
+
+def test_edge_cases(rt):
+    # a single triangle (root is a leaf), rays that miss, a ray through the
+    # origin (the zero-box padding quirk), t == tMax, zero direction parts
+    tri = np.array([[-1, -1, 5, 1, -1, 5, 0, 1, 5]], np.float32)
+    bvh = rt.Bvh(tri, 4, 4)
+    assert np.isnan(bvh.bounds[0, 0])
+    rays = rt.make_rays([[0, 0, 0], [0, 0, 0], [5, 5, 0], [0, 0, 0]],
+                        [[0, 0, 1], [0, 0, -1], [5, 5, 1], [0, 0, 5]], tmax=222222.0)
+    rays["tmax"][3] = 1.0  # hit exactly at t == tMax (inclusive)
+    print("edge cases: root leaf", flush=True)
+    scene = rt.Scene(bvh)
+    g, gc = scene.intersect(rays, counts=True)
+    ref = oracle_for(bvh).trace(rays)
+    assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], "root-leaf")
+    assert list(g["slot"]) == [0, -1, -1, 0]
+    # scene with triangles around the origin: rays from the origin hit the
+    # zero boxes of padded slots
+    tris = rt.interior_scene(3, 2000, 4)
+    tris = tris - np.tile(np.float32([0, 5, 0]), 3)[None, :]
+    bvh = rt.Bvh(tris, 6, 4)
+    dirs = np.random.default_rng(0).normal(size=(4096, 3)).astype(np.float32)
+    rays = rt.make_rays(np.zeros((4096, 3), np.float32), dirs)
+    scene = rt.Scene(bvh)
+    for mode, om in (("closest", oracle.ORC_CLOSEST), ("save_distance", oracle.ORC_SAVE_DIST)):
+        g, gc = scene.intersect(rays, mode=mode, counts=True)
+        ref = oracle_for(bvh).trace(rays, mode=om)
+        assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], f"origin rays {mode}")
+    occ, oc = scene.intersect(rays, mode="any", counts=True)
+    ro = oracle_for(bvh).trace(rays, mode=oracle.ORC_ANY)
+    assert np.array_equal(occ, ro["occluded"])
+    assert np.array_equal(oc["nodes"], ro["counts"]["nodes"])
