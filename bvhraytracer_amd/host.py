@@ -133,6 +133,32 @@ class Bvh:
             pass
 
 
+class FastNodeArrays:
+    """Caller-provided FastNode arrays (the reference layout, rt_bvh_desc).
+
+    Lets a host that already flattened its BVH (or a test with a hand-built
+    tree) hand the arrays to Scene without going through the builder."""
+
+    def __init__(self, node_memory, leaf_memory, gang, bounds, base, order, child_type,
+                 tri_points, tri_ids, branching=None, leaf_size=None):
+        self.node_memory, self.leaf_memory, self.gang = int(node_memory), int(leaf_memory), int(gang)
+        self.bounds = np.ascontiguousarray(bounds, np.float32).reshape(-1, 6 * self.node_memory)
+        self.base = np.ascontiguousarray(base, np.uint32)
+        self.order = np.ascontiguousarray(order, np.int8).reshape(-1, 3, self.node_memory)
+        self.child_type = np.ascontiguousarray(child_type, np.uint16)
+        self.tri_points = np.ascontiguousarray(tri_points, np.float32).reshape(-1)
+        self.tri_ids = np.ascontiguousarray(tri_ids, np.int32)
+        self.node_count = len(self.base)
+        self.desc = RtBvhDesc(branching or self.node_memory, leaf_size or self.leaf_memory, self.gang,
+                              self.node_memory, self.leaf_memory, self.node_count,
+                              ptr(self.bounds), ptr(self.base), ptr(self.order), ptr(self.child_type),
+                              self.tri_points.size, ptr(self.tri_points), ptr(self.tri_ids),
+                              0.0, 0, 0)
+
+    def is_leaf(self):
+        return np.isnan(self.bounds[:, 0])
+
+
 MODES = {"closest": _lib.RT_CLOSEST, "any": _lib.RT_ANY, "save_distance": _lib.RT_SAVE_DIST}
 
 

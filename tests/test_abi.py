@@ -1,0 +1,71 @@
+"""The C ABI: librt.so loads and exports every function include/rt.h
+declares, and the Python records match the C struct layouts (no GPU call)."""
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "rt.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(rt_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_all_declared_symbols_exported(rt):
+    names = declared_functions()
+    assert len(names) >= 17
+    lib = rt.lib()
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in rt.h but not exported"
+    bound = {s[0] for s in rt._lib.SIGNATURES}
+    assert set(names) <= bound, set(names) - bound
+
+
+def test_struct_layouts_match_c(rt):
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "rt.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(rt_ray), sizeof(rt_hit), sizeof(rt_counts),
+         sizeof(rt_bvh_desc), sizeof(rt_stats_t), sizeof(rt_build_params), sizeof(rt_camera));
+  printf("%zu %zu %zu %zu\n", offsetof(rt_bvh_desc, tri_float_count), offsetof(rt_bvh_desc, tri_ids),
+         offsetof(rt_bvh_desc, max_depth), offsetof(rt_stats_t, error_rays));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write(src)
+        subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe])
+        out = subprocess.check_output([exe]).decode().split()
+    import ctypes
+    L = rt._lib
+    got = [rt.RAY_DTYPE.itemsize, rt.HIT_DTYPE.itemsize, rt.COUNTS_DTYPE.itemsize,
+           ctypes.sizeof(L.RtBvhDesc), ctypes.sizeof(L.RtStats), ctypes.sizeof(L.RtBuildParams),
+           ctypes.sizeof(L.RtCamera),
+           L.RtBvhDesc.tri_float_count.offset, L.RtBvhDesc.tri_ids.offset, L.RtBvhDesc.max_depth.offset,
+           L.RtStats.error_rays.offset]
+    assert [int(x) for x in out] == got
+
+
+def test_errors_are_reported_not_raised_across_the_abi(rt):
+    lib = rt.lib()
+    assert lib.rt_trace(None, None, 0, 0, None, None, None) == -1
+    assert b"null scene" in lib.rt_last_error()
+    assert lib.rt_release(None) == 0
+    # a layout violation is rejected before any device work
+    arr = rt.FastNodeArrays(4, 4, 4, np.zeros((1, 24), np.float32), np.zeros(1, np.uint32),
+                            np.zeros((1, 3, 4), np.int8), np.zeros(1, np.uint16),
+                            np.zeros(35, np.float32), np.zeros(4, np.int32))
+    import ctypes
+    h = ctypes.c_void_p()
+    rc = lib.rt_upload_bvh(ctypes.byref(arr.desc), ctypes.byref(h))
+    assert rc == -5 and b"leaf blocks" in lib.rt_last_error()
