@@ -101,6 +101,12 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
     const uint32_t LSTRIDE = leaf_stride(p.Lm);
 
     bool has = false, exhausted = false;
+    // `cur` is the entry visited next.  The child pushed last is always the
+    // next one popped, so it stays in a register instead of going through
+    // the LDS stack; `sp` counts only the entries below it.
+    bool hasCur = false;
+    uint32_t cur = 0;
+    float curDist = 0.0f;
     uint32_t r = 0, sp = 0, live = 0, hitEntry = 0;
     int hitSlot = -1;
     bool occl = false, err = false;
@@ -137,12 +143,12 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                         axis = max_abs_dim(ray.dx, ray.dy, ray.dz);
                         const float da = axis == 0 ? ray.dx : (axis == 1 ? ray.dy : ray.dz);
                         rev = da <= 0.0f;
-                        stE[0] = p.rootEntry;
-                        if (MODE == kSaveDist) {
-                            stD[0] = 0.0f;
+                        cur = p.rootEntry;
+                        curDist = 0.0f;
+                        hasCur = true;
+                        if (MODE == kSaveDist)
                             for (int s = 0; s < M; ++s) aabbD[s * bd] = __builtin_nanf("");
-                        }
-                        sp = 1;
+                        sp = 0;
                         live = 0;
                         hitEntry = 0;
                         hitSlot = -1;
@@ -158,14 +164,21 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
         if (!has) continue;
 
         // ---- one traversal step: pop and visit ----
-        --sp;
-        const uint32_t slot = (sp & (S - 1)) * bd;
-        const uint32_t e = stE[slot];
+        uint32_t e;
         float edist = 0.0f;
-        if (MODE == kSaveDist) edist = stD[slot];
-        if (sp >= S) {
-            stE[slot] = spE[sp - S];
-            if (MODE == kSaveDist) stD[slot] = spD[sp - S];
+        if (hasCur) {
+            e = cur;
+            edist = curDist;
+            hasCur = false;
+        } else {
+            --sp;
+            const uint32_t slot = (sp & (S - 1)) * bd;
+            e = stE[slot];
+            if (MODE == kSaveDist) edist = stD[slot];
+            if (sp >= S) {
+                stE[slot] = spE[sp - S];
+                if (MODE == kSaveDist) stD[slot] = spD[sp - S];
+            }
         }
         bool visit = true;
         if (MODE == kSaveDist && edist > ray.tmax) visit = false;
@@ -197,6 +210,7 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                     if (any) {
                         occl = true;
                         sp = 0;
+                        hasCur = false;
                     }
                 } else {
                     // gang-G tie rule of triIntersect: each program instance
@@ -279,22 +293,29 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                         if ((leafMask >> s) & 1u) return kEntryLeaf | kc | (leafBase + __popc(leafMask & below));
                         return kc | (innerBase + __popc(~leafMask & below));
                     };
+                    // push(s): the previous pending child goes to the stack,
+                    // child s becomes the pending one (visited next)
                     auto push = [&](uint32_t s) {
-                        if (sp >= cap) {
-                            err = true;
-                            return;
+                        if (hasCur) {
+                            if (sp >= cap) {
+                                err = true;
+                                return;
+                            }
+                            const uint32_t sl = (sp & (S - 1)) * bd;
+                            if (sp >= S) {
+                                spE[sp - S] = stE[sl];
+                                if (MODE == kSaveDist) spD[sp - S] = stD[sl];
+                            }
+                            stE[sl] = cur;
+                            if (MODE == kSaveDist) stD[sl] = curDist;
+                            ++sp;
                         }
-                        const uint32_t sl = (sp & (S - 1)) * bd;
-                        if (sp >= S) {
-                            spE[sp - S] = stE[sl];
-                            if (MODE == kSaveDist) spD[sp - S] = stD[sl];
-                        }
-                        stE[sl] = entry_of(s);
+                        cur = entry_of(s);
                         if (MODE == kSaveDist) {
-                            stD[sl] = aabbD[s * bd];
+                            curDist = aabbD[s * bd];
                             aabbD[s * bd] = __builtin_nanf("");
                         }
-                        ++sp;
+                        hasCur = true;
                         live &= ~(1u << s);
                     };
                     if (MODE == kAny) {
@@ -315,11 +336,14 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                             if ((live >> s) & 1u) push(s);
                         }
                     }
-                    if (err) sp = 0;
+                    if (err) {
+                        sp = 0;
+                        hasCur = false;
+                    }
                 }
             }
         }
-        if (sp != 0) continue;
+        if (hasCur || sp != 0) continue;
 
         // ---- ray finished: write its result ----
         if (MODE == kAny) {
