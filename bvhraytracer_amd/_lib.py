@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librt.so")
 
 RT_OK = 0
+RT_ERR_BUSY = -6
 RT_CLOSEST = 0
 RT_ANY = 1
 RT_SAVE_DIST = 2
@@ -21,6 +22,8 @@ RT_COUNTS = 8
 RT_DEVICE_PTRS = 16
 RT_NO_SYNC = 32
 RT_COOP = 64
+
+RT_GRAPH = 128
 
 RT_SCENE_SPHERES = 1
 RT_SCENE_INTERIOR = 2
@@ -73,6 +76,28 @@ class RtStats(ctypes.Structure):
     ]
 
 
+class RtBatchItem(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("rays", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("counts", ctypes.c_void_p), ("n", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
+
+
+class RtServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_cap_bytes", ctypes.c_uint64), ("slots", ctypes.c_uint32),
+                ("max_items", ctypes.c_uint32), ("block_size", ctypes.c_uint32),
+                ("stack_window", ctypes.c_uint32), ("use_graph", ctypes.c_uint32)]
+
+
+class RtServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("failed", ctypes.c_uint64), ("rejected", ctypes.c_uint64),
+                ("cancelled", ctypes.c_uint64), ("rays_completed", ctypes.c_uint64),
+                ("launches", ctypes.c_uint64), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
+                ("max_ms", ctypes.c_double), ("wall_s", ctypes.c_double),
+                ("mrays_per_s", ctypes.c_double), ("peak_admitted_bytes", ctypes.c_uint64),
+                ("mean_requests_per_launch", ctypes.c_double)]
+
+
 # (name, restype, argtypes) of every exported entry point in include/rt.h
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -84,6 +109,19 @@ SIGNATURES = [
     ("rt_release", ctypes.c_int, [_P]),
     ("rt_last_error", ctypes.c_char_p, []),
     ("rt_scene_footprint", ctypes.c_int, [ctypes.POINTER(RtBvhDesc), ctypes.POINTER(ctypes.c_uint64)]),
+    ("rt_batch_create", ctypes.c_int, [_U32, ctypes.POINTER(_P)]),
+    ("rt_batch_trace", ctypes.c_int, [_P, ctypes.POINTER(RtBatchItem), _U32, _U32, _P]),
+    ("rt_batch_set_launch", ctypes.c_int, [_P, _U32, _U32]),
+    ("rt_batch_stats", ctypes.c_int, [_P, ctypes.POINTER(RtStats)]),
+    ("rt_batch_release", ctypes.c_int, [_P]),
+    ("rt_server_create", ctypes.c_int, [ctypes.POINTER(RtServerConfig), ctypes.POINTER(_P)]),
+    ("rt_server_add_scene", ctypes.c_int, [_P, _P, ctypes.POINTER(_U32)]),
+    ("rt_server_submit", ctypes.c_int, [_P, _U32, _P, _U32, _U32, _P, ctypes.POINTER(ctypes.c_uint64)]),
+    ("rt_server_wait", ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_double]),
+    ("rt_server_cancel", ctypes.c_int, [_P, ctypes.c_uint64]),
+    ("rt_server_drain", ctypes.c_int, [_P]),
+    ("rt_server_get_stats", ctypes.c_int, [_P, ctypes.POINTER(RtServerStats), ctypes.c_int]),
+    ("rt_server_destroy", ctypes.c_int, [_P]),
     ("rt_scene_generate", ctypes.c_int, [_U32, ctypes.c_uint64, _U32, _U32,
                                          ctypes.POINTER(ctypes.POINTER(ctypes.c_float)),
                                          ctypes.POINTER(_U32)]),

@@ -460,3 +460,289 @@ extern "C" int rt_release(rt_scene_t s) {
     free_scene(s);
     return RT_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Batched launch: many scenes and ray batches in one persistent launch.
+
+struct rt_batch {
+    std::mutex mu;
+    int device = 0;
+    uint32_t maxItems = 0;
+    BatchItem* dItems = nullptr;
+    BatchScene* dScenes = nullptr;
+    BatchItem* hItems = nullptr;  // pinned staging of the tables
+    BatchScene* hScenes = nullptr;
+    uint32_t* work = nullptr;
+    uint32_t* spill = nullptr;
+    size_t spillBytes = 0;
+    uint32_t blockSize = 256, stackWindow = 16, cus = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool pending = false;
+    hipStream_t ownStream = nullptr;
+    // captured graph: [reset kernel] -> [batched trace kernel]
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t traceNode = nullptr;
+    hipKernelNodeParams traceNodeParams{};
+    hipStream_t graphStream = nullptr;
+    uint32_t graphGrid = 0, graphBd = 0, graphS = 0, graphSpillCap = 0;
+    uint32_t* graphSpill = nullptr;
+    float lastMs = 0;
+    uint64_t raysTraced = 0;
+    uint32_t errorRays = 0;
+    uint64_t graphLaunches = 0;
+};
+
+namespace {
+
+void drop_graph(rt_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->traceNode = nullptr;
+}
+
+void free_batch(rt_batch* b) {
+    if (!b) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(b->device);
+    drop_graph(b);
+    if (b->dItems) (void)hipFree(b->dItems);
+    if (b->dScenes) (void)hipFree(b->dScenes);
+    if (b->hItems) (void)hipHostFree(b->hItems);
+    if (b->hScenes) (void)hipHostFree(b->hScenes);
+    if (b->work) (void)hipFree(b->work);
+    if (b->spill) (void)hipFree(b->spill);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->ownStream) (void)hipStreamDestroy(b->ownStream);
+    (void)hipSetDevice(cur);
+    delete b;
+}
+
+int capture_graph(rt_batch* b, hipStream_t st, const KParams& p, uint32_t grid, uint32_t bd, uint32_t S) {
+    drop_graph(b);
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipError_t e1 = launch_reset(p.work, st);
+    hipError_t e2 = launch_trace(0, kMixed, S, bd, grid, p, st);
+    hipGraph_t g = nullptr;
+    hipError_t e3 = hipStreamEndCapture(st, &g);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3), "graph capture");
+    }
+    b->graph = g;
+    size_t nn = 0;
+    HIP_TRY(hipGraphGetNodes(g, nullptr, &nn), "hipGraphGetNodes");
+    std::vector<hipGraphNode_t> nodes(nn);
+    HIP_TRY(hipGraphGetNodes(g, nodes.data(), &nn), "hipGraphGetNodes");
+    for (auto nd : nodes) {
+        hipGraphNodeType t;
+        if (hipGraphNodeGetType(nd, &t) != hipSuccess || t != hipGraphNodeTypeKernel) continue;
+        hipKernelNodeParams kp{};
+        HIP_TRY(hipGraphKernelNodeGetParams(nd, &kp), "hipGraphKernelNodeGetParams");
+        if (kp.gridDim.x == grid && kp.blockDim.x == bd && kp.sharedMemBytes > 0) {
+            b->traceNode = nd;
+            b->traceNodeParams = kp;
+        }
+    }
+    if (!b->traceNode) return set_error(RT_ERR_HIP, "graph capture: trace kernel node not found");
+    HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    b->graphStream = st;
+    b->graphGrid = grid;
+    b->graphBd = bd;
+    b->graphS = S;
+    b->graphSpill = b->spill;
+    b->graphSpillCap = p.spillCap;
+    return RT_OK;
+}
+
+}  // namespace
+
+extern "C" int rt_batch_create(uint32_t max_items, rt_batch_t* out) {
+    if (!out || max_items == 0) return set_error(RT_ERR_ARG, "rt_batch_create: bad argument");
+    rt_batch* b = new (std::nothrow) rt_batch;
+    if (!b) return set_error(RT_ERR_NOMEM, "rt_batch_create: out of host memory");
+    (void)hipGetDevice(&b->device);
+    b->maxItems = max_items;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess) b->cus = (uint32_t)prop.multiProcessorCount;
+    if (!b->cus) b->cus = 256;
+    hipError_t e = hipSuccess;
+    if ((e = hipMalloc(reinterpret_cast<void**>(&b->dItems), sizeof(BatchItem) * max_items)) != hipSuccess ||
+        (e = hipMalloc(reinterpret_cast<void**>(&b->dScenes), sizeof(BatchScene) * max_items)) != hipSuccess ||
+        (e = hipHostMalloc(reinterpret_cast<void**>(&b->hItems), sizeof(BatchItem) * max_items, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipHostMalloc(reinterpret_cast<void**>(&b->hScenes), sizeof(BatchScene) * max_items, hipHostMallocDefault)) != hipSuccess ||
+        (e = hipMalloc(reinterpret_cast<void**>(&b->work), 64)) != hipSuccess ||
+        (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess) {
+        free_batch(b);
+        return hip_fail(e, "rt_batch_create");
+    }
+    *out = b;
+    return RT_OK;
+}
+
+extern "C" int rt_batch_set_launch(rt_batch_t b, uint32_t block_size, uint32_t stack_window) {
+    if (!b) return set_error(RT_ERR_ARG, "rt_batch_set_launch: null batch");
+    if (block_size && (block_size % 64 || block_size > 256))
+        return set_error(RT_ERR_ARG, "rt_batch_set_launch: block size must be 64, 128, 192 or 256");
+    if (stack_window && stack_window != 8 && stack_window != 16 && stack_window != 32)
+        return set_error(RT_ERR_ARG, "rt_batch_set_launch: stack window must be 8, 16 or 32");
+    std::lock_guard<std::mutex> lk(b->mu);
+    if (block_size) b->blockSize = block_size;
+    if (stack_window) b->stackWindow = stack_window;
+    return RT_OK;
+}
+
+extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t n_items, uint32_t flags,
+                              void* hip_stream) {
+    if (!b) return set_error(RT_ERR_ARG, "rt_batch_trace: null batch");
+    if (n_items == 0) return RT_OK;
+    if (!items) return set_error(RT_ERR_ARG, "rt_batch_trace: null items");
+    if (n_items > b->maxItems) return set_error(RT_ERR_ARG, "rt_batch_trace: more items than the batch was created for");
+    std::lock_guard<std::mutex> lk(b->mu);
+    int prevDev = 0;
+    (void)hipGetDevice(&prevDev);
+    if (prevDev != b->device) HIP_TRY(hipSetDevice(b->device), "hipSetDevice");
+    struct Restore {
+        int dev, prev;
+        ~Restore() {
+            if (dev != prev) (void)hipSetDevice(prev);
+        }
+    } restore{b->device, prevDev};
+    hipStream_t st = reinterpret_cast<hipStream_t>(hip_stream);
+    if (!st) {
+        if (!b->ownStream) HIP_TRY(hipStreamCreateWithFlags(&b->ownStream, hipStreamNonBlocking), "hipStreamCreate");
+        st = b->ownStream;
+    }
+    // the staging tables are reused: the previous batch must have consumed them
+    if (b->pending) {
+        HIP_TRY(hipEventSynchronize(b->ev1), "previous batch");
+        b->pending = false;
+    }
+    uint32_t nScenes = 0, used = 0;
+    uint64_t total = 0;
+    uint32_t cap = 0;
+    std::vector<rt_scene*> scenes;
+    for (uint32_t i = 0; i < n_items; ++i) {
+        const rt_batch_item& it = items[i];
+        if (!it.scene || (it.n && (!it.rays || !it.out)))
+            return set_error(RT_ERR_ARG, "rt_batch_trace: item without scene, rays or output");
+        if (it.flags & ~RT_ANY) return set_error(RT_ERR_UNSUPPORTED, "rt_batch_trace: items support RT_CLOSEST and RT_ANY only");
+        if (it.scene->device != b->device) return set_error(RT_ERR_ARG, "rt_batch_trace: scene on another device");
+        if (it.n == 0) continue;
+        uint32_t si = 0;
+        while (si < nScenes && scenes[si] != it.scene) ++si;
+        if (si == nScenes) {
+            rt_scene* s = it.scene;
+            scenes.push_back(s);
+            BatchScene& bs = b->hScenes[nScenes++];
+            bs.nodes = s->nodes;
+            bs.tris = s->tris;
+            bs.triIds = s->triIds;
+            bs.nodeEntry = s->nodeEntry;
+            bs.rootEntry = s->rootEntry;
+            bs.gang = s->G;
+            bs.Lm = s->Lm;
+            bs.nodeCount = s->nodeCount;
+            bs.M = s->M;
+            bs.pad0 = bs.pad1 = bs.pad2 = 0;
+            cap = std::max(cap, stack_capacity(s));
+        }
+        BatchItem& bi = b->hItems[used++];
+        bi.rays = it.rays;
+        bi.out = it.out;
+        bi.counts = it.counts;
+        bi.scene = si;
+        bi.mode = (it.flags & RT_ANY) ? kAny : kClosest;
+        bi.start = (uint32_t)total;
+        bi.n = it.n;
+        total += it.n;
+        if (total >= 0x80000000ull) return set_error(RT_ERR_ARG, "rt_batch_trace: at most 2^31-1 rays per batch");
+    }
+    if (used == 0) return RT_OK;
+    const uint32_t S = b->stackWindow, bd = b->blockSize;
+    const bool graph = (flags & RT_GRAPH) != 0;
+    const int occ = std::max(1, trace_occupancy(0, kMixed, S, bd));
+    const uint64_t full = (uint64_t)b->cus * (uint64_t)occ;
+    const uint32_t grid = graph ? (uint32_t)full
+                                : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(full, (total + bd - 1) / bd));
+    uint32_t spillCap = cap > S ? cap - S : 0;
+    if (graph && b->exec && b->graphSpillCap > spillCap && b->graphGrid == grid) spillCap = b->graphSpillCap;
+    const size_t spillNeed = std::max<size_t>(64, (size_t)grid * bd * spillCap * 4);
+    int rc;
+    if ((rc = ensure(reinterpret_cast<void**>(&b->spill), b->spillBytes, spillNeed, "hipMalloc(spill)")) != RT_OK)
+        return rc;
+    HIP_TRY(hipMemcpyAsync(b->dItems, b->hItems, sizeof(BatchItem) * used, hipMemcpyHostToDevice, st), "items");
+    HIP_TRY(hipMemcpyAsync(b->dScenes, b->hScenes, sizeof(BatchScene) * nScenes, hipMemcpyHostToDevice, st), "scenes");
+    KParams p{};
+    p.n = (uint32_t)total;
+    p.spill = b->spill;
+    p.spillCap = spillCap;
+    p.work = b->work;
+    p.items = b->dItems;
+    p.scenes = b->dScenes;
+    p.nItems = used;
+    if (graph) {
+        if (!b->exec || b->graphStream != st || b->graphGrid != grid || b->graphBd != bd || b->graphS != S ||
+            b->graphSpill != b->spill || b->graphSpillCap != spillCap) {
+            if ((rc = capture_graph(b, st, p, grid, bd, S)) != RT_OK) return rc;
+        } else {
+            // re-arm: same graph, new kernel arguments (ray count, item count)
+            hipKernelNodeParams kp = b->traceNodeParams;
+            KParams copy = p;
+            void* args[] = {&copy};
+            kp.kernelParams = args;
+            kp.extra = nullptr;
+            HIP_TRY(hipGraphExecKernelNodeSetParams(b->exec, b->traceNode, &kp), "hipGraphExecKernelNodeSetParams");
+        }
+        HIP_TRY(hipEventRecord(b->ev0, st), "hipEventRecord");
+        HIP_TRY(hipGraphLaunch(b->exec, st), "hipGraphLaunch");
+        HIP_TRY(hipEventRecord(b->ev1, st), "hipEventRecord");
+        ++b->graphLaunches;
+    } else {
+        HIP_TRY(launch_reset(b->work, st), "reset kernel");
+        HIP_TRY(hipEventRecord(b->ev0, st), "hipEventRecord");
+        HIP_TRY(launch_trace(0, kMixed, S, bd, grid, p, st), "batched trace kernel");
+        HIP_TRY(hipEventRecord(b->ev1, st), "hipEventRecord");
+    }
+    b->pending = true;
+    b->raysTraced += total;
+    if (!(flags & RT_NO_SYNC)) {
+        HIP_TRY(hipEventSynchronize(b->ev1), "batched trace kernel");
+        b->pending = false;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess) b->lastMs = ms;
+        uint32_t errs = 0;
+        HIP_TRY(hipMemcpy(&errs, b->work + 1, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
+        b->errorRays += errs;
+    }
+    return RT_OK;
+}
+
+extern "C" int rt_batch_stats(rt_batch_t b, rt_stats_t* o) {
+    if (!b || !o) return set_error(RT_ERR_ARG, "rt_batch_stats: null argument");
+    std::lock_guard<std::mutex> lk(b->mu);
+    if (b->pending && hipEventQuery(b->ev1) == hipSuccess) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess) b->lastMs = ms;
+    }
+    std::memset(o, 0, sizeof(*o));
+    o->bytes_resident = b->spillBytes + b->maxItems * (sizeof(BatchItem) + sizeof(BatchScene));
+    o->last_kernel_ms = b->lastMs;
+    o->rays_traced = b->raysTraced;
+    o->block_size = b->blockSize;
+    o->stack_window = b->stackWindow;
+    o->max_stack = b->graphSpillCap + b->stackWindow;
+    o->error_rays = b->errorRays;
+    return RT_OK;
+}
+
+extern "C" int rt_batch_release(rt_batch_t b) {
+    if (!b) return RT_OK;
+    if (b->pending) (void)hipEventSynchronize(b->ev1);
+    free_batch(b);
+    return RT_OK;
+}

@@ -19,7 +19,25 @@
 
 namespace rt {
 
-enum TraceMode { kClosest = 0, kAny = 1, kSaveDist = 2 };
+enum TraceMode { kClosest = 0, kAny = 1, kSaveDist = 2, kMixed = 3 };
+
+// Batched launch tables (device memory).  A batch is a list of items, each a
+// contiguous ray range of one scene in one mode; the global ray index space
+// is the concatenation of the items (item i covers [start, start + n)).
+struct BatchScene {
+    const uint32_t* nodes;
+    const float* tris;
+    const int32_t* triIds;
+    const uint32_t* nodeEntry;
+    uint32_t rootEntry, gang, Lm, nodeCount, M, pad0, pad1, pad2;
+};
+
+struct BatchItem {
+    const rt_ray* rays;
+    void* out;          // rt_hit[n] (closest) or uint8_t[n] (any)
+    rt_counts* counts;  // optional
+    uint32_t scene, mode, start, n;
+};
 
 struct KParams {
     const uint32_t* __restrict__ nodes;  // inner records
@@ -38,9 +56,14 @@ struct KParams {
     uint32_t gang;
     uint32_t Lm;
     uint32_t nodeCount;
+    // batched launch only
+    const BatchItem* items;
+    const BatchScene* scenes;
+    uint32_t nItems;
 };
 
-// Launches the per-ray kernel for node memory M on `stream`.
+// Launches the per-ray kernel for node memory M on `stream`; mode kMixed
+// launches the batched kernel (M is then read per scene).
 hipError_t launch_trace(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize,
                         uint32_t grid, const KParams& p, hipStream_t stream);
 

@@ -82,23 +82,36 @@ __device__ __forceinline__ int max_abs_dim(float x, float y, float z) {
     return (x > y) ? ((x > z) ? 0 : 2) : ((y > z) ? 1 : 2);
 }
 
-template <int M, int MODE, int S>
+// One kernel template for the plain launch (one scene, uniform parameters,
+// node memory MT known at compile time) and the batched launch (BATCH: every
+// lane carries the scene and item of its current ray; MT = 0 reads the node
+// memory from the scene; MODE = kMixed takes closest/any per item).
+template <int MT, int MODE, int S, bool BATCH>
 __global__ void __launch_bounds__(256) k_trace(KParams p) {
     extern __shared__ uint32_t smem[];
-    constexpr uint32_t RECW = record_words(M);
-    constexpr uint32_t OW = order_words(M);
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
     const uint32_t lane = __lane_id();
-    uint32_t* stE = smem + tid;                                // stack entries
-    float* stD = reinterpret_cast<float*>(smem + S * bd) + tid;  // stack distances (save dist)
-    float* aabbD = reinterpret_cast<float*>(smem + 2 * S * bd) + tid;  // aabbDistances
+    uint32_t* stE = smem + tid;                                        // stack entries
+    float* stD = reinterpret_cast<float*>(smem + S * bd) + tid;        // stack distances (save dist)
+    float* aabbD = reinterpret_cast<float*>(smem + 2 * S * bd) + tid;  // aabbDistances (save dist)
     const size_t gslot = (size_t)blockIdx.x * bd + tid;
     const size_t nslots = (size_t)gridDim.x * bd;
     uint32_t* spE = p.spill + gslot * p.spillCap;
     float* spD = reinterpret_cast<float*>(p.spill + nslots * p.spillCap) + gslot * p.spillCap;
     const uint32_t cap = S + p.spillCap;
-    const uint32_t G = p.gang;
-    const uint32_t LSTRIDE = leaf_stride(p.Lm);
+
+    // scene / item context (uniform for the plain launch, per lane in a batch)
+    const uint32_t* nodes = p.nodes;
+    const float* tris = p.tris;
+    const int32_t* triIds = p.triIds;
+    const uint32_t* nodeEntry = p.nodeEntry;
+    uint32_t rootEntry = p.rootEntry, G = p.gang, Lm = p.Lm, nodeCount = p.nodeCount;
+    uint32_t Mrt = MT, recw = record_words(MT), ow = order_words(MT);
+    const rt_ray* rays = p.rays;
+    rt_hit* hits = p.hits;
+    uint8_t* occluded = p.occluded;
+    rt_counts* counts = p.counts;
+    bool anyMode = (MODE == kAny);
 
     bool has = false, exhausted = false;
     // `cur` is the entry visited next.  The child pushed last is always the
@@ -129,7 +142,35 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                     const uint32_t rr = base + (uint32_t)__popcll(need & __lanemask_lt());
                     if (rr < p.n) {
                         r = rr;
-                        const rt_ray& in = p.rays[r];
+                        if constexpr (BATCH) {
+                            // item of global ray rr: last item with start <= rr
+                            uint32_t lo = 0, hi = p.nItems - 1;
+                            while (lo < hi) {
+                                const uint32_t mid = (lo + hi + 1) >> 1;
+                                if (p.items[mid].start <= rr) lo = mid;
+                                else hi = mid - 1;
+                            }
+                            const BatchItem& it = p.items[lo];
+                            const BatchScene& sc = p.scenes[it.scene];
+                            nodes = sc.nodes;
+                            tris = sc.tris;
+                            triIds = sc.triIds;
+                            nodeEntry = sc.nodeEntry;
+                            rootEntry = sc.rootEntry;
+                            G = sc.gang;
+                            Lm = sc.Lm;
+                            nodeCount = sc.nodeCount;
+                            Mrt = sc.M;
+                            recw = record_words(Mrt);
+                            ow = order_words(Mrt);
+                            rays = it.rays;
+                            hits = static_cast<rt_hit*>(it.out);
+                            occluded = static_cast<uint8_t*>(it.out);
+                            counts = it.counts;
+                            anyMode = it.mode == kAny;
+                            r = rr - it.start;
+                        }
+                        const rt_ray& in = rays[r];
                         ray.tmax = in.tmax;
                         ray.ox = in.ox;
                         ray.oy = in.oy;
@@ -143,11 +184,11 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                         axis = max_abs_dim(ray.dx, ray.dy, ray.dz);
                         const float da = axis == 0 ? ray.dx : (axis == 1 ? ray.dy : ray.dz);
                         rev = da <= 0.0f;
-                        cur = p.rootEntry;
+                        cur = rootEntry;
                         curDist = 0.0f;
                         hasCur = true;
                         if (MODE == kSaveDist)
-                            for (int s = 0; s < M; ++s) aabbD[s * bd] = __builtin_nanf("");
+                            for (int s = 0; s < MT; ++s) aabbD[s * bd] = __builtin_nanf("");
                         sp = 0;
                         live = 0;
                         hitEntry = 0;
@@ -162,8 +203,10 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
         }
         if (__ballot(has) == 0) break;
         if (!has) continue;
+        const uint32_t M = BATCH ? Mrt : (uint32_t)MT;
+        const uint32_t LSTRIDE = leaf_stride(Lm);
 
-        // ---- one traversal step: pop and visit ----
+        // ---- one traversal step: take the next entry and visit it ----
         uint32_t e;
         float edist = 0.0f;
         if (hasCur) {
@@ -190,10 +233,10 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
             const uint32_t k = ((e >> kEntryKShift) & 3u) * 4u + 4u;
             const uint32_t idx = e & kEntryIndexMask;
             if (e & kEntryLeaf) {
-                const float* T = p.tris + (size_t)idx * LSTRIDE;
+                const float* T = tris + (size_t)idx * LSTRIDE;
                 ++cL;
                 cLS += k;
-                if (MODE == kAny) {
+                if (anyMode) {
                     bool any = false;
                     for (uint32_t s0 = 0; s0 < k; s0 += 4) {
                         const float4 a0 = ld4(T + s0), a1 = ld4(T + k + s0), a2 = ld4(T + 2 * k + s0);
@@ -248,13 +291,13 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                     }
                 }
             } else {
-                const uint32_t* rec = p.nodes + (size_t)idx * RECW;
+                const uint32_t* rec = nodes + (size_t)idx * recw;
                 ++cN;
                 cNS += k;
                 const uint4 h0 = ld4u(rec + 6 * M);
                 const uint4 h1 = ld4u(rec + 6 * M + 4);
                 uint4 h2 = make_uint4(0, 0, 0, 0);
-                if (OW == 2) h2 = ld4u(rec + 6 * M + 8);
+                if (ow == 2) h2 = ld4u(rec + 6 * M + 8);
                 uint32_t H = 0;
                 const float* bp = reinterpret_cast<const float*>(rec);
                 for (uint32_t s0 = 0; s0 < k; s0 += 4) {
@@ -276,7 +319,7 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                     const uint32_t leafMask = h0.z & 0xFFFFu, c = (h0.z >> 16) & 31u;
                     const uint32_t kcodes = h0.w, childBase = h1.x;
                     uint64_t row;
-                    if (OW == 1) {
+                    if (ow == 1) {
                         row = axis == 0 ? h1.y : (axis == 1 ? h1.z : h1.w);
                     } else {
                         const uint32_t lo = axis == 0 ? h1.y : (axis == 1 ? h1.w : h2.y);
@@ -286,7 +329,7 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                     auto entry_of = [&](uint32_t s) -> uint32_t {
                         if (s >= c) {  // padded slot: resolve the FastNode id like the reference
                             const uint32_t id = childBase + s;
-                            return id < p.nodeCount ? p.nodeEntry[id] : kEntryInvalid;
+                            return id < nodeCount ? nodeEntry[id] : kEntryInvalid;
                         }
                         const uint32_t below = (1u << s) - 1u;
                         const uint32_t kc = ((kcodes >> (2 * s)) & 3u) << kEntryKShift;
@@ -318,9 +361,9 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                         hasCur = true;
                         live &= ~(1u << s);
                     };
-                    if (MODE == kAny) {
+                    if (anyMode) {
                         // intersectSecondary: slot order 0..M-1 (fastNodeManager.h:1481-1488)
-                        for (uint32_t s = 0; s < (uint32_t)M; ++s)
+                        for (uint32_t s = 0; s < M; ++s)
                             if ((live >> s) & 1u) push(s);
                     } else if (rev) {
                         for (uint32_t q = 0; q < c; ++q) {
@@ -330,7 +373,7 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                     } else {
                         // backward over the zero-padded order row: the padding
                         // (slot 0) comes first when childCount < M
-                        if (c < (uint32_t)M && (live & 1u)) push(0);
+                        if (c < M && (live & 1u)) push(0);
                         for (uint32_t q = c; q-- > 0;) {
                             const uint32_t s = (uint32_t)(row >> (4 * q)) & 15u;
                             if ((live >> s) & 1u) push(s);
@@ -346,19 +389,19 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
         if (hasCur || sp != 0) continue;
 
         // ---- ray finished: write its result ----
-        if (MODE == kAny) {
-            p.occluded[r] = occl ? 1 : 0;
+        if (anyMode) {
+            occluded[r] = occl ? 1 : 0;
         } else {
             rt_hit h;
             if (hitSlot >= 0) {
                 const uint32_t ord = hitEntry & kEntryIndexMask;
                 const uint32_t k = ((hitEntry >> kEntryKShift) & 3u) * 4u + 4u;
-                const float* T = p.tris + (size_t)ord * LSTRIDE + hitSlot;
+                const float* T = tris + (size_t)ord * LSTRIDE + hitSlot;
                 float t, b1, b2;
                 (void)mt(T[0], T[k], T[2 * k], T[3 * k], T[4 * k], T[5 * k], T[6 * k], T[7 * k], T[8 * k], ray, t,
                          b1, b2);
-                h.tri = p.triIds[(size_t)ord * p.Lm + hitSlot];
-                h.leaf = ord * 9u * p.Lm;
+                h.tri = triIds[(size_t)ord * Lm + hitSlot];
+                h.leaf = ord * 9u * Lm;
                 h.slot = hitSlot;
                 h.t = ray.tmax;
                 h.b1 = b1;
@@ -371,15 +414,15 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                 h.b1 = 0.0f;
                 h.b2 = 0.0f;
             }
-            p.hits[r] = h;
+            hits[r] = h;
         }
-        if (p.counts) {
+        if (counts) {
             rt_counts c;
             c.nodes = cN;
             c.leaves = cL;
             c.node_slots = cNS;
             c.leaf_slots = cLS;
-            p.counts[r] = c;
+            counts[r] = c;
         }
         if (err) atomicAdd(p.work + 1, 1u);
         has = false;
@@ -392,7 +435,16 @@ __global__ void k_reset(uint32_t* work) {
 
 template <int M, int MODE, int S>
 void* kernel_ptr() {
-    return reinterpret_cast<void*>(&k_trace<M, MODE, S>);
+    return reinterpret_cast<void*>(&k_trace<M, MODE, S, false>);
+}
+
+void* pick_batch(uint32_t S) {
+    switch (S) {
+        case 8: return reinterpret_cast<void*>(&k_trace<0, kMixed, 8, true>);
+        case 16: return reinterpret_cast<void*>(&k_trace<0, kMixed, 16, true>);
+        case 32: return reinterpret_cast<void*>(&k_trace<0, kMixed, 32, true>);
+        default: return nullptr;
+    }
 }
 
 template <int M, int MODE>
@@ -416,6 +468,7 @@ void* pick_mode(int mode, uint32_t S) {
 }
 
 void* pick(uint32_t M, int mode, uint32_t S) {
+    if (mode == kMixed) return pick_batch(S);
     switch (M) {
         case 4: return pick_mode<4>(mode, S);
         case 8: return pick_mode<8>(mode, S);

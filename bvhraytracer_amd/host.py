@@ -22,8 +22,8 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import (COUNTS_DTYPE, HIT_DTYPE, RAY_DTYPE, RtBuildParams, RtBvhDesc, RtCamera,
-                   RtStats, check, lib, ptr)
+from ._lib import (COUNTS_DTYPE, HIT_DTYPE, RAY_DTYPE, RtBatchItem, RtBuildParams, RtBvhDesc,
+                   RtCamera, RtServerConfig, RtServerStats, RtStats, check, lib, ptr)
 
 
 def roundup(x, g):
@@ -214,6 +214,109 @@ class Scene:
         if getattr(self, "_h", None):
             lib().rt_release(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """One persistent launch over many (scene, rays) items (rt_batch_trace).
+
+    items: iterable of (scene, rays, out, mode[, counts]) with device buffers
+    (torch CUDA tensors); mode 'closest' writes rt_hit records, 'any' uint8."""
+
+    def __init__(self, max_items=1024):
+        h = ctypes.c_void_p()
+        check(lib().rt_batch_create(int(max_items), ctypes.byref(h)))
+        self._h = h
+        self.max_items = max_items
+
+    def set_launch(self, block_size=0, stack_window=0):
+        check(lib().rt_batch_set_launch(self._h, int(block_size), int(stack_window)))
+
+    def trace(self, items, graph=True, stream=None, sync=True):
+        arr = (RtBatchItem * len(items))()
+        for i, it in enumerate(items):
+            scene, rays, out, mode = it[:4]
+            counts = it[4] if len(it) > 4 else None
+            arr[i] = RtBatchItem(scene._h, ptr(rays), ptr(out), ptr(counts) if counts is not None else None,
+                                 int(rays.shape[0]), MODES[mode])
+        flags = (_lib.RT_GRAPH if graph else 0) | (0 if sync else _lib.RT_NO_SYNC)
+        s = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        check(lib().rt_batch_trace(self._h, arr, len(items), flags, s))
+
+    def stats(self):
+        s = RtStats()
+        check(lib().rt_batch_stats(self._h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in RtStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rt_batch_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop over resident scenes (rt_server_*): admission by device
+    footprint, continuous batching into batched persistent launches."""
+
+    def __init__(self, hbm_cap_bytes=0, slots=0, max_items=0, block_size=0, stack_window=0,
+                 use_graph=True):
+        cfg = RtServerConfig(int(hbm_cap_bytes), int(slots), int(max_items), int(block_size),
+                             int(stack_window), int(bool(use_graph)))
+        h = ctypes.c_void_p()
+        check(lib().rt_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._scenes = []
+        self._live = {}  # request id -> (rays, out) kept alive until waited
+
+    def add_scene(self, scene):
+        sid = ctypes.c_uint32()
+        check(lib().rt_server_add_scene(self._h, scene._h, ctypes.byref(sid)))
+        self._scenes.append(scene)
+        return sid.value
+
+    def submit(self, scene_id, rays, mode="closest"):
+        """Returns (request id, output array filled when the request completes)."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        out = np.empty(len(rays), np.uint8) if mode == "any" else np.empty(len(rays), HIT_DTYPE)
+        rid = ctypes.c_uint64()
+        check(lib().rt_server_submit(self._h, int(scene_id), ptr(rays), len(rays), MODES[mode], ptr(out),
+                                     ctypes.byref(rid)))
+        self._live[rid.value] = (rays, out)
+        return rid.value, out
+
+    def wait(self, rid, timeout=-1.0):
+        rc = lib().rt_server_wait(self._h, int(rid), float(timeout))
+        if rc != _lib.RT_ERR_BUSY:
+            self._live.pop(rid, None)
+        check(rc)
+
+    def cancel(self, rid):
+        check(lib().rt_server_cancel(self._h, int(rid)))
+
+    def drain(self):
+        check(lib().rt_server_drain(self._h))
+
+    def stats(self, reset=False):
+        s = RtServerStats()
+        check(lib().rt_server_get_stats(self._h, ctypes.byref(s), int(bool(reset))))
+        return {f: getattr(s, f) for f, _ in RtServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().rt_server_destroy(self._h)
+            self._h = None
+            self._live.clear()
 
     def __del__(self):
         try:

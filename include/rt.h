@@ -142,12 +142,13 @@ typedef struct rt_stats_t {
     uint32_t block_size;      /* launch configuration chosen for the scene     */
     uint32_t stack_window;    /* LDS stack entries per lane                    */
     uint32_t max_stack;       /* stack capacity per ray (LDS + spill)          */
-    uint32_t error_rays;      /* rays that hit a guarded condition (see DESIGN) */
+    uint32_t error_rays;      /* rays stopped by a guard: stack capacity exceeded,
+                                 or a padded child id past the node array       */
 } rt_stats_t;
 
 int rt_stats(rt_scene_t s, rt_stats_t* out);
 
-/* Overrides the launch configuration (block size 64..1024 in multiples of 64,
+/* Overrides the launch configuration (block size 64, 128, 192 or 256,
  * LDS stack window 8/16/32 entries).  0 keeps the current value. */
 int rt_set_launch(rt_scene_t s, uint32_t block_size, uint32_t stack_window);
 
@@ -157,6 +158,83 @@ const char* rt_last_error(void);
 
 /* Number of device bytes a scene built from desc would occupy (admission). */
 int rt_scene_footprint(const rt_bvh_desc* desc, uint64_t* bytes);
+
+/* ---- batched launch ------------------------------------------------------------------ */
+
+/* Many scenes and ray batches of uneven size in ONE persistent launch: the
+ * rays of all items form one index space that the waves drain through a
+ * shared work counter (work stealing at ray granularity); every lane carries
+ * the scene of its current ray.  No reference counterpart: the reference
+ * traces one scene per CameraFast render (cameraFast.h:318-582). */
+typedef struct rt_batch* rt_batch_t;
+
+typedef struct rt_batch_item {
+    rt_scene_t scene;
+    const rt_ray* rays; /* device pointer, n rays                              */
+    void* out;          /* device: rt_hit[n] (RT_CLOSEST) or uint8_t[n] (RT_ANY) */
+    rt_counts* counts;  /* device, optional (NULL)                              */
+    uint32_t n;
+    uint32_t flags;     /* RT_CLOSEST or RT_ANY                                 */
+} rt_batch_item;
+
+#define RT_GRAPH 128u /* rt_batch_trace: replay the captured hipGraph (re-armed
+                         with the new item table) instead of a direct launch  */
+
+int rt_batch_create(uint32_t max_items, rt_batch_t* out);
+/* Items must all live on the batch's device; total rays < 2^31. */
+int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t n_items, uint32_t flags,
+                   void* hip_stream);
+int rt_batch_set_launch(rt_batch_t b, uint32_t block_size, uint32_t stack_window);
+int rt_batch_stats(rt_batch_t b, rt_stats_t* out);
+int rt_batch_release(rt_batch_t b);
+
+/* ---- serving ----------------------------------------------------------------------- */
+
+/* A request loop over resident scenes with continuous batching: requests are
+ * admitted first-come first-served while the device bytes of admitted
+ * requests (rays + results) stay under hbm_cap_bytes, and everything admitted
+ * and waiting is coalesced into one batched persistent launch whenever one of
+ * `slots` launch slots (stream + device arena) is free, so requests that arrive
+ * while others run join the next launch and overlap with the running ones.
+ * Latency is measured from rt_server_submit to the results being in the
+ * caller's buffer.  No reference counterpart (the reference renders offline). */
+typedef struct rt_server* rt_server_t;
+
+typedef struct rt_server_config {
+    uint64_t hbm_cap_bytes; /* admission cap on admitted request bytes (0 = 4 GiB) */
+    uint32_t slots;         /* concurrent launches (0 = 2)                         */
+    uint32_t max_items;     /* requests per launch (0 = 512)                       */
+    uint32_t block_size;    /* launch configuration (0 = defaults)                 */
+    uint32_t stack_window;
+    uint32_t use_graph;     /* replay captured hipGraphs (1) or launch directly (0)  */
+} rt_server_config;
+
+typedef struct rt_server_stats {
+    uint64_t submitted, completed, failed, rejected, cancelled;
+    uint64_t rays_completed, launches;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* request latency since the last reset */
+    double wall_s;        /* first submit to last completion since the last reset  */
+    double mrays_per_s;   /* rays_completed / wall_s                                */
+    uint64_t peak_admitted_bytes;
+    double mean_requests_per_launch;
+} rt_server_stats;
+
+int rt_server_create(const rt_server_config* cfg, rt_server_t* out);
+/* Registers a resident scene (the server does not own it). */
+int rt_server_add_scene(rt_server_t sv, rt_scene_t scene, uint32_t* scene_id);
+/* Queues a request.  rays/out are host buffers that must stay valid until the
+ * request completes (out: rt_hit[n] for RT_CLOSEST, uint8_t[n] for RT_ANY).
+ * RT_ERR_BUSY: the request alone exceeds the admission cap. */
+int rt_server_submit(rt_server_t sv, uint32_t scene_id, const rt_ray* rays, uint32_t n,
+                     uint32_t flags, void* out, uint64_t* request_id);
+/* RT_OK when the request completed, RT_ERR_BUSY on timeout, the launch's error
+ * code if its launch failed, RT_ERR_ARG for an unknown/cancelled id. */
+int rt_server_wait(rt_server_t sv, uint64_t request_id, double timeout_s);
+/* Cancels a request that has not been launched yet. */
+int rt_server_cancel(rt_server_t sv, uint64_t request_id);
+int rt_server_drain(rt_server_t sv);
+int rt_server_get_stats(rt_server_t sv, rt_server_stats* out, int reset);
+int rt_server_destroy(rt_server_t sv);
 
 /* ---- host side: scenes, builder, camera, config ------------------------------- */
 

@@ -1,0 +1,92 @@
+"""Batched launch parity: one persistent launch over several scenes (mixed
+N/L, mixed closest/any items, uneven sizes) must give, for every ray, the
+same result as the plain per-scene launch and the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.helpers import assert_closest_equal, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def scenes(rt):
+    out = []
+    for seed, (N, L), tri in ((21, (4, 4), 20_000), (22, (8, 8), 50_000), (23, (16, 8), 30_000),
+                              (24, (12, 16), 40_000), (25, (8, 4), 5_000)):
+        tris = rt.interior_scene(seed, tri)
+        bvh = rt.Bvh(tris, N, L)
+        sc = rt.Scene(bvh)
+        prim = rt.camera_rays((0.0, 5.0, -14.0), (0.0, 3.0, 0.0), 64, 48, 16)
+        h = sc.intersect(prim)
+        nrm, pos = bvh.surface(prim, h)
+        diff = rt.spawn_diffuse(prim, nrm, pos, h, 5000 + 1000 * seed % 7000, seed=seed)
+        shadow, _ = rt.spawn_shadow(nrm, pos, h, (0.0, 9.5, 0.0), 1.0)
+        out.append((bvh, sc, np.concatenate([prim, diff]), shadow))
+    return out
+
+
+def dev(rt, a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.float32).reshape(-1, 10)).cuda()
+
+
+def run_batch(rt, scenes, order, graph, batch):
+    import torch
+    items, outs = [], []
+    for i, kind in order:
+        bvh, sc, rays, shadow = scenes[i]
+        if kind == "closest":
+            r = dev(rt, rays)
+            o = torch.empty((len(rays), 6), dtype=torch.int32, device="cuda")
+            c = torch.empty((len(rays), 4), dtype=torch.int32, device="cuda")
+            items.append((sc, r, o, "closest", c))
+        else:
+            r = dev(rt, shadow)
+            o = torch.empty(len(shadow), dtype=torch.uint8, device="cuda")
+            c = torch.empty((len(shadow), 4), dtype=torch.int32, device="cuda")
+            items.append((sc, r, o, "any", c))
+        outs.append((i, kind, o, c))
+    batch.trace(items, graph=graph)
+    torch.cuda.synchronize()
+    return outs
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_matches_oracle(rt, scenes, graph):
+    batch = rt.Batch(64)
+    order = [(0, "closest"), (1, "any"), (2, "closest"), (3, "closest"), (1, "closest"),
+             (4, "any"), (0, "any"), (4, "closest")]
+    # two calls with different item sets: the graph is re-armed, not re-captured
+    for rep, ordr in enumerate((order, order[::-1][:5])):
+        print(f"batch graph={graph} call {rep}: {len(ordr)} items", flush=True)
+        outs = run_batch(rt, scenes, ordr, graph, batch)
+        for i, kind, o, c in outs:
+            bvh, sc, rays, shadow = scenes[i]
+            cnt = c.cpu().numpy().view(rt.COUNTS_DTYPE).reshape(-1)
+            if kind == "closest":
+                hits = o.cpu().numpy().view(rt.HIT_DTYPE).reshape(-1)
+                ref = oracle_for(bvh).trace(rays)
+                assert_closest_equal(hits, ref["hits"], bvh, cnt, ref["counts"], f"batch scene {i}")
+            else:
+                occ = o.cpu().numpy()
+                ref = oracle_for(bvh).trace(shadow, mode=oracle.ORC_ANY)
+                assert np.array_equal(occ, ref["occluded"]), f"batch any scene {i}"
+                assert np.array_equal(cnt["leaves"], ref["counts"]["leaves"])
+    st = batch.stats()
+    assert st["error_rays"] == 0 and st["rays_traced"] > 0
+    batch.close()
+
+
+def test_batch_equals_plain(rt, scenes):
+    import torch
+    batch = rt.Batch(16)
+    order = [(i, "closest") for i in range(len(scenes))]
+    outs = run_batch(rt, scenes, order, True, batch)
+    for i, kind, o, c in outs:
+        bvh, sc, rays, _ = scenes[i]
+        r = dev(rt, rays)
+        ref = torch.empty((len(rays), 6), dtype=torch.int32, device="cuda")
+        sc.intersect_device(r, ref)
+        assert torch.equal(o, ref), f"scene {i}"
