@@ -44,6 +44,166 @@ def algorithmic_bytes(counts, M):
             + 36 * c["leaf_slots"].astype(np.float64).sum())
 
 
+def clamp_lognormal(rng, mu, sigma, lo, hi, size):
+    return np.clip(np.exp(rng.normal(mu, sigma, size)), lo, hi).astype(np.int64)
+
+
+def random_camera_rays(rt, rng, n):
+    """Primary rays of a random camera inside the 30x10x30 room: n rays of a
+    256-wide image (tile order), the last tile row cut to n."""
+    pos = rng.uniform([-12.0, 1.0, -12.0], [12.0, 9.0, 12.0]).astype(np.float32)
+    d = rng.normal(size=3)
+    d[1] *= 0.3
+    tgt = (pos + 5.0 * d / np.linalg.norm(d)).astype(np.float32)
+    h = max(16, int(np.ceil(n / 256 / 16)) * 16)
+    return rt.camera_rays(tuple(pos), tuple(tgt), 256, h, 16)[:n]
+
+
+def bench_batched(rt, torch, args, log):
+    """64 scenes (seeds 100..163), triangles clamp(LogNormal(ln 2e5, 1), 1e4, 2e6),
+    rays per scene clamp(LogNormal(ln 6.4e4, 1.2), 4096, 2^20), half primary
+    (random camera in the room), half diffuse bounces from those hits.
+    plain = one rt_trace per scene; batched = one rt_batch_trace (graph)."""
+    rng = np.random.default_rng(4)
+    ntris = clamp_lognormal(rng, np.log(2e5), 1.0, 1e4, 2e6, 64)
+    nrays = clamp_lognormal(rng, np.log(6.4e4), 1.2, 4096, 1 << 20, 64)
+    t0 = time.time()
+    scenes = []
+    for i in range(64):
+        tris = rt.interior_scene(100 + i, int(ntris[i]))
+        bvh = rt.Bvh(tris, args.branching, args.leaf)
+        sc = rt.Scene(bvh)
+        npri = int(nrays[i]) // 2
+        prim = random_camera_rays(rt, np.random.default_rng(1000 + i), npri)
+        ph = sc.intersect(prim)
+        nrm, pos = bvh.surface(prim, ph)
+        if (ph["slot"] >= 0).any():
+            diff = rt.spawn_diffuse(prim, nrm, pos, ph, int(nrays[i]) - npri, seed=100 + i)
+            rays = np.concatenate([prim, diff])
+        else:
+            rays = prim
+        d_r = torch.from_numpy(rays.view(np.float32).reshape(-1, 10)).cuda()
+        d_h = torch.empty((len(rays), 6), dtype=torch.int32, device="cuda")
+        scenes.append((bvh, sc, d_r, d_h))
+    total = sum(x[2].shape[0] for x in scenes)
+    log(f"batched workload: 64 scenes, {int(ntris.sum())} tris, {total} rays, setup {time.time() - t0:.1f}s")
+    stream = torch.cuda.current_stream()
+
+    def plain():
+        for bvh, sc, d_r, d_h in scenes:
+            sc.intersect_device(d_r, d_h, stream=stream)
+
+    batch = rt.Batch(64)
+    outs = [torch.empty_like(x[3]) for x in scenes]
+    items = [(sc, d_r, o, "closest") for (bvh, sc, d_r, d_h), o in zip(scenes, outs)]
+
+    def batched():
+        batch.trace(items, graph=True, stream=stream)
+
+    res = {}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        for _ in range(max(1, args.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(max(3, args.steps // 2)):
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t)
+        res[name] = total / float(np.median(ts)) / 1e6
+    same = all(torch.equal(o, x[3]) for o, x in zip(outs, scenes))
+    log(f"batched {res['batched']:.1f} Mrays/s vs plain {res['plain']:.1f} Mrays/s, identical={same}")
+    batch.close()
+    out = {"scenes": 64, "triangles": int(ntris.sum()), "rays": int(total),
+           "plain_mrays": round(res["plain"], 2), "batched_mrays": round(res["batched"], 2),
+           "speedup": round(res["batched"] / res["plain"], 3), "identical_to_plain": bool(same),
+           "n_l": f"N{args.branching}L{args.leaf}", "graph": True}
+    for bvh, sc, d_r, d_h in scenes:
+        sc.close()
+    return out
+
+
+def bench_serving(rt, args, log):
+    """16 resident scenes (100k..2M tris log-spaced, seeds 200..215), Poisson
+    arrivals at 200 req/s for args.serve_seconds (seed 5), rays per request
+    clamp(LogNormal(ln 2.3e4, 1.3), 1024, 2^19), half shadow (any-hit), half
+    closest-hit; latency from submit to results in host memory."""
+    import threading
+    t0 = time.time()
+    counts = np.round(np.exp(np.linspace(np.log(1e5), np.log(2e6), 16))).astype(np.int64)
+    world = []
+    for i in range(16):
+        tris = rt.interior_scene(200 + i, int(counts[i]))
+        bvh = rt.Bvh(tris, args.branching, args.leaf)
+        sc = rt.Scene(bvh)
+        prim = random_camera_rays(rt, np.random.default_rng(2000 + i), 1 << 19)
+        ph = sc.intersect(prim)
+        nrm, pos = bvh.surface(prim, ph)
+        diff = rt.spawn_diffuse(prim, nrm, pos, ph, 1 << 19, seed=200 + i)
+        shadow, _ = rt.spawn_shadow(nrm, pos, ph, (0.0, 9.5, 0.0), 1.0)
+        shadow = np.resize(shadow, 1 << 19)
+        world.append((bvh, sc, np.concatenate([prim, diff]), shadow))
+    log(f"serving workload: 16 scenes, {int(counts.sum())} tris, setup {time.time() - t0:.1f}s")
+    rng = np.random.default_rng(5)
+    n_req = int(200 * args.serve_seconds)
+    gaps = rng.exponential(1 / 200.0, n_req)
+    sizes = clamp_lognormal(rng, np.log(2.3e4), 1.3, 1024, 1 << 19, n_req)
+    scene_of = rng.integers(0, 16, n_req)
+    is_shadow = rng.random(n_req) < 0.5
+    offs = [int(rng.integers(0, (1 << 20) - sizes[i] + 1)) if not is_shadow[i]
+            else int(rng.integers(0, (1 << 19) - sizes[i] + 1)) for i in range(n_req)]
+    srv = rt.Server(hbm_cap_bytes=8 << 30, slots=3)
+    sids = [srv.add_scene(w[1]) for w in world]
+    # warm the launch path (graph capture) outside the measured window
+    for i in range(8):
+        rid, _ = srv.submit(sids[i], world[i][2][:4096])
+        srv.wait(rid, 30)
+    srv.stats(reset=True)
+    done_q = []
+    lock = threading.Lock()
+
+    def waiter():
+        k = 0
+        while k < n_req:
+            with lock:
+                ready = k < len(done_q)
+                rid = done_q[k] if ready else None
+            if not ready:
+                time.sleep(0.0005)
+                continue
+            srv.wait(rid, 120)
+            k += 1
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    start = time.perf_counter()
+    at = start
+    for i in range(n_req):
+        at += gaps[i]
+        dt = at - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        w = world[scene_of[i]]
+        src = w[3] if is_shadow[i] else w[2]
+        rid, _ = srv.submit(sids[scene_of[i]], src[offs[i]:offs[i] + sizes[i]],
+                            "any" if is_shadow[i] else "closest")
+        with lock:
+            done_q.append(rid)
+    th.join()
+    st = srv.stats()
+    srv.close()
+    for w in world:
+        w[1].close()
+    log(f"serving: {st['completed']} requests, {st['mrays_per_s']:.1f} Mrays/s, p50 {st['p50_ms']:.2f} ms, "
+        f"p99 {st['p99_ms']:.2f} ms, {st['mean_requests_per_launch']:.1f} requests/launch")
+    return {"scenes": 16, "requests": int(st["completed"]), "arrival_rate": 200,
+            "seconds": args.serve_seconds, "offered_mrays_per_s": round(float(sizes.sum()) / args.serve_seconds / 1e6, 2),
+            "mrays_per_s": round(st["mrays_per_s"], 2), "p50_ms": round(st["p50_ms"], 3),
+            "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
+            "requests_per_launch": round(st["mean_requests_per_launch"], 2), "failed": int(st["failed"])}
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -65,6 +225,9 @@ def main():
     ap.add_argument("--leaf", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=2.0, help="target seconds per CPU repetition")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (counter calibration)")
     args = ap.parse_args()
@@ -140,6 +303,9 @@ def main():
         pms.append(scene.stats()["last_kernel_ms"])
     primary_mrays = len(prim) / (float(np.median(pms)) / 1000.0) / 1e6
 
+    batched = None if args.no_batch else bench_batched(rt, torch, args, log)
+    serving = None if args.no_serve else bench_serving(rt, args, log)
+
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -184,6 +350,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

@@ -337,8 +337,9 @@ extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t f
     if (n >= 0x80000000u) return set_error(RT_ERR_ARG, "rt_trace: at most 2^31-1 rays per call");
     if ((flags & RT_ANY) && (flags & RT_SAVE_DIST))
         return set_error(RT_ERR_ARG, "rt_trace: RT_ANY and RT_SAVE_DIST are exclusive");
-    if (flags & (RT_STREAM | RT_COOP))
-        return set_error(RT_ERR_UNSUPPORTED, "rt_trace: stream and cooperative scheduling are not built yet");
+    if (flags & RT_COOP) return set_error(RT_ERR_UNSUPPORTED, "rt_trace: cooperative scheduling is not built yet");
+    if ((flags & RT_STREAM) && (flags & RT_SAVE_DIST))
+        return set_error(RT_ERR_UNSUPPORTED, "rt_trace: the ray-stream traversal has no save-distance form");
     const int mode = (flags & RT_ANY) ? kAny : ((flags & RT_SAVE_DIST) ? kSaveDist : kClosest);
     std::lock_guard<std::mutex> lk(s->mu);
     int prevDev = 0;
@@ -359,10 +360,11 @@ extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t f
     int rc = get_workspace(s, st, w);
     if (rc != RT_OK) return rc;
 
+    const bool stream = (flags & RT_STREAM) != 0;
     const uint32_t S = s->stackWindow;
-    uint32_t bd = s->blockSize;
-    while (bd > 64 && trace_lds_bytes(s->M, mode, S, bd) > 65536) bd -= 64;
-    const int occ = std::max(1, trace_occupancy(s->M, mode, S, bd));
+    uint32_t bd = stream ? stream_block() : s->blockSize;
+    while (!stream && bd > 64 && trace_lds_bytes(s->M, mode, S, bd) > 65536) bd -= 64;
+    const int occ = std::max(1, stream ? stream_occupancy(s->M, mode, S) : trace_occupancy(s->M, mode, S, bd));
     const uint64_t wantBlocks = (uint64_t)s->cus * (uint64_t)occ;
     const uint64_t needBlocks = ((uint64_t)n + bd - 1) / bd;
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min(wantBlocks, needBlocks));
@@ -409,7 +411,8 @@ extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t f
     p.nodeCount = s->nodeCount;
     HIP_TRY(launch_reset(w->work, st), "reset kernel");
     HIP_TRY(hipEventRecord(w->ev0, st), "hipEventRecord");
-    HIP_TRY(launch_trace(s->M, mode, S, bd, grid, p, st), "trace kernel launch");
+    if (stream) HIP_TRY(launch_stream(s->M, mode, S, grid, p, st), "stream kernel launch");
+    else HIP_TRY(launch_trace(s->M, mode, S, bd, grid, p, st), "trace kernel launch");
     HIP_TRY(hipEventRecord(w->ev1, st), "hipEventRecord");
     w->timed = true;
     s->raysTraced += n;

@@ -74,6 +74,12 @@ int trace_occupancy(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSi
 // save-distance mode the stack distances and aabbDistances).
 size_t trace_lds_bytes(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize);
 
+// Ray-stream kernel (closest / any): blocks of stream_block() ray slots.
+hipError_t launch_stream(uint32_t M, int mode, uint32_t stackWindow, uint32_t grid, const KParams& p,
+                         hipStream_t stream);
+int stream_occupancy(uint32_t M, int mode, uint32_t stackWindow);
+uint32_t stream_block();
+
 hipError_t launch_reset(uint32_t* work, hipStream_t stream);
 
 }  // namespace rt

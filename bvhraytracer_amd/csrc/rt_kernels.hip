@@ -82,350 +82,584 @@ __device__ __forceinline__ int max_abs_dim(float x, float y, float z) {
     return (x > y) ? ((x > z) ? 0 : 2) : ((y > z) ? 1 : 2);
 }
 
-// One kernel template for the plain launch (one scene, uniform parameters,
-// node memory MT known at compile time) and the batched launch (BATCH: every
-// lane carries the scene and item of its current ray; MT = 0 reads the node
-// memory from the scene; MODE = kMixed takes closest/any per item).
+// ---------------------------------------------------------------------------
+// Traversal state of one ray and the step function shared by the per-ray
+// kernel (state in registers) and the ray-stream kernel (state in LDS).
+
+struct SceneCtx {
+    const uint32_t* nodes;
+    const float* tris;
+    const int32_t* triIds;
+    const uint32_t* nodeEntry;
+    uint32_t rootEntry, G, Lm, nodeCount, M, recw, ow;
+};
+
+struct StackCtx {
+    uint32_t* stE;   // LDS stack entries of this ray, row stride `stride`
+    float* stD;      // LDS stack distances (save dist)
+    float* aabbD;    // LDS aabbDistances (save dist)
+    uint32_t* spE;   // global spill of this ray
+    float* spD;
+    uint32_t cap;    // stack capacity (LDS window + spill)
+    uint32_t stride;
+};
+
+struct Trav {
+    Ray ray;
+    uint32_t cur;     // entry visited next (the child pushed last)
+    float curDist;
+    uint32_t sp;      // entries below `cur` on the stack
+    uint32_t live;    // slots whose aabbDistances entry is not NaN
+    uint32_t hitEntry;
+    int hitSlot;
+    uint32_t cN, cL, cNS, cLS;
+    bool hasCur, occl, err, anyMode, rev;
+    int axis;
+};
+
+__device__ __forceinline__ void trav_init(Trav& t, const rt_ray& in, uint32_t rootEntry, bool anyMode) {
+    t.ray.tmax = in.tmax;
+    t.ray.ox = in.ox;
+    t.ray.oy = in.oy;
+    t.ray.oz = in.oz;
+    t.ray.ix = in.ix;
+    t.ray.iy = in.iy;
+    t.ray.iz = in.iz;
+    t.ray.dx = in.dx;
+    t.ray.dy = in.dy;
+    t.ray.dz = in.dz;
+    t.axis = max_abs_dim(t.ray.dx, t.ray.dy, t.ray.dz);
+    const float da = t.axis == 0 ? t.ray.dx : (t.axis == 1 ? t.ray.dy : t.ray.dz);
+    t.rev = da <= 0.0f;
+    t.cur = rootEntry;
+    t.curDist = 0.0f;
+    t.hasCur = true;
+    t.sp = 0;
+    t.live = 0;
+    t.hitEntry = 0;
+    t.hitSlot = -1;
+    t.occl = false;
+    t.err = false;
+    t.anyMode = anyMode;
+    t.cN = t.cL = t.cNS = t.cLS = 0;
+}
+
+// Entry the next step visits (without taking it).
+template <int S>
+__device__ __forceinline__ uint32_t trav_peek(const Trav& t, const StackCtx& k) {
+    return t.hasCur ? t.cur : k.stE[((t.sp - 1) & (S - 1)) * k.stride];
+}
+
+__device__ __forceinline__ bool trav_done(const Trav& t) { return !t.hasCur && t.sp == 0; }
+
+// One traversal step: take the next entry and visit it (node or leaf).
+template <int MT, int MODE, int S>
+__device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk, Trav& t) {
+    const uint32_t M = MT ? (uint32_t)MT : sc.M;
+    const uint32_t recw = MT ? record_words(MT) : sc.recw;
+    const uint32_t ow = MT ? order_words(MT) : sc.ow;
+    const uint32_t bd = sk.stride;
+    uint32_t e;
+    float edist = 0.0f;
+    if (t.hasCur) {
+        e = t.cur;
+        edist = t.curDist;
+        t.hasCur = false;
+    } else {
+        --t.sp;
+        const uint32_t slot = (t.sp & (S - 1)) * bd;
+        e = sk.stE[slot];
+        if (MODE == kSaveDist) edist = sk.stD[slot];
+        if (t.sp >= S) {
+            sk.stE[slot] = sk.spE[t.sp - S];
+            if (MODE == kSaveDist) sk.stD[slot] = sk.spD[t.sp - S];
+        }
+    }
+    if (MODE == kSaveDist && edist > t.ray.tmax) return;
+    if (e & kEntryInvalid) {
+        t.err = true;
+        return;
+    }
+    const uint32_t k = ((e >> kEntryKShift) & 3u) * 4u + 4u;
+    const uint32_t idx = e & kEntryIndexMask;
+    if (e & kEntryLeaf) {
+        const float* T = sc.tris + (size_t)idx * leaf_stride(sc.Lm);
+        ++t.cL;
+        t.cLS += k;
+        if (t.anyMode) {
+            bool any = false;
+            for (uint32_t s0 = 0; s0 < k; s0 += 4) {
+                const float4 a0 = ld4(T + s0), a1 = ld4(T + k + s0), a2 = ld4(T + 2 * k + s0);
+                const float4 a3 = ld4(T + 3 * k + s0), a4 = ld4(T + 4 * k + s0), a5 = ld4(T + 5 * k + s0);
+                const float4 a6 = ld4(T + 6 * k + s0), a7 = ld4(T + 7 * k + s0), a8 = ld4(T + 8 * k + s0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float tt, b1, b2;
+                    if (mt(comp(a0, j), comp(a1, j), comp(a2, j), comp(a3, j), comp(a4, j), comp(a5, j),
+                           comp(a6, j), comp(a7, j), comp(a8, j), t.ray, tt, b1, b2))
+                        any = true;
+                }
+            }
+            if (any) {
+                t.occl = true;
+                t.sp = 0;
+                t.hasCur = false;
+            }
+        } else {
+            // gang-G tie rule of triIntersect: each program instance
+            // (slot % G) keeps its first strict minimum, the result is the
+            // largest kept slot among instances at the minimum.
+            float bt = 1000000.0f;
+            int bidx = -1;
+            uint32_t gm = (1u << sc.G) - 1u;  // instances whose tStore == bt
+            for (uint32_t s0 = 0; s0 < k; s0 += 4) {
+                const float4 a0 = ld4(T + s0), a1 = ld4(T + k + s0), a2 = ld4(T + 2 * k + s0);
+                const float4 a3 = ld4(T + 3 * k + s0), a4 = ld4(T + 4 * k + s0), a5 = ld4(T + 5 * k + s0);
+                const float4 a6 = ld4(T + 6 * k + s0), a7 = ld4(T + 7 * k + s0), a8 = ld4(T + 8 * k + s0);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float tt, b1, b2;
+                    if (mt(comp(a0, j), comp(a1, j), comp(a2, j), comp(a3, j), comp(a4, j), comp(a5, j),
+                           comp(a6, j), comp(a7, j), comp(a8, j), t.ray, tt, b1, b2)) {
+                        const uint32_t s = s0 + (uint32_t)j;
+                        const uint32_t g = 1u << (s & (sc.G - 1u));
+                        if (tt < bt) {
+                            bt = tt;
+                            bidx = (int)s;
+                            gm = g;
+                        } else if (tt == bt && !(gm & g)) {
+                            bidx = (int)s;
+                            gm |= g;
+                        }
+                    }
+                }
+            }
+            if (bidx >= 0) {
+                t.ray.tmax = bt;
+                t.hitEntry = e;
+                t.hitSlot = bidx;
+            }
+        }
+        return;
+    }
+    const uint32_t* rec = sc.nodes + (size_t)idx * recw;
+    ++t.cN;
+    t.cNS += k;
+    const uint4 h0 = ld4u(rec + 6 * M);
+    const uint4 h1 = ld4u(rec + 6 * M + 4);
+    uint4 h2 = make_uint4(0, 0, 0, 0);
+    if (ow == 2) h2 = ld4u(rec + 6 * M + 8);
+    uint32_t H = 0;
+    const float* bp = reinterpret_cast<const float*>(rec);
+    for (uint32_t s0 = 0; s0 < k; s0 += 4) {
+        const float4 nx = ld4(bp + s0), ny = ld4(bp + M + s0), nz = ld4(bp + 2 * M + s0);
+        const float4 xx = ld4(bp + 3 * M + s0), xy = ld4(bp + 4 * M + s0), xz = ld4(bp + 5 * M + s0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float t0;
+            if (slab(comp(nx, j), comp(ny, j), comp(nz, j), comp(xx, j), comp(xy, j), comp(xz, j), t.ray, t0)) {
+                H |= 1u << (s0 + (uint32_t)j);
+                if (MODE == kSaveDist) sk.aabbD[(s0 + (uint32_t)j) * bd] = t0;
+            }
+        }
+    }
+    t.live |= H;
+    if (MODE != kSaveDist && H == 0) return;
+    const uint32_t innerBase = h0.x, leafBase = h0.y;
+    const uint32_t leafMask = h0.z & 0xFFFFu, c = (h0.z >> 16) & 31u;
+    const uint32_t kcodes = h0.w, childBase = h1.x;
+    uint64_t row;
+    if (ow == 1) {
+        row = t.axis == 0 ? h1.y : (t.axis == 1 ? h1.z : h1.w);
+    } else {
+        const uint32_t lo = t.axis == 0 ? h1.y : (t.axis == 1 ? h1.w : h2.y);
+        const uint32_t hi = t.axis == 0 ? h1.z : (t.axis == 1 ? h2.x : h2.z);
+        row = ((uint64_t)hi << 32) | lo;
+    }
+    auto entry_of = [&](uint32_t s) -> uint32_t {
+        if (s >= c) {  // padded slot: resolve the FastNode id like the reference
+            const uint32_t id = childBase + s;
+            return id < sc.nodeCount ? sc.nodeEntry[id] : kEntryInvalid;
+        }
+        const uint32_t below = (1u << s) - 1u;
+        const uint32_t kc = ((kcodes >> (2 * s)) & 3u) << kEntryKShift;
+        if ((leafMask >> s) & 1u) return kEntryLeaf | kc | (leafBase + __popc(leafMask & below));
+        return kc | (innerBase + __popc(~leafMask & below));
+    };
+    // push(s): the previous pending child goes to the stack, child s becomes
+    // the pending one (visited next)
+    auto push = [&](uint32_t s) {
+        if (t.hasCur) {
+            if (t.sp >= sk.cap) {
+                t.err = true;
+                return;
+            }
+            const uint32_t sl = (t.sp & (S - 1)) * bd;
+            if (t.sp >= S) {
+                sk.spE[t.sp - S] = sk.stE[sl];
+                if (MODE == kSaveDist) sk.spD[t.sp - S] = sk.stD[sl];
+            }
+            sk.stE[sl] = t.cur;
+            if (MODE == kSaveDist) sk.stD[sl] = t.curDist;
+            ++t.sp;
+        }
+        t.cur = entry_of(s);
+        if (MODE == kSaveDist) {
+            t.curDist = sk.aabbD[s * bd];
+            sk.aabbD[s * bd] = __builtin_nanf("");
+        }
+        t.hasCur = true;
+        t.live &= ~(1u << s);
+    };
+    if (t.anyMode) {
+        // intersectSecondary: slot order 0..M-1 (fastNodeManager.h:1481-1488)
+        for (uint32_t s = 0; s < M; ++s)
+            if ((t.live >> s) & 1u) push(s);
+    } else if (t.rev) {
+        for (uint32_t q = 0; q < c; ++q) {
+            const uint32_t s = (uint32_t)(row >> (4 * q)) & 15u;
+            if ((t.live >> s) & 1u) push(s);
+        }
+    } else {
+        // backward over the zero-padded order row: the padding (slot 0)
+        // comes first when childCount < M
+        if (c < M && (t.live & 1u)) push(0);
+        for (uint32_t q = c; q-- > 0;) {
+            const uint32_t s = (uint32_t)(row >> (4 * q)) & 15u;
+            if ((t.live >> s) & 1u) push(s);
+        }
+    }
+    if (t.err) {
+        t.sp = 0;
+        t.hasCur = false;
+    }
+}
+
+// Writes the result of a finished ray.
+__device__ __forceinline__ void trav_finish(const SceneCtx& sc, const Trav& t, rt_hit* hits, uint8_t* occluded,
+                                            rt_counts* counts, uint32_t r, uint32_t* errCounter) {
+    if (t.anyMode) {
+        occluded[r] = t.occl ? 1 : 0;
+    } else {
+        rt_hit h;
+        if (t.hitSlot >= 0) {
+            const uint32_t ord = t.hitEntry & kEntryIndexMask;
+            const uint32_t k = ((t.hitEntry >> kEntryKShift) & 3u) * 4u + 4u;
+            const float* T = sc.tris + (size_t)ord * leaf_stride(sc.Lm) + t.hitSlot;
+            float tt, b1, b2;
+            (void)mt(T[0], T[k], T[2 * k], T[3 * k], T[4 * k], T[5 * k], T[6 * k], T[7 * k], T[8 * k], t.ray, tt,
+                     b1, b2);
+            h.tri = sc.triIds[(size_t)ord * sc.Lm + t.hitSlot];
+            h.leaf = ord * 9u * sc.Lm;
+            h.slot = t.hitSlot;
+            h.t = t.ray.tmax;
+            h.b1 = b1;
+            h.b2 = b2;
+        } else {
+            h.tri = -1;
+            h.leaf = 0;
+            h.slot = -1;
+            h.t = t.ray.tmax;
+            h.b1 = 0.0f;
+            h.b2 = 0.0f;
+        }
+        hits[r] = h;
+    }
+    if (counts) {
+        rt_counts c;
+        c.nodes = t.cN;
+        c.leaves = t.cL;
+        c.node_slots = t.cNS;
+        c.leaf_slots = t.cLS;
+        counts[r] = c;
+    }
+    if (t.err) atomicAdd(errCounter, 1u);
+}
+
+__device__ __forceinline__ SceneCtx plain_scene(const KParams& p, uint32_t MT) {
+    SceneCtx sc;
+    sc.nodes = p.nodes;
+    sc.tris = p.tris;
+    sc.triIds = p.triIds;
+    sc.nodeEntry = p.nodeEntry;
+    sc.rootEntry = p.rootEntry;
+    sc.G = p.gang;
+    sc.Lm = p.Lm;
+    sc.nodeCount = p.nodeCount;
+    sc.M = MT;
+    sc.recw = record_words(MT);
+    sc.ow = order_words(MT);
+    return sc;
+}
+
+// Wave-aggregated fetch of ray indices for the lanes with want == true:
+// one atomicAdd per wave, lanes ranked by mbcnt.  Returns the lane's index
+// (>= n when the rays are exhausted) and sets `exhausted` wave-uniformly.
+__device__ __forceinline__ uint32_t fetch_rays(uint32_t* counter, uint32_t n, bool want, bool& exhausted) {
+    const uint64_t need = __ballot(want);
+    if (!need) return n;
+    const uint32_t cnt = (uint32_t)__popcll(need);
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need) - 1;
+    uint32_t base = 0;
+    if (__lane_id() == leader) base = atomicAdd(counter, cnt);
+    base = __shfl(base, (int)leader);
+    if (base + cnt >= n) exhausted = true;
+    return want ? base + (uint32_t)__popcll(need & __lanemask_lt()) : n;
+}
+
+// Per-ray kernel.  One template for the plain launch (one scene, uniform
+// parameters, node memory MT known at compile time) and the batched launch
+// (BATCH: every lane carries the scene and item of its current ray; MT = 0
+// reads the node memory from the scene; MODE = kMixed picks closest/any per
+// item).
 template <int MT, int MODE, int S, bool BATCH>
 __global__ void __launch_bounds__(256) k_trace(KParams p) {
     extern __shared__ uint32_t smem[];
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
-    const uint32_t lane = __lane_id();
-    uint32_t* stE = smem + tid;                                        // stack entries
-    float* stD = reinterpret_cast<float*>(smem + S * bd) + tid;        // stack distances (save dist)
-    float* aabbD = reinterpret_cast<float*>(smem + 2 * S * bd) + tid;  // aabbDistances (save dist)
     const size_t gslot = (size_t)blockIdx.x * bd + tid;
     const size_t nslots = (size_t)gridDim.x * bd;
-    uint32_t* spE = p.spill + gslot * p.spillCap;
-    float* spD = reinterpret_cast<float*>(p.spill + nslots * p.spillCap) + gslot * p.spillCap;
-    const uint32_t cap = S + p.spillCap;
+    StackCtx sk;
+    sk.stE = smem + tid;
+    sk.stD = reinterpret_cast<float*>(smem + S * bd) + tid;
+    sk.aabbD = reinterpret_cast<float*>(smem + 2 * S * bd) + tid;
+    sk.spE = p.spill + gslot * p.spillCap;
+    sk.spD = reinterpret_cast<float*>(p.spill + nslots * p.spillCap) + gslot * p.spillCap;
+    sk.cap = S + p.spillCap;
+    sk.stride = bd;
 
-    // scene / item context (uniform for the plain launch, per lane in a batch)
-    const uint32_t* nodes = p.nodes;
-    const float* tris = p.tris;
-    const int32_t* triIds = p.triIds;
-    const uint32_t* nodeEntry = p.nodeEntry;
-    uint32_t rootEntry = p.rootEntry, G = p.gang, Lm = p.Lm, nodeCount = p.nodeCount;
-    uint32_t Mrt = MT, recw = record_words(MT), ow = order_words(MT);
+    SceneCtx sc = plain_scene(p, MT);
     const rt_ray* rays = p.rays;
     rt_hit* hits = p.hits;
     uint8_t* occluded = p.occluded;
     rt_counts* counts = p.counts;
-    bool anyMode = (MODE == kAny);
 
     bool has = false, exhausted = false;
-    // `cur` is the entry visited next.  The child pushed last is always the
-    // next one popped, so it stays in a register instead of going through
-    // the LDS stack; `sp` counts only the entries below it.
-    bool hasCur = false;
-    uint32_t cur = 0;
-    float curDist = 0.0f;
-    uint32_t r = 0, sp = 0, live = 0, hitEntry = 0;
-    int hitSlot = -1;
-    bool occl = false, err = false;
-    uint32_t cN = 0, cL = 0, cNS = 0, cLS = 0;
-    int axis = 0;
-    bool rev = false;
-    Ray ray;
-
+    uint32_t r = 0;
+    Trav t;
+    t.hasCur = false;
+    t.sp = 0;
     while (true) {
         if (!exhausted) {
-            const uint64_t need = __ballot(!has);
-            if (need) {
-                const uint32_t cnt = (uint32_t)__popcll(need);
-                const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need) - 1;
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(p.work, cnt);
-                base = __shfl(base, (int)leader);
-                if (base + cnt >= p.n) exhausted = true;
-                if (!has) {
-                    const uint32_t rr = base + (uint32_t)__popcll(need & __lanemask_lt());
-                    if (rr < p.n) {
-                        r = rr;
-                        if constexpr (BATCH) {
-                            // item of global ray rr: last item with start <= rr
-                            uint32_t lo = 0, hi = p.nItems - 1;
-                            while (lo < hi) {
-                                const uint32_t mid = (lo + hi + 1) >> 1;
-                                if (p.items[mid].start <= rr) lo = mid;
-                                else hi = mid - 1;
-                            }
-                            const BatchItem& it = p.items[lo];
-                            const BatchScene& sc = p.scenes[it.scene];
-                            nodes = sc.nodes;
-                            tris = sc.tris;
-                            triIds = sc.triIds;
-                            nodeEntry = sc.nodeEntry;
-                            rootEntry = sc.rootEntry;
-                            G = sc.gang;
-                            Lm = sc.Lm;
-                            nodeCount = sc.nodeCount;
-                            Mrt = sc.M;
-                            recw = record_words(Mrt);
-                            ow = order_words(Mrt);
-                            rays = it.rays;
-                            hits = static_cast<rt_hit*>(it.out);
-                            occluded = static_cast<uint8_t*>(it.out);
-                            counts = it.counts;
-                            anyMode = it.mode == kAny;
-                            r = rr - it.start;
-                        }
-                        const rt_ray& in = rays[r];
-                        ray.tmax = in.tmax;
-                        ray.ox = in.ox;
-                        ray.oy = in.oy;
-                        ray.oz = in.oz;
-                        ray.ix = in.ix;
-                        ray.iy = in.iy;
-                        ray.iz = in.iz;
-                        ray.dx = in.dx;
-                        ray.dy = in.dy;
-                        ray.dz = in.dz;
-                        axis = max_abs_dim(ray.dx, ray.dy, ray.dz);
-                        const float da = axis == 0 ? ray.dx : (axis == 1 ? ray.dy : ray.dz);
-                        rev = da <= 0.0f;
-                        cur = rootEntry;
-                        curDist = 0.0f;
-                        hasCur = true;
-                        if (MODE == kSaveDist)
-                            for (int s = 0; s < MT; ++s) aabbD[s * bd] = __builtin_nanf("");
-                        sp = 0;
-                        live = 0;
-                        hitEntry = 0;
-                        hitSlot = -1;
-                        occl = false;
-                        err = false;
-                        cN = cL = cNS = cLS = 0;
-                        has = true;
+            const uint32_t rr = fetch_rays(p.work, p.n, !has, exhausted);
+            if (!has && rr < p.n) {
+                r = rr;
+                bool anyMode = (MODE == kAny);
+                if constexpr (BATCH) {
+                    // item of global ray rr: last item with start <= rr
+                    uint32_t lo = 0, hi = p.nItems - 1;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi + 1) >> 1;
+                        if (p.items[mid].start <= rr) lo = mid;
+                        else hi = mid - 1;
                     }
+                    const BatchItem& it = p.items[lo];
+                    const BatchScene& bs = p.scenes[it.scene];
+                    sc.nodes = bs.nodes;
+                    sc.tris = bs.tris;
+                    sc.triIds = bs.triIds;
+                    sc.nodeEntry = bs.nodeEntry;
+                    sc.rootEntry = bs.rootEntry;
+                    sc.G = bs.gang;
+                    sc.Lm = bs.Lm;
+                    sc.nodeCount = bs.nodeCount;
+                    sc.M = bs.M;
+                    sc.recw = record_words(bs.M);
+                    sc.ow = order_words(bs.M);
+                    rays = it.rays;
+                    hits = static_cast<rt_hit*>(it.out);
+                    occluded = static_cast<uint8_t*>(it.out);
+                    counts = it.counts;
+                    anyMode = it.mode == kAny;
+                    r = rr - it.start;
                 }
+                trav_init(t, rays[r], sc.rootEntry, anyMode);
+                if (MODE == kSaveDist)
+                    for (int s = 0; s < MT; ++s) sk.aabbD[s * bd] = __builtin_nanf("");
+                has = true;
             }
         }
         if (__ballot(has) == 0) break;
         if (!has) continue;
-        const uint32_t M = BATCH ? Mrt : (uint32_t)MT;
-        const uint32_t LSTRIDE = leaf_stride(Lm);
-
-        // ---- one traversal step: take the next entry and visit it ----
-        uint32_t e;
-        float edist = 0.0f;
-        if (hasCur) {
-            e = cur;
-            edist = curDist;
-            hasCur = false;
-        } else {
-            --sp;
-            const uint32_t slot = (sp & (S - 1)) * bd;
-            e = stE[slot];
-            if (MODE == kSaveDist) edist = stD[slot];
-            if (sp >= S) {
-                stE[slot] = spE[sp - S];
-                if (MODE == kSaveDist) stD[slot] = spD[sp - S];
-            }
-        }
-        bool visit = true;
-        if (MODE == kSaveDist && edist > ray.tmax) visit = false;
-        if (visit && (e & kEntryInvalid)) {
-            err = true;
-            visit = false;
-        }
-        if (visit) {
-            const uint32_t k = ((e >> kEntryKShift) & 3u) * 4u + 4u;
-            const uint32_t idx = e & kEntryIndexMask;
-            if (e & kEntryLeaf) {
-                const float* T = tris + (size_t)idx * LSTRIDE;
-                ++cL;
-                cLS += k;
-                if (anyMode) {
-                    bool any = false;
-                    for (uint32_t s0 = 0; s0 < k; s0 += 4) {
-                        const float4 a0 = ld4(T + s0), a1 = ld4(T + k + s0), a2 = ld4(T + 2 * k + s0);
-                        const float4 a3 = ld4(T + 3 * k + s0), a4 = ld4(T + 4 * k + s0), a5 = ld4(T + 5 * k + s0);
-                        const float4 a6 = ld4(T + 6 * k + s0), a7 = ld4(T + 7 * k + s0), a8 = ld4(T + 8 * k + s0);
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            float t, b1, b2;
-                            if (mt(comp(a0, j), comp(a1, j), comp(a2, j), comp(a3, j), comp(a4, j), comp(a5, j),
-                                   comp(a6, j), comp(a7, j), comp(a8, j), ray, t, b1, b2))
-                                any = true;
-                        }
-                    }
-                    if (any) {
-                        occl = true;
-                        sp = 0;
-                        hasCur = false;
-                    }
-                } else {
-                    // gang-G tie rule of triIntersect: each program instance
-                    // (slot % G) keeps its first strict minimum, the result is
-                    // the largest kept slot among instances at the minimum.
-                    float bt = 1000000.0f;
-                    int bidx = -1;
-                    uint32_t gm = (1u << G) - 1u;  // instances whose tStore == bt
-                    for (uint32_t s0 = 0; s0 < k; s0 += 4) {
-                        const float4 a0 = ld4(T + s0), a1 = ld4(T + k + s0), a2 = ld4(T + 2 * k + s0);
-                        const float4 a3 = ld4(T + 3 * k + s0), a4 = ld4(T + 4 * k + s0), a5 = ld4(T + 5 * k + s0);
-                        const float4 a6 = ld4(T + 6 * k + s0), a7 = ld4(T + 7 * k + s0), a8 = ld4(T + 8 * k + s0);
-#pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            float t, b1, b2;
-                            if (mt(comp(a0, j), comp(a1, j), comp(a2, j), comp(a3, j), comp(a4, j), comp(a5, j),
-                                   comp(a6, j), comp(a7, j), comp(a8, j), ray, t, b1, b2)) {
-                                const uint32_t s = s0 + (uint32_t)j;
-                                const uint32_t g = 1u << (s & (G - 1u));
-                                if (t < bt) {
-                                    bt = t;
-                                    bidx = (int)s;
-                                    gm = g;
-                                } else if (t == bt && !(gm & g)) {
-                                    bidx = (int)s;
-                                    gm |= g;
-                                }
-                            }
-                        }
-                    }
-                    if (bidx >= 0) {
-                        ray.tmax = bt;
-                        hitEntry = e;
-                        hitSlot = bidx;
-                    }
-                }
-            } else {
-                const uint32_t* rec = nodes + (size_t)idx * recw;
-                ++cN;
-                cNS += k;
-                const uint4 h0 = ld4u(rec + 6 * M);
-                const uint4 h1 = ld4u(rec + 6 * M + 4);
-                uint4 h2 = make_uint4(0, 0, 0, 0);
-                if (ow == 2) h2 = ld4u(rec + 6 * M + 8);
-                uint32_t H = 0;
-                const float* bp = reinterpret_cast<const float*>(rec);
-                for (uint32_t s0 = 0; s0 < k; s0 += 4) {
-                    const float4 nx = ld4(bp + s0), ny = ld4(bp + M + s0), nz = ld4(bp + 2 * M + s0);
-                    const float4 xx = ld4(bp + 3 * M + s0), xy = ld4(bp + 4 * M + s0), xz = ld4(bp + 5 * M + s0);
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        float t0;
-                        if (slab(comp(nx, j), comp(ny, j), comp(nz, j), comp(xx, j), comp(xy, j), comp(xz, j), ray,
-                                 t0)) {
-                            H |= 1u << (s0 + (uint32_t)j);
-                            if (MODE == kSaveDist) aabbD[(s0 + (uint32_t)j) * bd] = t0;
-                        }
-                    }
-                }
-                live |= H;
-                if (MODE == kSaveDist || H != 0) {
-                    const uint32_t innerBase = h0.x, leafBase = h0.y;
-                    const uint32_t leafMask = h0.z & 0xFFFFu, c = (h0.z >> 16) & 31u;
-                    const uint32_t kcodes = h0.w, childBase = h1.x;
-                    uint64_t row;
-                    if (ow == 1) {
-                        row = axis == 0 ? h1.y : (axis == 1 ? h1.z : h1.w);
-                    } else {
-                        const uint32_t lo = axis == 0 ? h1.y : (axis == 1 ? h1.w : h2.y);
-                        const uint32_t hi = axis == 0 ? h1.z : (axis == 1 ? h2.x : h2.z);
-                        row = ((uint64_t)hi << 32) | lo;
-                    }
-                    auto entry_of = [&](uint32_t s) -> uint32_t {
-                        if (s >= c) {  // padded slot: resolve the FastNode id like the reference
-                            const uint32_t id = childBase + s;
-                            return id < nodeCount ? nodeEntry[id] : kEntryInvalid;
-                        }
-                        const uint32_t below = (1u << s) - 1u;
-                        const uint32_t kc = ((kcodes >> (2 * s)) & 3u) << kEntryKShift;
-                        if ((leafMask >> s) & 1u) return kEntryLeaf | kc | (leafBase + __popc(leafMask & below));
-                        return kc | (innerBase + __popc(~leafMask & below));
-                    };
-                    // push(s): the previous pending child goes to the stack,
-                    // child s becomes the pending one (visited next)
-                    auto push = [&](uint32_t s) {
-                        if (hasCur) {
-                            if (sp >= cap) {
-                                err = true;
-                                return;
-                            }
-                            const uint32_t sl = (sp & (S - 1)) * bd;
-                            if (sp >= S) {
-                                spE[sp - S] = stE[sl];
-                                if (MODE == kSaveDist) spD[sp - S] = stD[sl];
-                            }
-                            stE[sl] = cur;
-                            if (MODE == kSaveDist) stD[sl] = curDist;
-                            ++sp;
-                        }
-                        cur = entry_of(s);
-                        if (MODE == kSaveDist) {
-                            curDist = aabbD[s * bd];
-                            aabbD[s * bd] = __builtin_nanf("");
-                        }
-                        hasCur = true;
-                        live &= ~(1u << s);
-                    };
-                    if (anyMode) {
-                        // intersectSecondary: slot order 0..M-1 (fastNodeManager.h:1481-1488)
-                        for (uint32_t s = 0; s < M; ++s)
-                            if ((live >> s) & 1u) push(s);
-                    } else if (rev) {
-                        for (uint32_t q = 0; q < c; ++q) {
-                            const uint32_t s = (uint32_t)(row >> (4 * q)) & 15u;
-                            if ((live >> s) & 1u) push(s);
-                        }
-                    } else {
-                        // backward over the zero-padded order row: the padding
-                        // (slot 0) comes first when childCount < M
-                        if (c < M && (live & 1u)) push(0);
-                        for (uint32_t q = c; q-- > 0;) {
-                            const uint32_t s = (uint32_t)(row >> (4 * q)) & 15u;
-                            if ((live >> s) & 1u) push(s);
-                        }
-                    }
-                    if (err) {
-                        sp = 0;
-                        hasCur = false;
-                    }
-                }
-            }
-        }
-        if (hasCur || sp != 0) continue;
-
-        // ---- ray finished: write its result ----
-        if (anyMode) {
-            occluded[r] = occl ? 1 : 0;
-        } else {
-            rt_hit h;
-            if (hitSlot >= 0) {
-                const uint32_t ord = hitEntry & kEntryIndexMask;
-                const uint32_t k = ((hitEntry >> kEntryKShift) & 3u) * 4u + 4u;
-                const float* T = tris + (size_t)ord * LSTRIDE + hitSlot;
-                float t, b1, b2;
-                (void)mt(T[0], T[k], T[2 * k], T[3 * k], T[4 * k], T[5 * k], T[6 * k], T[7 * k], T[8 * k], ray, t,
-                         b1, b2);
-                h.tri = triIds[(size_t)ord * Lm + hitSlot];
-                h.leaf = ord * 9u * Lm;
-                h.slot = hitSlot;
-                h.t = ray.tmax;
-                h.b1 = b1;
-                h.b2 = b2;
-            } else {
-                h.tri = -1;
-                h.leaf = 0;
-                h.slot = -1;
-                h.t = ray.tmax;
-                h.b1 = 0.0f;
-                h.b2 = 0.0f;
-            }
-            hits[r] = h;
-        }
-        if (counts) {
-            rt_counts c;
-            c.nodes = cN;
-            c.leaves = cL;
-            c.node_slots = cNS;
-            c.leaf_slots = cLS;
-            counts[r] = c;
-        }
-        if (err) atomicAdd(p.work + 1, 1u);
+        trav_step<MT, MODE, S>(sc, sk, t);
+        if (!trav_done(t)) continue;
+        trav_finish(sc, t, hits, occluded, counts, r, p.work + 1);
         has = false;
+    }
+}
+
+// Ray-stream kernel: the GPU form of intersectWideAlternative
+// (fastNodeManager.h:835-1049) with continuous refill (the refill prototype,
+// :277-450).  A block keeps kStreamSlots rays whose traversal state lives
+// in LDS.  Each round the active rays are split into a node list and a leaf
+// list by the type of their next entry; the block then runs every node
+// step, then every leaf step, one ray per lane, so the lanes of a phase run
+// the same code.  Each ray still takes exactly one step per round in its own
+// depth-first order: results and counts equal the per-ray traversal.
+constexpr uint32_t kStreamSlots = 256;
+
+template <int MT, int MODE, int S>
+__global__ void __launch_bounds__(256) k_stream(KParams p) {
+    constexpr uint32_t R = kStreamSlots;
+    extern __shared__ uint32_t smem[];
+    // SoA state per slot
+    float* sRay = reinterpret_cast<float*>(smem);                // 10 * R
+    uint32_t* sCur = smem + 10 * R;
+    uint32_t* sSp = sCur + R;
+    uint32_t* sLive = sSp + R;
+    uint32_t* sHitE = sLive + R;
+    int32_t* sHitS = reinterpret_cast<int32_t*>(sHitE + R);
+    uint32_t* sCnt = sHitE + 2 * R;                              // 4 * R
+    uint32_t* sFlags = sCnt + 4 * R;                             // active|hasCur|occl|err|rev|axis
+    uint32_t* sRid = sFlags + R;
+    uint32_t* sStack = sRid + R;                                 // S * R
+    uint32_t* sNodeList = sStack + S * R;
+    uint32_t* sLeafList = sNodeList + R;
+    uint32_t* sCount = sLeafList + R;                            // [0] nodes, [1] leaves
+    const uint32_t tid = threadIdx.x;
+    const size_t nslots = (size_t)gridDim.x * R;
+    const SceneCtx sc = plain_scene(p, MT);
+
+    auto stack_of = [&](uint32_t slot) {
+        StackCtx k;
+        k.stE = sStack + slot;
+        k.stD = nullptr;
+        k.aabbD = nullptr;
+        k.spE = p.spill + ((size_t)blockIdx.x * R + slot) * p.spillCap;
+        k.spD = nullptr;
+        k.cap = S + p.spillCap;
+        k.stride = R;
+        (void)nslots;
+        return k;
+    };
+    auto load = [&](uint32_t s, Trav& t) {
+        t.ray.tmax = sRay[s];
+        t.ray.ox = sRay[R + s];
+        t.ray.oy = sRay[2 * R + s];
+        t.ray.oz = sRay[3 * R + s];
+        t.ray.ix = sRay[4 * R + s];
+        t.ray.iy = sRay[5 * R + s];
+        t.ray.iz = sRay[6 * R + s];
+        t.ray.dx = sRay[7 * R + s];
+        t.ray.dy = sRay[8 * R + s];
+        t.ray.dz = sRay[9 * R + s];
+        t.cur = sCur[s];
+        t.curDist = 0.0f;
+        t.sp = sSp[s];
+        t.live = sLive[s];
+        t.hitEntry = sHitE[s];
+        t.hitSlot = sHitS[s];
+        t.cN = sCnt[s];
+        t.cL = sCnt[R + s];
+        t.cNS = sCnt[2 * R + s];
+        t.cLS = sCnt[3 * R + s];
+        const uint32_t f = sFlags[s];
+        t.hasCur = f & 2u;
+        t.occl = f & 4u;
+        t.err = f & 8u;
+        t.rev = f & 16u;
+        t.axis = (int)((f >> 5) & 3u);
+        t.anyMode = (MODE == kAny);
+    };
+    auto store = [&](uint32_t s, const Trav& t) {
+        sRay[s] = t.ray.tmax;
+        sCur[s] = t.cur;
+        sSp[s] = t.sp;
+        sLive[s] = t.live;
+        sHitE[s] = t.hitEntry;
+        sHitS[s] = t.hitSlot;
+        sCnt[s] = t.cN;
+        sCnt[R + s] = t.cL;
+        sCnt[2 * R + s] = t.cNS;
+        sCnt[3 * R + s] = t.cLS;
+        sFlags[s] = 1u | (t.hasCur ? 2u : 0u) | (t.occl ? 4u : 0u) | (t.err ? 8u : 0u) | (t.rev ? 16u : 0u) |
+                    ((uint32_t)t.axis << 5);
+    };
+
+    sFlags[tid] = 0;
+    bool exhausted = false;
+    while (true) {
+        // refill the slots this thread owns (slot == tid)
+        const bool free = !(sFlags[tid] & 1u);
+        if (!exhausted) {
+            const uint32_t rr = fetch_rays(p.work, p.n, free, exhausted);
+            if (free && rr < p.n) {
+                Trav t;
+                trav_init(t, p.rays[rr], sc.rootEntry, MODE == kAny);
+                const rt_ray& in = p.rays[rr];
+                sRay[R + tid] = in.ox;
+                sRay[2 * R + tid] = in.oy;
+                sRay[3 * R + tid] = in.oz;
+                sRay[4 * R + tid] = in.ix;
+                sRay[5 * R + tid] = in.iy;
+                sRay[6 * R + tid] = in.iz;
+                sRay[7 * R + tid] = in.dx;
+                sRay[8 * R + tid] = in.dy;
+                sRay[9 * R + tid] = in.dz;
+                sRid[tid] = rr;
+                store(tid, t);
+            }
+        }
+        if (tid < 2) sCount[tid] = 0;
+        __syncthreads();
+        // classify by the type of the next entry
+        const bool active = sFlags[tid] & 1u;
+        bool isNode = false, isLeaf = false;
+        if (active) {
+            const uint32_t f = sFlags[tid];
+            const bool hasCur = f & 2u;
+            const uint32_t e = hasCur ? sCur[tid] : sStack[((sSp[tid] - 1) & (S - 1)) * R + tid];
+            isLeaf = (e & kEntryLeaf) && !(e & kEntryInvalid);
+            isNode = !isLeaf;
+        }
+        {
+            const uint64_t mn = __ballot(isNode), ml = __ballot(isLeaf);
+            const uint32_t lane = __lane_id();
+            uint32_t bn = 0, bl = 0;
+            if (lane == 0) {
+                if (mn) bn = atomicAdd(&sCount[0], (uint32_t)__popcll(mn));
+                if (ml) bl = atomicAdd(&sCount[1], (uint32_t)__popcll(ml));
+            }
+            bn = __shfl(bn, 0);
+            bl = __shfl(bl, 0);
+            const uint64_t lt = __lanemask_lt();
+            if (isNode) sNodeList[bn + __popcll(mn & lt)] = tid;
+            if (isLeaf) sLeafList[bl + __popcll(ml & lt)] = tid;
+        }
+        // both reductions are block barriers and run unconditionally; the
+        // block is done when no slot is active and every wave's fetch is
+        // exhausted
+        const int anyActive = __syncthreads_or(active);
+        const int allExhausted = __syncthreads_and(exhausted);
+        if (!anyActive && allExhausted) break;
+        const uint32_t nNode = sCount[0], nLeaf = sCount[1];
+        // node phase
+        if (tid < nNode) {
+            const uint32_t s = sNodeList[tid];
+            Trav t;
+            load(s, t);
+            trav_step<MT, MODE, S>(sc, stack_of(s), t);
+            store(s, t);
+        }
+        __syncthreads();
+        // leaf phase
+        if (tid < nLeaf) {
+            const uint32_t s = sLeafList[tid];
+            Trav t;
+            load(s, t);
+            trav_step<MT, MODE, S>(sc, stack_of(s), t);
+            store(s, t);
+        }
+        __syncthreads();
+        // retire finished rays of the slot this thread owns
+        if (sFlags[tid] & 1u) {
+            Trav t;
+            load(tid, t);
+            if (trav_done(t)) {
+                trav_finish(sc, t, p.hits, p.occluded, p.counts, sRid[tid], p.work + 1);
+                sFlags[tid] = 0;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -437,6 +671,29 @@ template <int M, int MODE, int S>
 void* kernel_ptr() {
     return reinterpret_cast<void*>(&k_trace<M, MODE, S, false>);
 }
+
+template <int M, int MODE>
+void* pick_stream_s(uint32_t S) {
+    switch (S) {
+        case 8: return reinterpret_cast<void*>(&k_stream<M, MODE, 8>);
+        case 16: return reinterpret_cast<void*>(&k_stream<M, MODE, 16>);
+        case 32: return reinterpret_cast<void*>(&k_stream<M, MODE, 32>);
+        default: return nullptr;
+    }
+}
+
+void* pick_stream(uint32_t M, int mode, uint32_t S) {
+    const bool any = mode == kAny;
+    switch (M) {
+        case 4: return any ? pick_stream_s<4, kAny>(S) : pick_stream_s<4, kClosest>(S);
+        case 8: return any ? pick_stream_s<8, kAny>(S) : pick_stream_s<8, kClosest>(S);
+        case 12: return any ? pick_stream_s<12, kAny>(S) : pick_stream_s<12, kClosest>(S);
+        case 16: return any ? pick_stream_s<16, kAny>(S) : pick_stream_s<16, kClosest>(S);
+        default: return nullptr;
+    }
+}
+
+size_t stream_lds(uint32_t S) { return (size_t)(10 + 11 + S + 2) * kStreamSlots * 4 + 16; }
 
 void* pick_batch(uint32_t S) {
     switch (S) {
@@ -506,6 +763,27 @@ int trace_occupancy(uint32_t M, int mode, uint32_t S, uint32_t bd) {
         return 0;
     return blocks;
 }
+
+hipError_t launch_stream(uint32_t M, int mode, uint32_t S, uint32_t grid, const KParams& p, hipStream_t stream) {
+    if (mode == kSaveDist) return hipErrorInvalidValue;
+    void* fn = pick_stream(M, mode, S);
+    if (!fn) return hipErrorInvalidValue;
+    KParams copy = p;
+    void* args[] = {&copy};
+    return hipLaunchKernel(fn, dim3(grid), dim3(kStreamSlots), args, stream_lds(S), stream);
+}
+
+int stream_occupancy(uint32_t M, int mode, uint32_t S) {
+    void* fn = pick_stream(M, mode, S);
+    if (!fn) return 0;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(fn), (int)kStreamSlots,
+                                                     stream_lds(S)) != hipSuccess)
+        return 0;
+    return blocks;
+}
+
+uint32_t stream_block() { return kStreamSlots; }
 
 hipError_t launch_reset(uint32_t* work, hipStream_t stream) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, work);

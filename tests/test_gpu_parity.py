@@ -120,3 +120,47 @@ def test_edge_cases(rt):
     ro = oracle_for(bvh).trace(rays, mode=oracle.ORC_ANY)
     assert np.array_equal(occ, ro["occluded"])
     assert np.array_equal(oc["nodes"], ro["counts"]["nodes"])
+
+
+@pytest.mark.parametrize("N,L", [(4, 4), (8, 8), (16, 16), (12, 4)])
+def test_stream_scheduling_matches_per_ray(rt, N, L):
+    """RT_STREAM (node phase / leaf phase over a block's ray slots) keeps each
+    ray's own depth-first order: same hits and counts as the oracle."""
+    tris = rt.interior_scene(17, 50_000, 25)
+    bvh = rt.Bvh(tris, N, L)
+    prim = rt.camera_rays((0.0, 5.0, -14.0), (0.0, 3.0, 0.0), 128, 96, 16)
+    scene = rt.Scene(bvh)
+    hits = scene.intersect(prim)
+    nrm, pos = bvh.surface(prim, hits)
+    rays = np.concatenate([prim, rt.spawn_diffuse(prim, nrm, pos, hits, 30000, seed=9)])
+    print(f"stream N{N}L{L}: {len(rays)} rays", flush=True)
+    g, gc = scene.intersect(rays, counts=True, stream=True)
+    ref = oracle_for(bvh).trace(rays)
+    assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], f"stream N{N}L{L}")
+    srays, _ = rt.spawn_shadow(nrm, pos, hits, (0.0, 9.5, 0.0), 1.0)
+    occ, oc = scene.intersect(srays, mode="any", counts=True, stream=True)
+    ro = oracle_for(bvh).trace(srays, mode=oracle.ORC_ANY)
+    assert np.array_equal(occ, ro["occluded"])
+    assert np.array_equal(oc["nodes"], ro["counts"]["nodes"])
+    assert np.array_equal(oc["leaves"], ro["counts"]["leaves"])
+
+
+def test_stream_edge_cases(rt):
+    tri = np.array([[-1, -1, 5, 1, -1, 5, 0, 1, 5]], np.float32)
+    bvh = rt.Bvh(tri, 4, 4)  # root is a leaf
+    rays = rt.make_rays([[0, 0, 0], [0, 0, 0], [5, 5, 0]], [[0, 0, 1], [0, 0, -1], [5, 5, 1]])
+    g, gc = rt.Scene(bvh).intersect(rays, counts=True, stream=True)
+    ref = oracle_for(bvh).trace(rays)
+    assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], "stream root-leaf")
+    # fewer rays than one block of slots, and origin rays (stale zero boxes)
+    tris = rt.interior_scene(3, 2000, 4) - np.tile(np.float32([0, 5, 0]), 3)[None, :]
+    bvh = rt.Bvh(tris, 6, 4)
+    dirs = np.random.default_rng(0).normal(size=(100, 3)).astype(np.float32)
+    rays = rt.make_rays(np.zeros((100, 3), np.float32), dirs)
+    scene = rt.Scene(bvh)
+    g, gc = scene.intersect(rays, counts=True, stream=True)
+    ref = oracle_for(bvh).trace(rays)
+    assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], "stream origin rays")
+    occ, oc = scene.intersect(rays, mode="any", counts=True, stream=True)
+    ro = oracle_for(bvh).trace(rays, mode=oracle.ORC_ANY)
+    assert np.array_equal(occ, ro["occluded"]) and np.array_equal(oc["nodes"], ro["counts"]["nodes"])
