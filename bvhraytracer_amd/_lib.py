@@ -154,6 +154,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: run bvhraytracer_amd/build.py "
                               "(or __graft_entry__.build()) first")
+        # PyTorch-ROCm ships its own libamdhip64.so (soname libamdhip64.so.7).
+        # Loading torch first makes librt bind to that same runtime by soname,
+        # so device memory, streams and kernels share one HIP runtime; loading
+        # librt first would bring a second HIP/HSA runtime into the process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             f = getattr(L, name)

@@ -1,0 +1,96 @@
+"""Per-scene autotuner: N, L, workgroup size and LDS stack window.
+
+The reference picks N and L by sweeping them offline and keeps the best SAH
+node factor per (N, L) in a table (sahFactorConfigSse.txt, loaded by
+RayTracer::loadSahFactor, rayTracer.cpp:880-925).  Here the sweep is done per
+scene on the GPU: every candidate (N, L) is built with the reference builder
+and uploaded, and every launch shape is timed on a sample of the scene's own
+rays; the fastest configuration wins.  Results are cached by a content hash
+of the triangles and the ray sample, so a scene is measured once.
+"""
+import hashlib
+import json
+import os
+import time
+
+import numpy as np
+
+from .host import Bvh, Scene
+
+DEFAULT_NL = ((4, 4), (8, 8), (8, 4), (16, 8), (4, 8), (16, 16))
+DEFAULT_BLOCKS = (64, 128, 256)
+DEFAULT_WINDOWS = (8, 16, 32)
+
+
+def scene_key(tris, rays):
+    h = hashlib.sha1()
+    h.update(np.ascontiguousarray(tris).view(np.uint8))
+    h.update(np.ascontiguousarray(rays).view(np.uint8))
+    return h.hexdigest()
+
+
+class Autotuner:
+    def __init__(self, cache_path=None, nl=DEFAULT_NL, blocks=DEFAULT_BLOCKS, windows=DEFAULT_WINDOWS,
+                 sample=1 << 16, reps=3):
+        self.cache_path = cache_path
+        self.nl, self.blocks, self.windows = tuple(nl), tuple(blocks), tuple(windows)
+        self.sample, self.reps = int(sample), int(reps)
+        self.cache = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                self.cache = json.load(f)
+
+    def save(self):
+        if self.cache_path:
+            os.makedirs(os.path.dirname(os.path.abspath(self.cache_path)), exist_ok=True)
+            with open(self.cache_path, "w") as f:
+                json.dump(self.cache, f, indent=1, sort_keys=True)
+
+    def _time(self, scene, d_rays, d_out, mode):
+        best = float("inf")
+        for _ in range(self.reps):
+            scene.intersect_device(d_rays, d_out, mode=mode)
+            best = min(best, scene.stats()["last_kernel_ms"])
+        return best
+
+    def tune_launch(self, scene, d_rays, d_out, mode="closest"):
+        """Fastest (block size, stack window) for this scene and ray set."""
+        res = []
+        for w in self.windows:
+            for b in self.blocks:
+                scene.set_launch(b, w)
+                res.append((self._time(scene, d_rays, d_out, mode), b, w))
+        ms, b, w = min(res)
+        scene.set_launch(b, w)
+        return {"block": b, "window": w, "ms": ms}
+
+    def tune_scene(self, tris, rays, mode="closest", key=None):
+        """Returns {"N", "L", "block", "window", "mrays", "tried"} for the scene.
+        rays: host RAY_DTYPE array; a sample of `sample` rays is used."""
+        import torch
+
+        sample = rays[: self.sample]
+        key = key or scene_key(tris, sample) + f":{mode}"
+        if key in self.cache:
+            return dict(self.cache[key], cached=True)
+        d_rays = torch.from_numpy(np.ascontiguousarray(sample).view(np.float32).reshape(-1, 10)).cuda()
+        if mode == "any":
+            d_out = torch.empty(len(sample), dtype=torch.uint8, device="cuda")
+        else:
+            d_out = torch.empty((len(sample), 6), dtype=torch.int32, device="cuda")
+        t0 = time.time()
+        best, tried = None, []
+        for N, L in self.nl:
+            bvh = Bvh(tris, N, L)
+            scene = Scene(bvh)
+            r = self.tune_launch(scene, d_rays, d_out, mode)
+            mrays = len(sample) / r["ms"] / 1e3
+            tried.append({"N": N, "L": L, "block": r["block"], "window": r["window"], "mrays": round(mrays, 2)})
+            if best is None or mrays > best["mrays"]:
+                best = {"N": N, "L": L, "block": r["block"], "window": r["window"], "mrays": round(mrays, 2)}
+            scene.close()
+            bvh.close()
+        best["tried"] = tried
+        best["seconds"] = round(time.time() - t0, 2)
+        self.cache[key] = best
+        return dict(best, cached=False)

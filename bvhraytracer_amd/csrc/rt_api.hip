@@ -337,7 +337,8 @@ extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t f
     if (n >= 0x80000000u) return set_error(RT_ERR_ARG, "rt_trace: at most 2^31-1 rays per call");
     if ((flags & RT_ANY) && (flags & RT_SAVE_DIST))
         return set_error(RT_ERR_ARG, "rt_trace: RT_ANY and RT_SAVE_DIST are exclusive");
-    if (flags & RT_COOP) return set_error(RT_ERR_UNSUPPORTED, "rt_trace: cooperative scheduling is not built yet");
+    if ((flags & RT_COOP) && (flags & (RT_SAVE_DIST | RT_STREAM)))
+        return set_error(RT_ERR_UNSUPPORTED, "rt_trace: the cooperative traversal has no save-distance or stream form");
     if ((flags & RT_STREAM) && (flags & RT_SAVE_DIST))
         return set_error(RT_ERR_UNSUPPORTED, "rt_trace: the ray-stream traversal has no save-distance form");
     const int mode = (flags & RT_ANY) ? kAny : ((flags & RT_SAVE_DIST) ? kSaveDist : kClosest);
@@ -361,12 +362,20 @@ extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t f
     if (rc != RT_OK) return rc;
 
     const bool stream = (flags & RT_STREAM) != 0;
+    const bool coop = (flags & RT_COOP) != 0;
     const uint32_t S = s->stackWindow;
-    uint32_t bd = stream ? stream_block() : s->blockSize;
-    while (!stream && bd > 64 && trace_lds_bytes(s->M, mode, S, bd) > 65536) bd -= 64;
-    const int occ = std::max(1, stream ? stream_occupancy(s->M, mode, S) : trace_occupancy(s->M, mode, S, bd));
+    const uint32_t W = coop_width(s->M, s->Lm);
+    const uint32_t coopDepth = (stack_capacity(s) + 63) / 64 * 64;
+    if (coop && (size_t)(256 / W) * coopDepth * 4 > 65536)
+        return set_error(RT_ERR_UNSUPPORTED, "rt_trace: tree too deep for the cooperative kernel's LDS stack");
+    uint32_t bd = (stream || coop) ? 256u : s->blockSize;
+    while (!stream && !coop && bd > 64 && trace_lds_bytes(s->M, mode, S, bd) > 65536) bd -= 64;
+    const int occ = std::max(1, stream ? stream_occupancy(s->M, mode, S)
+                                       : (coop ? coop_occupancy(W, mode, coopDepth)
+                                               : trace_occupancy(s->M, mode, S, bd)));
     const uint64_t wantBlocks = (uint64_t)s->cus * (uint64_t)occ;
-    const uint64_t needBlocks = ((uint64_t)n + bd - 1) / bd;
+    const uint64_t raysPerBlock = coop ? bd / W : bd;
+    const uint64_t needBlocks = ((uint64_t)n + raysPerBlock - 1) / raysPerBlock;
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min(wantBlocks, needBlocks));
     const uint32_t cap = stack_capacity(s);
     const uint32_t spillCap = cap > S ? cap - S : 0;
@@ -409,9 +418,12 @@ extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t f
     p.gang = s->G;
     p.Lm = s->Lm;
     p.nodeCount = s->nodeCount;
+    p.M = s->M;
+    p.coopDepth = coopDepth;
     HIP_TRY(launch_reset(w->work, st), "reset kernel");
     HIP_TRY(hipEventRecord(w->ev0, st), "hipEventRecord");
-    if (stream) HIP_TRY(launch_stream(s->M, mode, S, grid, p, st), "stream kernel launch");
+    if (coop) HIP_TRY(launch_coop(W, mode, grid, p, st), "cooperative kernel launch");
+    else if (stream) HIP_TRY(launch_stream(s->M, mode, S, grid, p, st), "stream kernel launch");
     else HIP_TRY(launch_trace(s->M, mode, S, bd, grid, p, st), "trace kernel launch");
     HIP_TRY(hipEventRecord(w->ev1, st), "hipEventRecord");
     w->timed = true;

@@ -56,6 +56,9 @@ struct KParams {
     uint32_t gang;
     uint32_t Lm;
     uint32_t nodeCount;
+    // cooperative launch only: node memory and LDS stack entries per ray
+    uint32_t M;
+    uint32_t coopDepth;
     // batched launch only
     const BatchItem* items;
     const BatchScene* scenes;
@@ -79,6 +82,11 @@ hipError_t launch_stream(uint32_t M, int mode, uint32_t stackWindow, uint32_t gr
                          hipStream_t stream);
 int stream_occupancy(uint32_t M, int mode, uint32_t stackWindow);
 uint32_t stream_block();
+
+// Cooperative kernel (closest / any): 256-thread blocks, W lanes per ray.
+uint32_t coop_width(uint32_t M, uint32_t Lm);
+hipError_t launch_coop(uint32_t W, int mode, uint32_t grid, const KParams& p, hipStream_t stream);
+int coop_occupancy(uint32_t W, int mode, uint32_t depth);
 
 hipError_t launch_reset(uint32_t* work, hipStream_t stream);
 

@@ -663,6 +663,210 @@ __global__ void __launch_bounds__(256) k_stream(KParams p) {
     }
 }
 
+// Cooperative kernel: W lanes (a power of two >= max(M, Lm)) trace one ray.
+// Lane j tests child slot j of a node / triangle j of a leaf; a ballot gives
+// the hit mask.  The pushes are placed in parallel: lane q takes position q
+// of the ray's order row, and a prefix popcount of the live positions gives
+// its index in the reference's push sequence (forward, backward with the
+// padding first, or slot order for any-hit).  Each group's stack lives in
+// LDS, `depth` entries deep, so no spill is needed.  The gang-G tie rule is a
+// group min-reduction plus, per program instance (slot % G), the lowest slot
+// at the minimum; the result is the highest of those.
+template <int W, int MODE>
+__global__ void __launch_bounds__(256) k_coop(KParams p) {
+    extern __shared__ uint32_t smem[];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint32_t gl = lane & (W - 1);
+    const uint32_t gbase = lane & ~(uint32_t)(W - 1);
+    const uint32_t wmask = (W == 32) ? 0xFFFFFFFFu : ((1u << W) - 1u);
+    const uint32_t D = p.coopDepth;
+    uint32_t* stk = smem + (tid / W) * D;
+    const uint32_t M = p.M, G = p.gang;
+    SceneCtx sc = plain_scene(p, 0);
+    sc.M = M;
+    sc.recw = record_words(M);
+    sc.ow = order_words(M);
+    const uint32_t LSTRIDE = leaf_stride(sc.Lm);
+    // slots of the same program instance as this lane's slot
+    uint32_t resMask = 0;
+    for (uint32_t j = gl % G; j < (uint32_t)W; j += G) resMask |= 1u << j;
+
+    bool has = false, exhausted = false;
+    uint32_t r = 0, sp = 0, live = 0, hitEntry = 0;
+    int hitSlot = -1;
+    bool occl = false, err = false, rev = false;
+    int axis = 0;
+    uint32_t cN = 0, cL = 0, cNS = 0, cLS = 0;
+    Ray ray;
+    auto group_ballot = [&](bool v) -> uint32_t { return (uint32_t)(__ballot(v) >> gbase) & wmask; };
+
+    while (true) {
+        if (!exhausted) {
+            uint32_t rr = fetch_rays(p.work, p.n, gl == 0 && !has, exhausted);
+            rr = __shfl(rr, (int)gbase);
+            if (!has && rr < p.n) {
+                r = rr;
+                const rt_ray& in = p.rays[r];
+                ray.tmax = in.tmax;
+                ray.ox = in.ox;
+                ray.oy = in.oy;
+                ray.oz = in.oz;
+                ray.ix = in.ix;
+                ray.iy = in.iy;
+                ray.iz = in.iz;
+                ray.dx = in.dx;
+                ray.dy = in.dy;
+                ray.dz = in.dz;
+                axis = max_abs_dim(ray.dx, ray.dy, ray.dz);
+                const float da = axis == 0 ? ray.dx : (axis == 1 ? ray.dy : ray.dz);
+                rev = da <= 0.0f;
+                if (gl == 0) stk[0] = sc.rootEntry;
+                sp = 1;
+                live = 0;
+                hitEntry = 0;
+                hitSlot = -1;
+                occl = false;
+                err = false;
+                cN = cL = cNS = cLS = 0;
+                has = true;
+            }
+        }
+        if (__ballot(has) == 0) break;
+        if (!has) continue;
+
+        --sp;
+        const uint32_t e = stk[sp];
+        if (e & kEntryInvalid) {
+            err = true;
+        } else if (e & kEntryLeaf) {
+            const uint32_t k = ((e >> kEntryKShift) & 3u) * 4u + 4u;
+            const float* T = sc.tris + (size_t)(e & kEntryIndexMask) * LSTRIDE + gl;
+            ++cL;
+            cLS += k;
+            bool hit = false;
+            float t = 0.0f, b1, b2;
+            if (gl < k)
+                hit = mt(T[0], T[k], T[2 * k], T[3 * k], T[4 * k], T[5 * k], T[6 * k], T[7 * k], T[8 * k], ray, t, b1,
+                         b2);
+            if (MODE == kAny) {
+                if (group_ballot(hit)) {
+                    occl = true;
+                    sp = 0;
+                }
+            } else {
+                const bool rec = hit && t < 1000000.0f;  // tStore starts at 1e6
+                float best = rec ? t : 1000000.0f;
+#pragma unroll
+                for (int off = W / 2; off > 0; off >>= 1) {
+                    const float o = __shfl_xor(best, off);
+                    best = (o < best) ? o : best;
+                }
+                if (best < 1000000.0f) {
+                    const bool flag = rec && t == best;
+                    const uint32_t F = group_ballot(flag);
+                    const bool first = flag && (F & resMask & ((1u << gl) - 1u)) == 0;
+                    const uint32_t Fi = group_ballot(first);
+                    ray.tmax = best;
+                    hitEntry = e;
+                    hitSlot = 31 - __clz(Fi);
+                }
+            }
+        } else {
+            const uint32_t k = ((e >> kEntryKShift) & 3u) * 4u + 4u;
+            const uint32_t* rec = sc.nodes + (size_t)(e & kEntryIndexMask) * sc.recw;
+            ++cN;
+            cNS += k;
+            bool hit = false;
+            if (gl < k) {
+                const float* bp = reinterpret_cast<const float*>(rec) + gl;
+                float t0;
+                hit = slab(bp[0], bp[M], bp[2 * M], bp[3 * M], bp[4 * M], bp[5 * M], ray, t0);
+            }
+            const uint32_t H = group_ballot(hit);
+            live |= H;
+            if (H) {
+                const uint4 h0 = ld4u(rec + 6 * M);
+                const uint4 h1 = ld4u(rec + 6 * M + 4);
+                uint4 h2 = make_uint4(0, 0, 0, 0);
+                if (sc.ow == 2) h2 = ld4u(rec + 6 * M + 8);
+                const uint32_t innerBase = h0.x, leafBase = h0.y;
+                const uint32_t leafMask = h0.z & 0xFFFFu, c = (h0.z >> 16) & 31u;
+                const uint32_t kcodes = h0.w, childBase = h1.x;
+                uint64_t row;
+                if (sc.ow == 1) {
+                    row = axis == 0 ? h1.y : (axis == 1 ? h1.z : h1.w);
+                } else {
+                    const uint32_t lo = axis == 0 ? h1.y : (axis == 1 ? h1.w : h2.y);
+                    const uint32_t hi = axis == 0 ? h1.z : (axis == 1 ? h2.x : h2.z);
+                    row = ((uint64_t)hi << 32) | lo;
+                }
+                // lane gl plays position gl of the push sequence source
+                uint32_t s;
+                bool valid;
+                if (MODE == kAny) {
+                    s = gl;
+                    valid = gl < M;
+                } else {
+                    valid = gl < c;
+                    s = valid ? (uint32_t)(row >> (4 * gl)) & 15u : 0u;
+                }
+                const bool liveAt = valid && ((live >> s) & 1u);
+                const uint32_t P = group_ballot(liveAt);
+                uint32_t idx, T;
+                if (MODE == kAny || rev) {
+                    idx = __popc(P & ((1u << gl) - 1u));
+                    T = __popc(P);
+                } else {
+                    const bool pad0 = c < M && (live & 1u);
+                    const uint32_t Q0 = group_ballot(valid && s == 0);
+                    const uint32_t Pp = pad0 ? (P & ~Q0) : P;
+                    if (pad0 && ((Q0 >> gl) & 1u)) idx = 0;
+                    else idx = (pad0 ? 1u : 0u) + __popc(Pp & ~((2u << gl) - 1u));
+                    T = __popc(Pp) + (pad0 ? 1u : 0u);
+                }
+                if (sp + T > D) {
+                    err = true;
+                    sp = 0;
+                } else {
+                    if (liveAt) {
+                        uint32_t en;
+                        if (s >= c) {
+                            const uint32_t id = childBase + s;
+                            en = id < sc.nodeCount ? sc.nodeEntry[id] : kEntryInvalid;
+                        } else {
+                            const uint32_t below = (1u << s) - 1u;
+                            const uint32_t kc = ((kcodes >> (2 * s)) & 3u) << kEntryKShift;
+                            en = ((leafMask >> s) & 1u) ? (kEntryLeaf | kc | (leafBase + __popc(leafMask & below)))
+                                                        : (kc | (innerBase + __popc(~leafMask & below)));
+                        }
+                        stk[sp + idx] = en;
+                    }
+                    sp += T;
+                    const uint32_t pushedRange = (MODE == kAny) ? M : c;
+                    live &= ~((pushedRange >= 32) ? 0xFFFFFFFFu : ((1u << pushedRange) - 1u));
+                }
+            }
+        }
+        if (sp != 0) continue;
+        if (gl == 0) {
+            Trav t;
+            t.ray = ray;
+            t.hitEntry = hitEntry;
+            t.hitSlot = hitSlot;
+            t.cN = cN;
+            t.cL = cL;
+            t.cNS = cNS;
+            t.cLS = cLS;
+            t.occl = occl;
+            t.err = err;
+            t.anyMode = MODE == kAny;
+            trav_finish(sc, t, p.hits, p.occluded, p.counts, r, p.work + 1);
+        }
+        has = false;
+    }
+}
+
 __global__ void k_reset(uint32_t* work) {
     if (threadIdx.x < 4) work[threadIdx.x] = 0;
 }
@@ -689,6 +893,16 @@ void* pick_stream(uint32_t M, int mode, uint32_t S) {
         case 8: return any ? pick_stream_s<8, kAny>(S) : pick_stream_s<8, kClosest>(S);
         case 12: return any ? pick_stream_s<12, kAny>(S) : pick_stream_s<12, kClosest>(S);
         case 16: return any ? pick_stream_s<16, kAny>(S) : pick_stream_s<16, kClosest>(S);
+        default: return nullptr;
+    }
+}
+
+void* pick_coop(uint32_t W, int mode) {
+    const bool any = mode == kAny;
+    switch (W) {
+        case 4: return any ? reinterpret_cast<void*>(&k_coop<4, kAny>) : reinterpret_cast<void*>(&k_coop<4, kClosest>);
+        case 8: return any ? reinterpret_cast<void*>(&k_coop<8, kAny>) : reinterpret_cast<void*>(&k_coop<8, kClosest>);
+        case 16: return any ? reinterpret_cast<void*>(&k_coop<16, kAny>) : reinterpret_cast<void*>(&k_coop<16, kClosest>);
         default: return nullptr;
     }
 }
@@ -784,6 +998,30 @@ int stream_occupancy(uint32_t M, int mode, uint32_t S) {
 }
 
 uint32_t stream_block() { return kStreamSlots; }
+
+uint32_t coop_width(uint32_t M, uint32_t Lm) {
+    const uint32_t m = M > Lm ? M : Lm;
+    return m <= 4 ? 4u : (m <= 8 ? 8u : 16u);
+}
+
+hipError_t launch_coop(uint32_t W, int mode, uint32_t grid, const KParams& p, hipStream_t stream) {
+    if (mode == kSaveDist || mode == kMixed) return hipErrorInvalidValue;
+    void* fn = pick_coop(W, mode);
+    if (!fn) return hipErrorInvalidValue;
+    KParams copy = p;
+    void* args[] = {&copy};
+    return hipLaunchKernel(fn, dim3(grid), dim3(256), args, (size_t)(256 / W) * p.coopDepth * 4, stream);
+}
+
+int coop_occupancy(uint32_t W, int mode, uint32_t depth) {
+    void* fn = pick_coop(W, mode);
+    if (!fn) return 0;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(fn), 256,
+                                                     (size_t)(256 / W) * depth * 4) != hipSuccess)
+        return 0;
+    return blocks;
+}
 
 hipError_t launch_reset(uint32_t* work, hipStream_t stream) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, work);

@@ -164,3 +164,54 @@ def test_stream_edge_cases(rt):
     occ, oc = scene.intersect(rays, mode="any", counts=True, stream=True)
     ro = oracle_for(bvh).trace(rays, mode=oracle.ORC_ANY)
     assert np.array_equal(occ, ro["occluded"]) and np.array_equal(oc["nodes"], ro["counts"]["nodes"])
+
+
+@pytest.mark.parametrize("N,L,G", [(4, 4, 4), (8, 8, 4), (16, 16, 4), (4, 16, 4), (12, 8, 4), (8, 8, 8)])
+def test_cooperative_matches_oracle(rt, N, L, G):
+    """RT_COOP (W lanes per ray, ballot + prefix-popcount pushes, group
+    reduction for the gang-G tie rule): same hits and counts as the oracle."""
+    tris = rt.interior_scene(19, 50_000, 25)
+    bvh = rt.Bvh(tris, N, L, gang=G)
+    prim = rt.camera_rays((0.0, 5.0, -14.0), (0.0, 3.0, 0.0), 128, 96, 16)
+    scene = rt.Scene(bvh)
+    hits = scene.intersect(prim)
+    nrm, pos = bvh.surface(prim, hits)
+    rays = np.concatenate([prim, rt.spawn_diffuse(prim, nrm, pos, hits, 30000, seed=11)])
+    print(f"coop N{N}L{L}G{G}: {len(rays)} rays", flush=True)
+    g, gc = scene.intersect(rays, counts=True, coop=True)
+    ref = oracle_for(bvh).trace(rays)
+    assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], f"coop N{N}L{L}")
+    srays, _ = rt.spawn_shadow(nrm, pos, hits, (0.0, 9.5, 0.0), 1.0)
+    occ, oc = scene.intersect(srays, mode="any", counts=True, coop=True)
+    ro = oracle_for(bvh).trace(srays, mode=oracle.ORC_ANY)
+    assert np.array_equal(occ, ro["occluded"])
+    assert np.array_equal(oc["nodes"], ro["counts"]["nodes"])
+    assert np.array_equal(oc["leaves"], ro["counts"]["leaves"])
+
+
+def test_cooperative_edge_cases(rt):
+    tri = np.array([[-1, -1, 5, 1, -1, 5, 0, 1, 5]], np.float32)
+    bvh = rt.Bvh(tri, 4, 4)
+    rays = rt.make_rays([[0, 0, 0], [0, 0, 0], [5, 5, 0]], [[0, 0, 1], [0, 0, -1], [5, 5, 1]])
+    g, gc = rt.Scene(bvh).intersect(rays, counts=True, coop=True)
+    ref = oracle_for(bvh).trace(rays)
+    assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], "coop root-leaf")
+    tris = rt.interior_scene(3, 2000, 4) - np.tile(np.float32([0, 5, 0]), 3)[None, :]
+    bvh = rt.Bvh(tris, 6, 4)
+    dirs = np.random.default_rng(0).normal(size=(3000, 3)).astype(np.float32)
+    rays = rt.make_rays(np.zeros((3000, 3), np.float32), dirs)
+    scene = rt.Scene(bvh)
+    g, gc = scene.intersect(rays, counts=True, coop=True)
+    ref = oracle_for(bvh).trace(rays)
+    assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], "coop origin rays")
+    occ, oc = scene.intersect(rays, mode="any", counts=True, coop=True)
+    ro = oracle_for(bvh).trace(rays, mode=oracle.ORC_ANY)
+    assert np.array_equal(occ, ro["occluded"]) and np.array_equal(oc["nodes"], ro["counts"]["nodes"])
+    # exact ties inside a leaf resolved by the gang rule: duplicated triangles
+    base = rt.interior_scene(4, 500, 2)
+    dup = np.concatenate([base, base, base])
+    bvh = rt.Bvh(dup, 8, 8)
+    prim = rt.camera_rays((0.0, 5.0, -14.0), (0.0, 3.0, 0.0), 64, 64, 16)
+    g, gc = rt.Scene(bvh).intersect(prim, counts=True, coop=True)
+    ref = oracle_for(bvh).trace(prim)
+    assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], "coop ties")
