@@ -237,6 +237,15 @@ void free_scene(rt_scene* s) {
     delete s;
 }
 
+// Rays a wave reserves per atomicAdd: about 1/8 of an even share of the
+// launch, between 64 (one per lane) and 1024.
+uint32_t ray_chunk(uint64_t n, uint64_t grid, uint32_t bd) {
+    const uint64_t waves = std::max<uint64_t>(1, grid * (bd / 64));
+    uint64_t c = n / (waves * 8);
+    c = (c + 63) / 64 * 64;
+    return (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(64, c));
+}
+
 // Stack capacity per ray: every level keeps at most M-1 pending siblings
 // plus the children of the node being expanded.
 uint32_t stack_capacity(const rt_scene* s) { return ((1 + (s->maxDepth + 1) * s->M + 8) + 7) / 8 * 8; }
@@ -420,6 +429,7 @@ extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t f
     p.nodeCount = s->nodeCount;
     p.M = s->M;
     p.coopDepth = coopDepth;
+    p.chunk = ray_chunk(n, grid, bd);
     HIP_TRY(launch_reset(w->work, st), "reset kernel");
     HIP_TRY(hipEventRecord(w->ev0, st), "hipEventRecord");
     if (coop) HIP_TRY(launch_coop(W, mode, grid, p, st), "cooperative kernel launch");
@@ -700,6 +710,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     p.items = b->dItems;
     p.scenes = b->dScenes;
     p.nItems = used;
+    p.chunk = ray_chunk(total, grid, bd);
     if (graph) {
         if (!b->exec || b->graphStream != st || b->graphGrid != grid || b->graphBd != bd || b->graphS != S ||
             b->graphSpill != b->spill || b->graphSpillCap != spillCap) {

@@ -56,6 +56,7 @@ struct KParams {
     uint32_t gang;
     uint32_t Lm;
     uint32_t nodeCount;
+    uint32_t chunk;  // rays reserved per wave per atomicAdd on the counter
     // cooperative launch only: node memory and LDS stack entries per ray
     uint32_t M;
     uint32_t coopDepth;

@@ -74,6 +74,31 @@ __device__ __forceinline__ float comp(const float4& v, int j) {
     return j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
 }
 
+// Triangles of a leaf are read LEAF_CHUNK slots at a time: 9 plane loads of
+// LEAF_CHUNK consecutive floats (16- or 8-byte aligned: k and s0 are
+// multiples of 4 / 2).  Two slots per chunk keep 18 VGPRs of vertex data live
+// instead of 36.
+#ifndef LEAF_CHUNK
+#define LEAF_CHUNK 2
+#endif
+struct LeafChunk {
+#if LEAF_CHUNK == 4
+    float4 p[9];
+    __device__ __forceinline__ void load(const float* T, uint32_t k, uint32_t s0) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) p[i] = ld4(T + i * k + s0);
+    }
+    __device__ __forceinline__ float v(int i, int j) const { return comp(p[i], j); }
+#else
+    float2 p[9];
+    __device__ __forceinline__ void load(const float* T, uint32_t k, uint32_t s0) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) p[i] = *reinterpret_cast<const float2*>(T + i * k + s0);
+    }
+    __device__ __forceinline__ float v(int i, int j) const { return j == 0 ? p[i].x : p[i].y; }
+#endif
+};
+
 // maxAbsDimension (glmUtil.h:18-23)
 __device__ __forceinline__ int max_abs_dim(float x, float y, float z) {
     x = fabsf(x);
@@ -188,15 +213,14 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
         t.cLS += k;
         if (t.anyMode) {
             bool any = false;
-            for (uint32_t s0 = 0; s0 < k; s0 += 4) {
-                const float4 a0 = ld4(T + s0), a1 = ld4(T + k + s0), a2 = ld4(T + 2 * k + s0);
-                const float4 a3 = ld4(T + 3 * k + s0), a4 = ld4(T + 4 * k + s0), a5 = ld4(T + 5 * k + s0);
-                const float4 a6 = ld4(T + 6 * k + s0), a7 = ld4(T + 7 * k + s0), a8 = ld4(T + 8 * k + s0);
+            for (uint32_t s0 = 0; s0 < k; s0 += LEAF_CHUNK) {
+                LeafChunk a;
+                a.load(T, k, s0);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < LEAF_CHUNK; ++j) {
                     float tt, b1, b2;
-                    if (mt(comp(a0, j), comp(a1, j), comp(a2, j), comp(a3, j), comp(a4, j), comp(a5, j),
-                           comp(a6, j), comp(a7, j), comp(a8, j), t.ray, tt, b1, b2))
+                    if (mt(a.v(0, j), a.v(1, j), a.v(2, j), a.v(3, j), a.v(4, j), a.v(5, j), a.v(6, j), a.v(7, j),
+                           a.v(8, j), t.ray, tt, b1, b2))
                         any = true;
                 }
             }
@@ -212,15 +236,14 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
             float bt = 1000000.0f;
             int bidx = -1;
             uint32_t gm = (1u << sc.G) - 1u;  // instances whose tStore == bt
-            for (uint32_t s0 = 0; s0 < k; s0 += 4) {
-                const float4 a0 = ld4(T + s0), a1 = ld4(T + k + s0), a2 = ld4(T + 2 * k + s0);
-                const float4 a3 = ld4(T + 3 * k + s0), a4 = ld4(T + 4 * k + s0), a5 = ld4(T + 5 * k + s0);
-                const float4 a6 = ld4(T + 6 * k + s0), a7 = ld4(T + 7 * k + s0), a8 = ld4(T + 8 * k + s0);
+            for (uint32_t s0 = 0; s0 < k; s0 += LEAF_CHUNK) {
+                LeafChunk a;
+                a.load(T, k, s0);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < LEAF_CHUNK; ++j) {
                     float tt, b1, b2;
-                    if (mt(comp(a0, j), comp(a1, j), comp(a2, j), comp(a3, j), comp(a4, j), comp(a5, j),
-                           comp(a6, j), comp(a7, j), comp(a8, j), t.ray, tt, b1, b2)) {
+                    if (mt(a.v(0, j), a.v(1, j), a.v(2, j), a.v(3, j), a.v(4, j), a.v(5, j), a.v(6, j), a.v(7, j),
+                           a.v(8, j), t.ray, tt, b1, b2)) {
                         const uint32_t s = s0 + (uint32_t)j;
                         const uint32_t g = 1u << (s & (sc.G - 1u));
                         if (tt < bt) {
@@ -392,20 +415,46 @@ __device__ __forceinline__ SceneCtx plain_scene(const KParams& p, uint32_t MT) {
     return sc;
 }
 
-// Wave-aggregated fetch of ray indices for the lanes with want == true:
-// one atomicAdd per wave, lanes ranked by mbcnt.  Returns the lane's index
-// (>= n when the rays are exhausted) and sets `exhausted` wave-uniformly.
-__device__ __forceinline__ uint32_t fetch_rays(uint32_t* counter, uint32_t n, bool want, bool& exhausted) {
-    const uint64_t need = __ballot(want);
-    if (!need) return n;
-    const uint32_t cnt = (uint32_t)__popcll(need);
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)need) - 1;
-    uint32_t base = 0;
-    if (__lane_id() == leader) base = atomicAdd(counter, cnt);
-    base = __shfl(base, (int)leader);
-    if (base + cnt >= n) exhausted = true;
-    return want ? base + (uint32_t)__popcll(need & __lanemask_lt()) : n;
-}
+// Ray fetching.  Rays are handed out in chunks: one atomicAdd on the global
+// counter reserves `chunk` consecutive rays for a wave, and the wave's idle
+// lanes take them from that private range without further atomics.  (A
+// single counter shared by all XCDs, hit once per refill, serializes the
+// waves; chunking cuts those atomics by the chunk size.)
+struct WaveQueue {
+    uint32_t next = 0, end = 0;  // wave-uniform private range [next, end)
+    bool exhausted = false;      // the global counter has passed n
+
+    // Lanes with `want` get consecutive ray indices (< n); the others get n.
+    __device__ __forceinline__ uint32_t fetch(uint32_t* counter, uint32_t n, bool want, uint32_t chunk) {
+        const uint64_t lt = __lanemask_lt();
+        uint64_t need = __ballot(want);
+        uint32_t got = n;
+        while (need) {
+            if (next == end) {
+                if (exhausted) break;
+                uint32_t base = 0;
+                if (__lane_id() == 0) base = atomicAdd(counter, chunk);
+                base = __shfl(base, 0);
+                if (base >= n) {
+                    exhausted = true;
+                    break;
+                }
+                next = base;
+                end = (n - base < chunk) ? n : base + chunk;
+            }
+            const uint32_t take = min(end - next, (uint32_t)__popcll(need));
+            const bool mine = (need >> __lane_id()) & 1u;
+            const uint32_t rank = (uint32_t)__popcll(need & lt);
+            const bool assigned = mine && rank < take;
+            if (assigned) got = next + rank;
+            need &= ~__ballot(assigned);
+            next += take;
+        }
+        return got;
+    }
+    // no ray left in the private range nor behind the counter
+    __device__ __forceinline__ bool drained() const { return exhausted && next == end; }
+};
 
 // Per-ray kernel.  One template for the plain launch (one scene, uniform
 // parameters, node memory MT known at compile time) and the batched launch
@@ -433,14 +482,15 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
     uint8_t* occluded = p.occluded;
     rt_counts* counts = p.counts;
 
-    bool has = false, exhausted = false;
+    bool has = false;
+    WaveQueue q;
     uint32_t r = 0;
     Trav t;
     t.hasCur = false;
     t.sp = 0;
     while (true) {
-        if (!exhausted) {
-            const uint32_t rr = fetch_rays(p.work, p.n, !has, exhausted);
+        if (!q.drained()) {
+            const uint32_t rr = q.fetch(p.work, p.n, !has, p.chunk);
             if (!has && rr < p.n) {
                 r = rr;
                 bool anyMode = (MODE == kAny);
@@ -576,12 +626,12 @@ __global__ void __launch_bounds__(256) k_stream(KParams p) {
     };
 
     sFlags[tid] = 0;
-    bool exhausted = false;
+    WaveQueue q;
     while (true) {
         // refill the slots this thread owns (slot == tid)
         const bool free = !(sFlags[tid] & 1u);
-        if (!exhausted) {
-            const uint32_t rr = fetch_rays(p.work, p.n, free, exhausted);
+        if (!q.drained()) {
+            const uint32_t rr = q.fetch(p.work, p.n, free, p.chunk);
             if (free && rr < p.n) {
                 Trav t;
                 trav_init(t, p.rays[rr], sc.rootEntry, MODE == kAny);
@@ -629,7 +679,7 @@ __global__ void __launch_bounds__(256) k_stream(KParams p) {
         // block is done when no slot is active and every wave's fetch is
         // exhausted
         const int anyActive = __syncthreads_or(active);
-        const int allExhausted = __syncthreads_and(exhausted);
+        const int allExhausted = __syncthreads_and(q.drained());
         if (!anyActive && allExhausted) break;
         const uint32_t nNode = sCount[0], nLeaf = sCount[1];
         // node phase
@@ -692,7 +742,8 @@ __global__ void __launch_bounds__(256) k_coop(KParams p) {
     uint32_t resMask = 0;
     for (uint32_t j = gl % G; j < (uint32_t)W; j += G) resMask |= 1u << j;
 
-    bool has = false, exhausted = false;
+    bool has = false;
+    WaveQueue q;
     uint32_t r = 0, sp = 0, live = 0, hitEntry = 0;
     int hitSlot = -1;
     bool occl = false, err = false, rev = false;
@@ -702,8 +753,8 @@ __global__ void __launch_bounds__(256) k_coop(KParams p) {
     auto group_ballot = [&](bool v) -> uint32_t { return (uint32_t)(__ballot(v) >> gbase) & wmask; };
 
     while (true) {
-        if (!exhausted) {
-            uint32_t rr = fetch_rays(p.work, p.n, gl == 0 && !has, exhausted);
+        if (!q.drained()) {
+            uint32_t rr = q.fetch(p.work, p.n, gl == 0 && !has, p.chunk);
             rr = __shfl(rr, (int)gbase);
             if (!has && rr < p.n) {
                 r = rr;
