@@ -23,6 +23,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+
+# HBM bytes per launch of the timed trace kernel, from two rocprofv3 --pmc
+# runs of this benchmark (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch),
+# each scaled by what the same counter reads for a plain 1 GiB device copy
+# (bench.py --calibrate-copy): bytes = FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
+# in KiB.  Keyed by (branching, leaf size, rays); the commands and the
+# per-dispatch values are in profiles/k_trace_N8L8_diffuse8M_hbm_traffic.md.
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 8745699, "write": 392030}}
 METRIC = "Mrays/s per MI355X (closest-hit, N8L8) batched vs plain; p99 request latency"
 
 
@@ -289,6 +298,8 @@ def main():
     value = n / (ms_per_step / 1000.0) / 1e6
     k_ms = float(np.mean(kms))
     achieved = alg_bytes / (k_ms / 1000.0) / 1e9
+    mt = MEASURED_TRAFFIC_KIB.get((N, L, n))
+    traffic = None if mt is None else int((mt["fetch"] * FETCH_SCALE + mt["write"] * WRITE_SCALE) * 1024)
     st = scene.stats()
     log(f"kernel {k_ms:.3f} ms avg, step {ms_per_step:.3f} ms, {value:.1f} Mrays/s, "
         f"{achieved:.0f} GB/s algorithmic, errors {st['error_rays']}")
@@ -348,7 +359,7 @@ def main():
                    "primary_mrays_1920x1080": round(primary_mrays, 2),
                    "kernel_ms": round(k_ms, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,

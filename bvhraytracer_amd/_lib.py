@@ -106,6 +106,7 @@ SIGNATURES = [
     ("rt_trace", ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P]),
     ("rt_stats", ctypes.c_int, [_P, ctypes.POINTER(RtStats)]),
     ("rt_set_launch", ctypes.c_int, [_P, _U32, _U32]),
+    ("rt_set_schedule", ctypes.c_int, [_P, _U32]),
     ("rt_release", ctypes.c_int, [_P]),
     ("rt_last_error", ctypes.c_char_p, []),
     ("rt_scene_footprint", ctypes.c_int, [ctypes.POINTER(RtBvhDesc), ctypes.POINTER(ctypes.c_uint64)]),

@@ -37,24 +37,26 @@ def _run(cmd):
     return r.stdout
 
 
-def build(verbose=False):
-    os.makedirs(OBJ, exist_ok=True)
+def build(verbose=False, out=OUT, obj_dir=OBJ, defines=()):
+    """Compiles and links the library to `out`; `defines` are extra -D
+    options for the HIP sources (kernel variants)."""
+    os.makedirs(obj_dir, exist_ok=True)
     jobs = []
     for src in HOST_SOURCES:
-        obj = os.path.join(OBJ, src + ".o")
+        obj = os.path.join(obj_dir, src + ".o")
         jobs.append(["g++", *CXXFLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
     for src in HIP_SOURCES:
-        obj = os.path.join(OBJ, src + ".o")
-        jobs.append([HIPCC, *HIPFLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        obj = os.path.join(obj_dir, src + ".o")
+        jobs.append([HIPCC, *HIPFLAGS, *[f"-D{d}" for d in defines], "-c", os.path.join(CSRC, src), "-o", obj])
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
-        for out in ex.map(_run, jobs):
-            if verbose and out.strip():
-                print(out)
-    objs = [os.path.join(OBJ, s + ".o") for s in HOST_SOURCES + HIP_SOURCES]
-    tmp = OUT + ".tmp"
+        for text in ex.map(_run, jobs):
+            if verbose and text.strip():
+                print(text)
+    objs = [os.path.join(obj_dir, s + ".o") for s in HOST_SOURCES + HIP_SOURCES]
+    tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-pthread"])
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 def build_oracle():

@@ -26,6 +26,10 @@ using namespace rt;
 
 namespace {
 
+// Wave schedule of the per-ray kernel used unless rt_set_schedule says
+// otherwise (see KParams::sched).
+constexpr uint32_t kDefaultSched = 16;
+
 int hip_fail(hipError_t e, const char* what) {
     return set_error(RT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -38,7 +42,7 @@ int hip_fail(hipError_t e, const char* what) {
 
 // Per-stream scratch: work counters, stack spill and host staging buffers.
 struct Workspace {
-    uint32_t* work = nullptr;  // 4 words
+    uint32_t* work = nullptr;  // kWorkWords words
     uint32_t* spill = nullptr;
     size_t spillBytes = 0;
     void* stage = nullptr;  // rays | out | counts for host-pointer calls
@@ -49,6 +53,7 @@ struct Workspace {
 
 struct LayoutInfo {
     uint32_t inner = 0, leaves = 0, maxDepth = 0;
+    bool boxesBounded = true;  // every stored box coordinate is within +-2^100
 };
 
 uint32_t decode_node_k(const rt_bvh_desc* d, uint32_t i) {
@@ -79,6 +84,8 @@ struct rt_scene {
     uint32_t M = 0, Lm = 0, G = 0, nodeCount = 0;
     uint32_t inner = 0, leaves = 0, maxDepth = 0;
     uint32_t rootEntry = 0;
+    bool boxesBounded = false;
+    uint32_t sched = kDefaultSched;
     uint32_t* nodes = nullptr;
     float* tris = nullptr;
     int32_t* triIds = nullptr;
@@ -147,6 +154,7 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<floa
         for (uint32_t p = 0; p < 6; ++p)
             for (uint32_t s = 0; s < k; ++s) {
                 float v = bd[p * k + s];
+                if (!(std::fabs(v) <= 0x1p100f)) info.boxesBounded = false;
                 std::memcpy(&rec[p * M + s], &v, 4);
             }
         const int8_t* ord = d->order + (size_t)i * 3 * M;
@@ -195,10 +203,26 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<floa
             for (uint32_t q = 0; q < M; ++q)
                 h[5 + a * OW + q / 8] |= ((uint32_t)ord[a * M + q] & 15u) << (4 * (q % 8));
     }
+    // leaf blocks: p0 planes as stored, then e1 = p1 - p0 and e2 = p2 - p0
+    // (the edge vectors triIntersect/triAnyHit form first, in the same FP32
+    // subtraction) so the kernels do not recompute them for every ray
     const uint32_t LS = leaf_stride(Lm);
     tris.assign((size_t)nLeafBlocks * LS, 0.0f);
-    for (uint64_t l = 0; l < nLeafBlocks; ++l)
-        std::memcpy(&tris[l * LS], d->tri_points + l * block, block * sizeof(float));
+    std::vector<uint32_t> blockK(nLeafBlocks, 0);
+    for (uint32_t i = 0; i < n; ++i)
+        if (isLeaf[i]) blockK[index[i]] = kOf[i];
+    for (uint64_t l = 0; l < nLeafBlocks; ++l) {
+        float* dst = &tris[l * LS];
+        const float* src = d->tri_points + l * block;
+        std::memcpy(dst, src, block * sizeof(float));
+        const uint32_t k = blockK[l];
+        for (uint32_t c = 0; c < 3; ++c)
+            for (uint32_t s = 0; s < k; ++s) {
+                const float p0 = src[c * k + s];
+                dst[(3 + c) * k + s] = src[(3 + c) * k + s] - p0;
+                dst[(6 + c) * k + s] = src[(6 + c) * k + s] - p0;
+            }
+    }
     triIds.assign(d->tri_ids, d->tri_ids + nLeafBlocks * Lm);
     info.inner = nInner;
     info.leaves = (uint32_t)nLeafBlocks;
@@ -253,7 +277,7 @@ uint32_t stack_capacity(const rt_scene* s) { return ((1 + (s->maxDepth + 1) * s-
 int get_workspace(rt_scene* s, hipStream_t st, Workspace*& out) {
     Workspace& w = s->ws[st];
     if (!w.work) {
-        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.work), 64), "hipMalloc(work)");
+        HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.work), kWorkWords * 4), "hipMalloc(work)");
         HIP_TRY(hipEventCreate(&w.ev0), "hipEventCreate");
         HIP_TRY(hipEventCreate(&w.ev1), "hipEventCreate");
     }
@@ -311,6 +335,7 @@ extern "C" int rt_upload_bvh(const rt_bvh_desc* d, rt_scene_t* out) {
     s->inner = info.inner;
     s->leaves = info.leaves;
     s->maxDepth = info.maxDepth;
+    s->boxesBounded = info.boxesBounded;
     s->rootEntry = entry[0];
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, s->device) == hipSuccess) s->cus = (uint32_t)prop.multiProcessorCount;
@@ -334,6 +359,14 @@ extern "C" int rt_set_launch(rt_scene_t s, uint32_t block_size, uint32_t stack_w
     std::lock_guard<std::mutex> lk(s->mu);
     if (block_size) s->blockSize = block_size;
     if (stack_window) s->stackWindow = stack_window;
+    return RT_OK;
+}
+
+extern "C" int rt_set_schedule(rt_scene_t s, uint32_t leaf_ratio16) {
+    if (!s) return set_error(RT_ERR_ARG, "rt_set_schedule: null scene");
+    if (leaf_ratio16 > 1024) return set_error(RT_ERR_ARG, "rt_set_schedule: ratio must be at most 1024 (64.0)");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->sched = leaf_ratio16;
     return RT_OK;
 }
 
@@ -430,6 +463,8 @@ extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t f
     p.M = s->M;
     p.coopDepth = coopDepth;
     p.chunk = ray_chunk(n, grid, bd);
+    p.boxesBounded = s->boxesBounded ? 1u : 0u;
+    p.sched = s->sched;
     HIP_TRY(launch_reset(w->work, st), "reset kernel");
     HIP_TRY(hipEventRecord(w->ev0, st), "hipEventRecord");
     if (coop) HIP_TRY(launch_coop(W, mode, grid, p, st), "cooperative kernel launch");
@@ -447,7 +482,7 @@ extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t f
         float ms = 0;
         if (hipEventElapsedTime(&ms, w->ev0, w->ev1) == hipSuccess) s->lastMs = ms;
         uint32_t errs = 0;
-        HIP_TRY(hipMemcpy(&errs, w->work + 1, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
+        HIP_TRY(hipMemcpy(&errs, w->work + kErrWord, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
         s->errorRays += errs;
     }
     return RT_OK;
@@ -600,7 +635,7 @@ extern "C" int rt_batch_create(uint32_t max_items, rt_batch_t* out) {
         (e = hipMalloc(reinterpret_cast<void**>(&b->dScenes), sizeof(BatchScene) * max_items)) != hipSuccess ||
         (e = hipHostMalloc(reinterpret_cast<void**>(&b->hItems), sizeof(BatchItem) * max_items, hipHostMallocDefault)) != hipSuccess ||
         (e = hipHostMalloc(reinterpret_cast<void**>(&b->hScenes), sizeof(BatchScene) * max_items, hipHostMallocDefault)) != hipSuccess ||
-        (e = hipMalloc(reinterpret_cast<void**>(&b->work), 64)) != hipSuccess ||
+        (e = hipMalloc(reinterpret_cast<void**>(&b->work), kWorkWords * 4)) != hipSuccess ||
         (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess) {
         free_batch(b);
         return hip_fail(e, "rt_batch_create");
@@ -673,7 +708,8 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
             bs.Lm = s->Lm;
             bs.nodeCount = s->nodeCount;
             bs.M = s->M;
-            bs.pad0 = bs.pad1 = bs.pad2 = 0;
+            bs.boxesBounded = s->boxesBounded ? 1u : 0u;
+            bs.pad1 = bs.pad2 = 0;
             cap = std::max(cap, stack_capacity(s));
         }
         BatchItem& bi = b->hItems[used++];
@@ -711,6 +747,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     p.scenes = b->dScenes;
     p.nItems = used;
     p.chunk = ray_chunk(total, grid, bd);
+    p.sched = kDefaultSched;
     if (graph) {
         if (!b->exec || b->graphStream != st || b->graphGrid != grid || b->graphBd != bd || b->graphS != S ||
             b->graphSpill != b->spill || b->graphSpillCap != spillCap) {
@@ -742,7 +779,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
         float ms = 0;
         if (hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess) b->lastMs = ms;
         uint32_t errs = 0;
-        HIP_TRY(hipMemcpy(&errs, b->work + 1, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
+        HIP_TRY(hipMemcpy(&errs, b->work + kErrWord, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
         b->errorRays += errs;
     }
     return RT_OK;

@@ -21,6 +21,12 @@ namespace rt {
 
 enum TraceMode { kClosest = 0, kAny = 1, kSaveDist = 2, kMixed = 3 };
 
+// Work counters: the ray counter and the guarded-condition counter, each on
+// its own 128-byte line.
+constexpr uint32_t kRayCounter = 0;
+constexpr uint32_t kErrWord = 32;  // rays stopped by a guard
+constexpr uint32_t kWorkWords = 64;
+
 // Batched launch tables (device memory).  A batch is a list of items, each a
 // contiguous ray range of one scene in one mode; the global ray index space
 // is the concatenation of the items (item i covers [start, start + n)).
@@ -29,7 +35,7 @@ struct BatchScene {
     const float* tris;
     const int32_t* triIds;
     const uint32_t* nodeEntry;
-    uint32_t rootEntry, gang, Lm, nodeCount, M, pad0, pad1, pad2;
+    uint32_t rootEntry, gang, Lm, nodeCount, M, boxesBounded, pad1, pad2;
 };
 
 struct BatchItem {
@@ -49,7 +55,7 @@ struct KParams {
     uint8_t* occluded;
     rt_counts* counts;
     uint32_t* spill;       // per thread slot: spillCap entries (+ spillCap floats)
-    uint32_t* work;        // [0] ray counter, [1] guarded-condition counter
+    uint32_t* work;        // kWorkWords words: partition counters, guard counter
     uint32_t n;
     uint32_t rootEntry;
     uint32_t spillCap;
@@ -57,6 +63,11 @@ struct KParams {
     uint32_t Lm;
     uint32_t nodeCount;
     uint32_t chunk;  // rays reserved per wave per atomicAdd on the counter
+    uint32_t boxesBounded;  // every tested box coordinate is within +-2^100
+    // per-ray kernel wave schedule: 0 = every lane steps each iteration;
+    // r > 0 = one phase per iteration, the leaf phase when
+    // 16 * leafLanes >= r * nodeLanes
+    uint32_t sched;
     // cooperative launch only: node memory and LDS stack entries per ray
     uint32_t M;
     uint32_t coopDepth;

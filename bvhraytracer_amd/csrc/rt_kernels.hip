@@ -11,6 +11,8 @@
 // and correctly rounded division, in the order of rayTracerTemplate.ispc;
 // max/min are written as the sse4 maxps/minps selects the ISPC code lowers
 // to, so NaN and signed-zero cases resolve as in the reference.
+#include <type_traits>
+
 #include "rt_kernels.h"
 
 namespace rt {
@@ -43,12 +45,34 @@ __device__ __forceinline__ bool slab(float bnx, float bny, float bnz, float bxx,
     return t0 <= t1;
 }
 
+// slab() for operands that cannot produce a NaN: finite inverse direction,
+// |origin| and |box| <= 2^100 (so b - o is finite and (b - o) * inv is finite
+// or infinite, never NaN) and tmax not NaN.  Without NaNs the maxps/minps
+// selects and the swap equal fmaxf/fminf in value; only the sign of a zero
+// t0 may differ, and t0 is only ever compared.
+__device__ __forceinline__ bool slab_fast(float bnx, float bny, float bnz, float bxx, float bxy, float bxz,
+                                          const Ray& r, float& t0out) {
+    const float tnx = (bnx - r.ox) * r.ix, tny = (bny - r.oy) * r.iy, tnz = (bnz - r.oz) * r.iz;
+    const float tfx = (bxx - r.ox) * r.ix, tfy = (bxy - r.oy) * r.iy, tfz = (bxz - r.oz) * r.iz;
+    const float t0 = fmaxf(fmaxf(fminf(tnx, tfx), fminf(tny, tfy)), fmaxf(fminf(tnz, tfz), 0.0f));
+    const float t1 = fminf(fminf(fmaxf(tnx, tfx), fmaxf(tny, tfy)), fminf(fmaxf(tnz, tfz), r.tmax));
+    t0out = t0;
+    return t0 <= t1;
+}
+
+// Per-ray condition for slab_fast (the box bound is checked per scene).
+__device__ __forceinline__ bool ray_fast(const Ray& r) {
+    constexpr float kMaxO = 0x1p100f, kMaxF = 3.40282347e38f;
+    return fabsf(r.ox) <= kMaxO && fabsf(r.oy) <= kMaxO && fabsf(r.oz) <= kMaxO && fabsf(r.ix) <= kMaxF &&
+           fabsf(r.iy) <= kMaxF && fabsf(r.iz) <= kMaxF && r.tmax == r.tmax;
+}
+
 // Moeller-Trumbore body of triIntersect / triAnyHit (rayTracerTemplate.ispc:49-75).
-__device__ __forceinline__ bool mt(float p0x, float p0y, float p0z, float p1x, float p1y, float p1z,
-                                   float p2x, float p2y, float p2z, const Ray& r, float& t,
+// The edge vectors e1 = p1 - p0 and e2 = p2 - p0 come precomputed from the
+// leaf block (same FP32 subtraction, done once at upload).
+__device__ __forceinline__ bool mt(float p0x, float p0y, float p0z, float e1x, float e1y, float e1z,
+                                   float e2x, float e2y, float e2z, const Ray& r, float& t,
                                    float& b1, float& b2) {
-    const float e1x = p1x - p0x, e1y = p1y - p0y, e1z = p1z - p0z;
-    const float e2x = p2x - p0x, e2y = p2y - p0y, e2z = p2z - p0z;
     const float s1x = r.dy * e2z - r.dz * e2y;
     const float s1y = r.dz * e2x - r.dx * e2z;
     const float s1z = r.dx * e2y - r.dy * e2x;
@@ -117,6 +141,7 @@ struct SceneCtx {
     const int32_t* triIds;
     const uint32_t* nodeEntry;
     uint32_t rootEntry, G, Lm, nodeCount, M, recw, ow;
+    bool boxesBounded;  // every tested box coordinate is within +-2^100
 };
 
 struct StackCtx {
@@ -138,11 +163,12 @@ struct Trav {
     uint32_t hitEntry;
     int hitSlot;
     uint32_t cN, cL, cNS, cLS;
-    bool hasCur, occl, err, anyMode, rev;
+    bool hasCur, occl, err, anyMode, rev, fast;
     int axis;
 };
 
-__device__ __forceinline__ void trav_init(Trav& t, const rt_ray& in, uint32_t rootEntry, bool anyMode) {
+__device__ __forceinline__ void trav_init(Trav& t, const rt_ray& in, uint32_t rootEntry, bool anyMode,
+                                          bool boxesBounded) {
     t.ray.tmax = in.tmax;
     t.ray.ox = in.ox;
     t.ray.oy = in.oy;
@@ -156,6 +182,7 @@ __device__ __forceinline__ void trav_init(Trav& t, const rt_ray& in, uint32_t ro
     t.axis = max_abs_dim(t.ray.dx, t.ray.dy, t.ray.dz);
     const float da = t.axis == 0 ? t.ray.dx : (t.axis == 1 ? t.ray.dy : t.ray.dz);
     t.rev = da <= 0.0f;
+    t.fast = boxesBounded && ray_fast(t.ray);
     t.cur = rootEntry;
     t.curDist = 0.0f;
     t.hasCur = true;
@@ -274,18 +301,27 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
     if (ow == 2) h2 = ld4u(rec + 6 * M + 8);
     uint32_t H = 0;
     const float* bp = reinterpret_cast<const float*>(rec);
-    for (uint32_t s0 = 0; s0 < k; s0 += 4) {
-        const float4 nx = ld4(bp + s0), ny = ld4(bp + M + s0), nz = ld4(bp + 2 * M + s0);
-        const float4 xx = ld4(bp + 3 * M + s0), xy = ld4(bp + 4 * M + s0), xz = ld4(bp + 5 * M + s0);
+    auto boxes = [&](auto fastTag) {
+        for (uint32_t s0 = 0; s0 < k; s0 += 4) {
+            const float4 nx = ld4(bp + s0), ny = ld4(bp + M + s0), nz = ld4(bp + 2 * M + s0);
+            const float4 xx = ld4(bp + 3 * M + s0), xy = ld4(bp + 4 * M + s0), xz = ld4(bp + 5 * M + s0);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            float t0;
-            if (slab(comp(nx, j), comp(ny, j), comp(nz, j), comp(xx, j), comp(xy, j), comp(xz, j), t.ray, t0)) {
-                H |= 1u << (s0 + (uint32_t)j);
-                if (MODE == kSaveDist) sk.aabbD[(s0 + (uint32_t)j) * bd] = t0;
+            for (int j = 0; j < 4; ++j) {
+                float t0;
+                const float bnx = comp(nx, j), bny = comp(ny, j), bnz = comp(nz, j);
+                const float bxx = comp(xx, j), bxy = comp(xy, j), bxz = comp(xz, j);
+                bool hit;
+                if constexpr (decltype(fastTag)::value) hit = slab_fast(bnx, bny, bnz, bxx, bxy, bxz, t.ray, t0);
+                else hit = slab(bnx, bny, bnz, bxx, bxy, bxz, t.ray, t0);
+                if (hit) {
+                    H |= 1u << (s0 + (uint32_t)j);
+                    if (MODE == kSaveDist) sk.aabbD[(s0 + (uint32_t)j) * bd] = t0;
+                }
             }
         }
-    }
+    };
+    if (t.fast) boxes(std::true_type{});
+    else boxes(std::false_type{});
     t.live |= H;
     if (MODE != kSaveDist && H == 0) return;
     const uint32_t innerBase = h0.x, leafBase = h0.y;
@@ -299,24 +335,21 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
         const uint32_t hi = t.axis == 0 ? h1.z : (t.axis == 1 ? h2.x : h2.z);
         row = ((uint64_t)hi << 32) | lo;
     }
-    auto entry_of = [&](uint32_t s) -> uint32_t {
-        if (s >= c) {  // padded slot: resolve the FastNode id like the reference
-            const uint32_t id = childBase + s;
-            return id < sc.nodeCount ? sc.nodeEntry[id] : kEntryInvalid;
-        }
+    // stack entry of a real child slot s < c (no branches: both child kinds
+    // are computed and selected)
+    const uint32_t innerMask = ~leafMask;
+    auto entry_real = [&](uint32_t s) -> uint32_t {
         const uint32_t below = (1u << s) - 1u;
         const uint32_t kc = ((kcodes >> (2 * s)) & 3u) << kEntryKShift;
-        if ((leafMask >> s) & 1u) return kEntryLeaf | kc | (leafBase + __popc(leafMask & below));
-        return kc | (innerBase + __popc(~leafMask & below));
+        const bool isLeaf = (leafMask >> s) & 1u;
+        const uint32_t idx = (isLeaf ? leafBase : innerBase) + __popc((isLeaf ? leafMask : innerMask) & below);
+        return (isLeaf ? kEntryLeaf : 0u) | kc | idx;
     };
-    // push(s): the previous pending child goes to the stack, child s becomes
-    // the pending one (visited next)
-    auto push = [&](uint32_t s) {
+    // push(e, s): the previous pending child goes to the stack, child s
+    // (entry e) becomes the pending one (visited next).  The stack capacity
+    // is checked once per node, before the pushes.
+    auto push = [&](uint32_t e, uint32_t s) {
         if (t.hasCur) {
-            if (t.sp >= sk.cap) {
-                t.err = true;
-                return;
-            }
             const uint32_t sl = (t.sp & (S - 1)) * bd;
             if (t.sp >= S) {
                 sk.spE[t.sp - S] = sk.stE[sl];
@@ -326,7 +359,7 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
             if (MODE == kSaveDist) sk.stD[sl] = t.curDist;
             ++t.sp;
         }
-        t.cur = entry_of(s);
+        t.cur = e;
         if (MODE == kSaveDist) {
             t.curDist = sk.aabbD[s * bd];
             sk.aabbD[s * bd] = __builtin_nanf("");
@@ -334,22 +367,52 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
         t.hasCur = true;
         t.live &= ~(1u << s);
     };
+    // entries pushed after the first land on the stack
+    auto fits = [&](uint32_t npush) { return npush == 0 || t.sp + npush - 1u <= sk.cap; };
+    // The push sequence is the order row filtered by the live mask.  Each
+    // slot occurs once in the row, so the positions to push are known before
+    // the first push: the loops below run once per push, not once per slot.
     if (t.anyMode) {
-        // intersectSecondary: slot order 0..M-1 (fastNodeManager.h:1481-1488)
-        for (uint32_t s = 0; s < M; ++s)
-            if ((t.live >> s) & 1u) push(s);
-    } else if (t.rev) {
-        for (uint32_t q = 0; q < c; ++q) {
-            const uint32_t s = (uint32_t)(row >> (4 * q)) & 15u;
-            if ((t.live >> s) & 1u) push(s);
+        // intersectSecondary: slot order 0..M-1 (fastNodeManager.h:1481-1488),
+        // padded slots s >= c included: their FastNode id resolves like the
+        // reference's, through the entry table
+        const uint32_t m0 = t.live;
+        if (!fits((uint32_t)__popc(m0))) {
+            t.err = true;
+        } else {
+            for (uint32_t m = m0; m; m &= m - 1u) {
+                const uint32_t s = (uint32_t)__builtin_ctz(m);
+                uint32_t e;
+                if (s >= c) {
+                    const uint32_t id = childBase + s;
+                    e = id < sc.nodeCount ? sc.nodeEntry[id] : kEntryInvalid;
+                } else {
+                    e = entry_real(s);
+                }
+                push(e, s);
+            }
         }
     } else {
+        uint32_t live = t.live;
         // backward over the zero-padded order row: the padding (slot 0)
         // comes first when childCount < M
-        if (c < M && (t.live & 1u)) push(0);
-        for (uint32_t q = c; q-- > 0;) {
-            const uint32_t s = (uint32_t)(row >> (4 * q)) & 15u;
-            if ((t.live >> s) & 1u) push(s);
+        const bool pad0 = !t.rev && c < M && (live & 1u);
+        if (pad0) live &= ~1u;
+        uint32_t pm = 0;  // row positions q < c whose slot is live
+        for (uint32_t q = 0; q < c; ++q) pm |= ((live >> ((uint32_t)(row >> (4 * q)) & 15u)) & 1u) << q;
+        if (!fits((uint32_t)__popc(pm) + (pad0 ? 1u : 0u))) {
+            t.err = true;
+        } else {
+            if (pad0) push(entry_real(0), 0);
+            // one loop for both directions (forward takes the lowest position
+            // first, backward the highest), so lanes of either kind share
+            // the iterations; row positions q < c hold real slots only
+            while (pm) {
+                const uint32_t q = t.rev ? (uint32_t)__builtin_ctz(pm) : 31u - (uint32_t)__builtin_clz(pm);
+                pm &= ~(1u << q);
+                const uint32_t s = (uint32_t)(row >> (4 * q)) & 15u;
+                push(entry_real(s), s);
+            }
         }
     }
     if (t.err) {
@@ -409,6 +472,7 @@ __device__ __forceinline__ SceneCtx plain_scene(const KParams& p, uint32_t MT) {
     sc.G = p.gang;
     sc.Lm = p.Lm;
     sc.nodeCount = p.nodeCount;
+    sc.boxesBounded = p.boxesBounded != 0;
     sc.M = MT;
     sc.recw = record_words(MT);
     sc.ow = order_words(MT);
@@ -461,8 +525,12 @@ struct WaveQueue {
 // (BATCH: every lane carries the scene and item of its current ray; MT = 0
 // reads the node memory from the scene; MODE = kMixed picks closest/any per
 // item).
+// Register budget of the plain launch: at most 96 VGPRs, 5 waves per SIMD
+// (measured faster than the unconstrained 100-VGPR build at 4 waves; 6 waves
+// spills).  The batched launch keeps its per-lane scene context in registers
+// and stays at 4 waves.
 template <int MT, int MODE, int S, bool BATCH>
-__global__ void __launch_bounds__(256) k_trace(KParams p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BATCH ? 4 : 5))) k_trace(KParams p) {
     extern __shared__ uint32_t smem[];
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
     const size_t gslot = (size_t)blockIdx.x * bd + tid;
@@ -490,7 +558,7 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
     t.sp = 0;
     while (true) {
         if (!q.drained()) {
-            const uint32_t rr = q.fetch(p.work, p.n, !has, p.chunk);
+            const uint32_t rr = q.fetch(p.work + kRayCounter, p.n, !has, p.chunk);
             if (!has && rr < p.n) {
                 r = rr;
                 bool anyMode = (MODE == kAny);
@@ -512,6 +580,7 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                     sc.G = bs.gang;
                     sc.Lm = bs.Lm;
                     sc.nodeCount = bs.nodeCount;
+                    sc.boxesBounded = bs.boxesBounded != 0;
                     sc.M = bs.M;
                     sc.recw = record_words(bs.M);
                     sc.ow = order_words(bs.M);
@@ -522,7 +591,7 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
                     anyMode = it.mode == kAny;
                     r = rr - it.start;
                 }
-                trav_init(t, rays[r], sc.rootEntry, anyMode);
+                trav_init(t, rays[r], sc.rootEntry, anyMode, sc.boxesBounded);
                 if (MODE == kSaveDist)
                     for (int s = 0; s < MT; ++s) sk.aabbD[s * bd] = __builtin_nanf("");
                 has = true;
@@ -530,9 +599,17 @@ __global__ void __launch_bounds__(256) k_trace(KParams p) {
         }
         if (__ballot(has) == 0) break;
         if (!has) continue;
+        if (p.sched) {
+            // one phase per iteration: lanes whose next entry is of the other
+            // type wait (each ray still takes its own steps in order)
+            const bool leafNext = (trav_peek<S>(t, sk) & kEntryLeaf) != 0;
+            const uint32_t nl = (uint32_t)__popcll(__ballot(leafNext));
+            const uint32_t nn = (uint32_t)__popcll(__ballot(!leafNext));
+            if (leafNext != (16u * nl >= p.sched * nn)) continue;
+        }
         trav_step<MT, MODE, S>(sc, sk, t);
         if (!trav_done(t)) continue;
-        trav_finish(sc, t, hits, occluded, counts, r, p.work + 1);
+        trav_finish(sc, t, hits, occluded, counts, r, p.work + kErrWord);
         has = false;
     }
 }
@@ -608,6 +685,7 @@ __global__ void __launch_bounds__(256) k_stream(KParams p) {
         t.err = f & 8u;
         t.rev = f & 16u;
         t.axis = (int)((f >> 5) & 3u);
+        t.fast = f & 128u;
         t.anyMode = (MODE == kAny);
     };
     auto store = [&](uint32_t s, const Trav& t) {
@@ -622,7 +700,7 @@ __global__ void __launch_bounds__(256) k_stream(KParams p) {
         sCnt[2 * R + s] = t.cNS;
         sCnt[3 * R + s] = t.cLS;
         sFlags[s] = 1u | (t.hasCur ? 2u : 0u) | (t.occl ? 4u : 0u) | (t.err ? 8u : 0u) | (t.rev ? 16u : 0u) |
-                    ((uint32_t)t.axis << 5);
+                    ((uint32_t)t.axis << 5) | (t.fast ? 128u : 0u);
     };
 
     sFlags[tid] = 0;
@@ -631,10 +709,10 @@ __global__ void __launch_bounds__(256) k_stream(KParams p) {
         // refill the slots this thread owns (slot == tid)
         const bool free = !(sFlags[tid] & 1u);
         if (!q.drained()) {
-            const uint32_t rr = q.fetch(p.work, p.n, free, p.chunk);
+            const uint32_t rr = q.fetch(p.work + kRayCounter, p.n, free, p.chunk);
             if (free && rr < p.n) {
                 Trav t;
-                trav_init(t, p.rays[rr], sc.rootEntry, MODE == kAny);
+                trav_init(t, p.rays[rr], sc.rootEntry, MODE == kAny, sc.boxesBounded);
                 const rt_ray& in = p.rays[rr];
                 sRay[R + tid] = in.ox;
                 sRay[2 * R + tid] = in.oy;
@@ -705,7 +783,7 @@ __global__ void __launch_bounds__(256) k_stream(KParams p) {
             Trav t;
             load(tid, t);
             if (trav_done(t)) {
-                trav_finish(sc, t, p.hits, p.occluded, p.counts, sRid[tid], p.work + 1);
+                trav_finish(sc, t, p.hits, p.occluded, p.counts, sRid[tid], p.work + kErrWord);
                 sFlags[tid] = 0;
             }
         }
@@ -746,7 +824,7 @@ __global__ void __launch_bounds__(256) k_coop(KParams p) {
     WaveQueue q;
     uint32_t r = 0, sp = 0, live = 0, hitEntry = 0;
     int hitSlot = -1;
-    bool occl = false, err = false, rev = false;
+    bool occl = false, err = false, rev = false, fast = false;
     int axis = 0;
     uint32_t cN = 0, cL = 0, cNS = 0, cLS = 0;
     Ray ray;
@@ -754,7 +832,7 @@ __global__ void __launch_bounds__(256) k_coop(KParams p) {
 
     while (true) {
         if (!q.drained()) {
-            uint32_t rr = q.fetch(p.work, p.n, gl == 0 && !has, p.chunk);
+            uint32_t rr = q.fetch(p.work + kRayCounter, p.n, gl == 0 && !has, p.chunk);
             rr = __shfl(rr, (int)gbase);
             if (!has && rr < p.n) {
                 r = rr;
@@ -772,6 +850,7 @@ __global__ void __launch_bounds__(256) k_coop(KParams p) {
                 axis = max_abs_dim(ray.dx, ray.dy, ray.dz);
                 const float da = axis == 0 ? ray.dx : (axis == 1 ? ray.dy : ray.dz);
                 rev = da <= 0.0f;
+                fast = sc.boxesBounded && ray_fast(ray);
                 if (gl == 0) stk[0] = sc.rootEntry;
                 sp = 1;
                 live = 0;
@@ -832,7 +911,8 @@ __global__ void __launch_bounds__(256) k_coop(KParams p) {
             if (gl < k) {
                 const float* bp = reinterpret_cast<const float*>(rec) + gl;
                 float t0;
-                hit = slab(bp[0], bp[M], bp[2 * M], bp[3 * M], bp[4 * M], bp[5 * M], ray, t0);
+                if (fast) hit = slab_fast(bp[0], bp[M], bp[2 * M], bp[3 * M], bp[4 * M], bp[5 * M], ray, t0);
+                else hit = slab(bp[0], bp[M], bp[2 * M], bp[3 * M], bp[4 * M], bp[5 * M], ray, t0);
             }
             const uint32_t H = group_ballot(hit);
             live |= H;
@@ -912,14 +992,14 @@ __global__ void __launch_bounds__(256) k_coop(KParams p) {
             t.occl = occl;
             t.err = err;
             t.anyMode = MODE == kAny;
-            trav_finish(sc, t, p.hits, p.occluded, p.counts, r, p.work + 1);
+            trav_finish(sc, t, p.hits, p.occluded, p.counts, r, p.work + kErrWord);
         }
         has = false;
     }
 }
 
 __global__ void k_reset(uint32_t* work) {
-    if (threadIdx.x < 4) work[threadIdx.x] = 0;
+    for (uint32_t i = threadIdx.x; i < kWorkWords; i += blockDim.x) work[i] = 0;
 }
 
 template <int M, int MODE, int S>

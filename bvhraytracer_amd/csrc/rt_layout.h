@@ -22,8 +22,10 @@
 //
 // Leaf block (one per leaf, in FastNode order == trianglePoints order):
 //   LSTRIDE(Lm) floats (9*Lm rounded up to 16 floats = 64 bytes) holding the
-//   reference's 9*Lm-float block unchanged: planes p0.x p0.y p0.z p1.x ...
-//   p2.z at stride k = leafRealSize.
+//   reference's 9*Lm-float block with its planes at stride k (the leaf's
+//   loop count): p0.x p0.y p0.z as stored, then e1 = p1 - p0 and
+//   e2 = p2 - p0 (x, y, z planes each), the FP32 differences the
+//   intersection kernels form first.
 //
 // Stack entry (32 bits):
 //   bit 31        leaf

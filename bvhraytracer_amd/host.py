@@ -174,6 +174,10 @@ class Scene:
     def set_launch(self, block_size=0, stack_window=0):
         check(lib().rt_set_launch(self._h, int(block_size), int(stack_window)))
 
+    def set_schedule(self, leaf_ratio16):
+        """Wave schedule of the per-ray kernel (rt_set_schedule)."""
+        check(lib().rt_set_schedule(self._h, int(leaf_ratio16)))
+
     def intersect(self, rays, mode="closest", counts=False, stream=False, coop=False):
         """Host-array form: rays (n,) RAY_DTYPE -> hits (n,) HIT_DTYPE, or
         occluded (n,) uint8 for mode='any'; plus counts when asked."""

@@ -152,6 +152,15 @@ int rt_stats(rt_scene_t s, rt_stats_t* out);
  * LDS stack window 8/16/32 entries).  0 keeps the current value. */
 int rt_set_launch(rt_scene_t s, uint32_t block_size, uint32_t stack_window);
 
+/* Wave schedule of the per-ray kernel.  0: every lane takes one step per
+ * iteration, so a wave runs the node and the leaf code whenever its lanes
+ * disagree.  r > 0: each iteration runs one phase only, the leaf phase when
+ * 16 * (lanes at a leaf) >= r * (lanes at a node); the other lanes wait.
+ * Every ray still takes its own steps in its own order, so results and
+ * counts do not depend on r.  No counterpart in the reference (its
+ * traversal is one ray per call); tuned per scene like the launch shape. */
+int rt_set_schedule(rt_scene_t s, uint32_t leaf_ratio16);
+
 int rt_release(rt_scene_t s);
 
 const char* rt_last_error(void);

@@ -215,3 +215,74 @@ def test_cooperative_edge_cases(rt):
     g, gc = rt.Scene(bvh).intersect(prim, counts=True, coop=True)
     ref = oracle_for(bvh).trace(prim)
     assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], "coop ties")
+
+
+def _special_rays(prim, seed):
+    """Ordinary primary rays mixed with rays whose slab operands can produce
+    NaNs: infinite inverse direction (denormal or -0 direction part), NaN
+    tMax, an origin coordinate beyond 2^100."""
+    rng = np.random.default_rng(seed)
+    sp = prim[rng.choice(len(prim), 4000)].copy()
+    k = np.arange(len(sp)) % 5
+    sp["dx"][k == 0] = np.float32(1e-40)
+    sp["ix"][k == 0] = np.float32(np.inf)
+    sp["tmax"][k == 1] = np.float32(np.nan)
+    sp["ox"][k == 2] = np.float32(3e30)
+    sp["dy"][k == 3] = np.float32(-0.0)
+    sp["iy"][k == 3] = np.float32(-np.inf)
+    rays = np.concatenate([prim, sp])
+    return rays[rng.permutation(len(rays))]
+
+
+@pytest.mark.parametrize("far", [False, True])
+def test_slab_special_operands(rt, far):
+    """The min/max slab form is taken only where no NaN can arise; the rays
+    above (and, with a box coordinate beyond 2^100 in the scene, every ray)
+    take the maxps/minps select sequence.  Both kinds share waves."""
+    tris = rt.interior_scene(5, 20000, 10)
+    if far:
+        tris = np.concatenate([tris, np.float32([[3e30, 0, 0, 3e30, 1, 0, 3e30, 0, 1]])])
+    bvh = rt.Bvh(tris, 8, 8)
+    prim = rt.camera_rays((0.0, 5.0, -14.0), (0.0, 3.0, 0.0), 96, 64, 16)
+    rays = _special_rays(prim, 5)
+    scene = rt.Scene(bvh)
+    print(f"special slab operands far={far}: {len(rays)} rays", flush=True)
+    for kw in ({}, {"stream": True}, {"coop": True}):
+        g, gc = scene.intersect(rays, counts=True, **kw)
+        ref = oracle_for(bvh).trace(rays)
+        assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], f"special {kw}")
+        occ, oc = scene.intersect(rays, mode="any", counts=True, **kw)
+        ro = oracle_for(bvh).trace(rays, mode=oracle.ORC_ANY)
+        assert np.array_equal(occ, ro["occluded"]), kw
+        assert np.array_equal(oc["nodes"], ro["counts"]["nodes"]), kw
+    g, gc = scene.intersect(rays, mode="save_distance", counts=True)
+    ref = oracle_for(bvh).trace(rays, mode=oracle.ORC_SAVE_DIST)
+    assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], "special save-distance")
+
+
+@pytest.mark.parametrize("sched", [4, 16, 64])
+@pytest.mark.parametrize("N,L", [(8, 8), (4, 16)])
+def test_wave_schedules(rt, sched, N, L):
+    """One phase per iteration (rt_set_schedule): lanes wait, rays keep their
+    own step order, so hits and counts equal the oracle's."""
+    tris = rt.interior_scene(23, 40_000, 20)
+    bvh = rt.Bvh(tris, N, L)
+    prim = rt.camera_rays((0.0, 5.0, -14.0), (0.0, 3.0, 0.0), 96, 64, 16)
+    scene = rt.Scene(bvh)
+    hits = scene.intersect(prim)
+    nrm, pos = bvh.surface(prim, hits)
+    rays = np.concatenate([prim, rt.spawn_diffuse(prim, nrm, pos, hits, 20000, seed=13)])
+    scene.set_schedule(sched)
+    print(f"schedule {sched} N{N}L{L}: {len(rays)} rays", flush=True)
+    g, gc = scene.intersect(rays, counts=True)
+    ref = oracle_for(bvh).trace(rays)
+    assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], f"sched {sched}")
+    srays, _ = rt.spawn_shadow(nrm, pos, hits, (0.0, 9.5, 0.0), 1.0)
+    occ, oc = scene.intersect(srays, mode="any", counts=True)
+    ro = oracle_for(bvh).trace(srays, mode=oracle.ORC_ANY)
+    assert np.array_equal(occ, ro["occluded"])
+    assert np.array_equal(oc["nodes"], ro["counts"]["nodes"])
+    assert np.array_equal(oc["leaves"], ro["counts"]["leaves"])
+    g, gc = scene.intersect(rays, mode="save_distance", counts=True)
+    ref = oracle_for(bvh).trace(rays, mode=oracle.ORC_SAVE_DIST)
+    assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], f"sched {sched} save-distance")
