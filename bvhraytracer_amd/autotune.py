@@ -1,4 +1,4 @@
-"""Per-scene autotuner: N, L, workgroup size and LDS stack window.
+"""Per-scene autotuner: N, L, workgroup size, LDS stack window and wave schedule.
 
 The reference picks N and L by sweeping them offline and keeps the best SAH
 node factor per (N, L) in a table (sahFactorConfigSse.txt, loaded by
@@ -20,6 +20,7 @@ from .host import Bvh, Scene
 DEFAULT_NL = ((4, 4), (8, 8), (8, 4), (16, 8), (4, 8), (16, 16))
 DEFAULT_BLOCKS = (64, 128, 256)
 DEFAULT_WINDOWS = (8, 16, 32)
+DEFAULT_SCHEDULES = (0, 8, 16, 24)  # rt_set_schedule leaf ratios (x16)
 
 
 def scene_key(tris, rays):
@@ -31,9 +32,10 @@ def scene_key(tris, rays):
 
 class Autotuner:
     def __init__(self, cache_path=None, nl=DEFAULT_NL, blocks=DEFAULT_BLOCKS, windows=DEFAULT_WINDOWS,
-                 sample=1 << 16, reps=3):
+                 schedules=DEFAULT_SCHEDULES, sample=1 << 16, reps=3):
         self.cache_path = cache_path
         self.nl, self.blocks, self.windows = tuple(nl), tuple(blocks), tuple(windows)
+        self.schedules = tuple(schedules)
         self.sample, self.reps = int(sample), int(reps)
         self.cache = {}
         if cache_path and os.path.exists(cache_path):
@@ -54,18 +56,23 @@ class Autotuner:
         return best
 
     def tune_launch(self, scene, d_rays, d_out, mode="closest"):
-        """Fastest (block size, stack window) for this scene and ray set."""
+        """Fastest (block size, stack window, wave schedule) for this scene
+        and ray set."""
         res = []
         for w in self.windows:
             for b in self.blocks:
-                scene.set_launch(b, w)
-                res.append((self._time(scene, d_rays, d_out, mode), b, w))
-        ms, b, w = min(res)
+                for sc in self.schedules:
+                    scene.set_launch(b, w)
+                    scene.set_schedule(sc)
+                    res.append((self._time(scene, d_rays, d_out, mode), b, w, sc))
+        ms, b, w, sc = min(res)
         scene.set_launch(b, w)
-        return {"block": b, "window": w, "ms": ms}
+        scene.set_schedule(sc)
+        return {"block": b, "window": w, "schedule": sc, "ms": ms}
 
     def tune_scene(self, tris, rays, mode="closest", key=None):
-        """Returns {"N", "L", "block", "window", "mrays", "tried"} for the scene.
+        """Returns {"N", "L", "block", "window", "schedule", "mrays", "tried"}
+        for the scene.
         rays: host RAY_DTYPE array; a sample of `sample` rays is used."""
         import torch
 
@@ -85,9 +92,11 @@ class Autotuner:
             scene = Scene(bvh)
             r = self.tune_launch(scene, d_rays, d_out, mode)
             mrays = len(sample) / r["ms"] / 1e3
-            tried.append({"N": N, "L": L, "block": r["block"], "window": r["window"], "mrays": round(mrays, 2)})
+            cfg = {"N": N, "L": L, "block": r["block"], "window": r["window"], "schedule": r["schedule"],
+                   "mrays": round(mrays, 2)}
+            tried.append(cfg)
             if best is None or mrays > best["mrays"]:
-                best = {"N": N, "L": L, "block": r["block"], "window": r["window"], "mrays": round(mrays, 2)}
+                best = dict(cfg)
             scene.close()
             bvh.close()
         best["tried"] = tried
