@@ -234,19 +234,29 @@ def _special_rays(prim, seed):
     return rays[rng.permutation(len(rays))]
 
 
-@pytest.mark.parametrize("far", [False, True])
-def test_slab_special_operands(rt, far):
+@pytest.mark.parametrize("scene_kind", ["regular", "far", "inverted"])
+def test_slab_special_operands(rt, scene_kind):
     """The min/max slab form is taken only where no NaN can arise; the rays
-    above (and, with a box coordinate beyond 2^100 in the scene, every ray)
-    take the maxps/minps select sequence.  Both kinds share waves."""
+    above (and, in a scene with a box coordinate beyond 2^100, every ray)
+    take the maxps/minps select sequence.  Both kinds share waves.  A box
+    with min > max on an axis tests like the ordered box in both forms (the
+    reference swaps per axis)."""
     tris = rt.interior_scene(5, 20000, 10)
-    if far:
+    if scene_kind == "far":
         tris = np.concatenate([tris, np.float32([[3e30, 0, 0, 3e30, 1, 0, 3e30, 0, 1]])])
     bvh = rt.Bvh(tris, 8, 8)
+    if scene_kind == "inverted":
+        # swap min and max x of child slot 0 of the root (the reference's
+        # slab test swaps per axis, so its interval is unchanged)
+        k = 8
+        b = bvh.bounds
+        assert not np.isnan(b[0, 0])
+        b[0, 0], b[0, 3 * k] = b[0, 3 * k].copy(), b[0, 0].copy()
+        assert b[0, 0] > b[0, 3 * k]
     prim = rt.camera_rays((0.0, 5.0, -14.0), (0.0, 3.0, 0.0), 96, 64, 16)
     rays = _special_rays(prim, 5)
     scene = rt.Scene(bvh)
-    print(f"special slab operands far={far}: {len(rays)} rays", flush=True)
+    print(f"special slab operands {scene_kind}: {len(rays)} rays", flush=True)
     for kw in ({}, {"stream": True}, {"coop": True}):
         g, gc = scene.intersect(rays, counts=True, **kw)
         ref = oracle_for(bvh).trace(rays)
