@@ -136,6 +136,11 @@ SIGNATURES = [
                                        ctypes.POINTER(_U32)]),
     ("rt_spawn_diffuse", ctypes.c_int, [_P, _P, _P, _P, _U32, ctypes.c_uint64, _U32,
                                         ctypes.c_float, _P]),
+    ("rt_camera_rays_device", ctypes.c_int, [ctypes.POINTER(RtCamera), _U32, _U32, _U32, _P, _P]),
+    ("rt_hit_surface_device", ctypes.c_int, [_P, _P, _P, _U32, _P, _P, _P]),
+    ("rt_spawn_shadow_device", ctypes.c_int, [_P, _P, _P, _U32, _P, ctypes.c_float, _P, _P]),
+    ("rt_spawn_diffuse_device", ctypes.c_int, [_P, _P, _P, _P, _U32, ctypes.c_uint64, _U32,
+                                               ctypes.c_float, _P, _P]),
     ("rt_sah_factor", ctypes.c_float, [_U32, _U32, _U32, ctypes.c_char_p]),
 ]
 

@@ -17,7 +17,7 @@ OUT = os.path.join(HERE, "librt.so")
 OBJ = os.path.join(HERE, "_build")
 
 HOST_SOURCES = ["scene.cpp", "builder.cpp", "host_api.cpp"]
-HIP_SOURCES = ["rt_kernels.hip", "rt_api.hip", "rt_serve.hip"]
+HIP_SOURCES = ["rt_kernels.hip", "rt_api.hip", "rt_serve.hip", "rt_shade.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

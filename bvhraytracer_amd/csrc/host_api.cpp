@@ -9,6 +9,7 @@
 #include <sstream>
 
 #include "host_common.h"
+#include "rt_raymath.h"
 
 namespace rt {
 
@@ -22,50 +23,6 @@ int set_error(int code, const std::string& msg) {
 int build_bvh(const float* tris, uint32_t n, const rt_build_params& p, HostBvh& out);
 
 namespace {
-
-// FastRay::FastRay (fastRay.cpp:3-20): dir = target - pos (not normalised),
-// inv = 1/dir with FLT_MAX where a component is 0.
-rt_ray make_ray(const float o[3], const float target[3], float tmax) {
-    rt_ray r;
-    r.tmax = tmax;
-    r.ox = o[0];
-    r.oy = o[1];
-    r.oz = o[2];
-    r.dx = target[0] - o[0];
-    r.dy = target[1] - o[1];
-    r.dz = target[2] - o[2];
-    r.ix = (r.dx == 0.0f) ? FLT_MAX : 1.0f / r.dx;
-    r.iy = (r.dy == 0.0f) ? FLT_MAX : 1.0f / r.dy;
-    r.iz = (r.dz == 0.0f) ? FLT_MAX : 1.0f / r.dz;
-    return r;
-}
-
-// FastRay::updateDirection (fastRay.cpp:22-38)
-void set_direction(rt_ray& r, float dx, float dy, float dz) {
-    r.dx = dx;
-    r.dy = dy;
-    r.dz = dz;
-    r.ix = (dx == 0.0f) ? FLT_MAX : 1.0f / dx;
-    r.iy = (dy == 0.0f) ? FLT_MAX : 1.0f / dy;
-    r.iz = (dz == 0.0f) ? FLT_MAX : 1.0f / dz;
-}
-
-inline void cross3(const float a[3], const float b[3], float o[3]) {
-    o[0] = a[1] * b[2] - b[1] * a[2];
-    o[1] = a[2] * b[0] - b[2] * a[0];
-    o[2] = a[0] * b[1] - b[0] * a[1];
-}
-inline float dot3(const float a[3], const float b[3]) {
-    float x = a[0] * b[0], y = a[1] * b[1], z = a[2] * b[2];
-    return x + y + z;
-}
-// glm::normalize: v * (1 / sqrt(dot(v, v)))
-inline void normalize3(float v[3]) {
-    float s = 1.0f / std::sqrt(dot3(v, v));
-    v[0] *= s;
-    v[1] *= s;
-    v[2] *= s;
-}
 
 // sahFactorConfigSse.txt:2-17 — (branching, leaf size) -> SAH node factor.
 struct SahRow {
@@ -174,38 +131,12 @@ extern "C" int rt_camera_rays(const rt_camera* cam, uint32_t width, uint32_t hei
     if (!cam || !out || width == 0 || height == 0 || wg == 0)
         return set_error(RT_ERR_ARG, "rt_camera_rays: bad argument");
     if (width % wg != 0) return set_error(RT_ERR_ARG, "rt_camera_rays: width must be a multiple of wg");
-    float f[3] = {cam->target[0] - cam->pos[0], cam->target[1] - cam->pos[1], cam->target[2] - cam->pos[2]};
-    normalize3(f);
-    float s[3];
-    cross3(f, cam->up, s);
-    normalize3(s);
-    float u[3];
-    cross3(s, f, u);
-    const float nf[3] = {-f[0], -f[1], -f[2]};
-    const float invW = 1.0f / (float)width;
-    const uint32_t tilesX = width / wg;
-    const uint32_t tileRows = (height + wg - 1) / wg;
-    const uint32_t wgsq = wg * wg;
+    const CameraBasis b = camera_basis(*cam, width, height, wg);
+    const uint32_t tiles = width / wg * ((height + wg - 1) / wg);
     size_t k = 0;
-    for (uint32_t i = 0; i < tilesX * tileRows; ++i) {
-        for (uint32_t j = 0; j < wgsq; ++j) {
-            const uint32_t row = (i * wg) / width * wg + j / wg;
-            if (row >= height) continue;  // partial last tile row
-            // RenderInfo(w, h) of cameraFast.h:324-327 (float -> int)
-            const int w = (int)(((float)((i * wg) % width) - width / 2.f) + (float)(j % wg));
-            const int h = (int)(-((float)(((i * wg) / width) * wg) - height / 2.f) - (float)(j / wg));
-            // centerOffset = (w, h, 0, 0) * (1/width) + (0, 0, -focal, 0)
-            const float cx = (float)w * invW, cy = (float)h * invW, cz = 0.0f * invW + -cam->focal;
-            float tgt[3];
-            for (int a = 0; a < 3; ++a) {
-                // glm mat4*vec4: (m0*v0 + m1*v1) + (m2*v2 + m3*v3), v3 = 0
-                const float add0 = s[a] * cx + u[a] * cy;
-                const float add1 = nf[a] * cz + cam->pos[a] * 0.0f;
-                tgt[a] = cam->pos[a] + (add0 + add1);
-            }
-            out[k++] = make_ray(cam->pos, tgt, 222222.0f);
-        }
-    }
+    for (uint32_t i = 0; i < tiles; ++i)
+        for (uint32_t j = 0; j < wg * wg; ++j)
+            if (camera_ray(b, i, j, out[k])) ++k;
     return RT_OK;
 }
 
@@ -239,20 +170,9 @@ extern "C" int rt_hit_surface(rt_host_bvh_t hb, const rt_ray* rays, const rt_hit
             p1[a] = b.tri[t0 + (size_t)lc * (3 + a)];
             p2[a] = b.tri[t0 + (size_t)lc * (6 + a)];
         }
-        float e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
-        float e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
-        float nrm[3];
-        cross3(e1, e2, nrm);
-        normalize3(nrm);
-        const rt_ray& r = rays[i];
-        const float dir[3] = {r.dx, r.dy, r.dz}, o[3] = {r.ox, r.oy, r.oz};
-        const float denom = dot3(nrm, dir);
-        const float p0l0[3] = {p0[0] - o[0], p0[1] - o[1], p0[2] - o[2]};
-        const float t = dot3(p0l0, nrm) / denom;
-        for (int a = 0; a < 3; ++a) {
-            nn[a] = nrm[a];
-            pp[a] = o[a] + dir[a] * t;
-        }
+        const float e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+        const float e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+        surface(p0, e1, e2, rays[i], nn, pp);
     }
     return RT_OK;
 }
@@ -267,11 +187,7 @@ extern "C" int rt_spawn_shadow(const float* normal, const float* position, const
     uint32_t k = 0;
     for (uint32_t i = 0; i < n; ++i) {
         if (hits[i].slot < 0) continue;
-        const float* nn = normal + (size_t)i * 3;
-        const float* pp = position + (size_t)i * 3;
-        float o[3];
-        for (int a = 0; a < 3; ++a) o[a] = pp[a] + nn[a] * 0.001f;
-        out[k] = make_ray(o, light, tmax);
+        out[k] = shadow_ray(normal + (size_t)i * 3, position + (size_t)i * 3, light, tmax);
         if (src_index) src_index[k] = i;
         ++k;
     }
@@ -279,8 +195,8 @@ extern "C" int rt_spawn_shadow(const float* normal, const float* position, const
     return RT_OK;
 }
 
-// Cosine-distributed bounce rays (Malley's method, orthonormal basis of Duff
-// et al. 2017) around the hit normal face-forwarded against the primary ray.
+// Cosine-distributed bounce rays (diffuse_ray) cycling over the hit pixels:
+// ray r starts at hit number r % hits.
 extern "C" int rt_spawn_diffuse(const rt_ray* primary, const float* normal, const float* position,
                                 const rt_hit* hits, uint32_t n, uint64_t seed, uint32_t n_out,
                                 float tmax, rt_ray* out) {
@@ -292,31 +208,8 @@ extern "C" int rt_spawn_diffuse(const rt_ray* primary, const float* normal, cons
     if (hitIdx.empty() && n_out) return set_error(RT_ERR_ARG, "rt_spawn_diffuse: no hits to spawn from");
     for (uint32_t r = 0; r < n_out; ++r) {
         const uint32_t i = hitIdx[r % hitIdx.size()];
-        const float* pp = position + (size_t)i * 3;
-        float nn[3] = {normal[i * 3 + 0], normal[i * 3 + 1], normal[i * 3 + 2]};
         const float pd[3] = {primary[i].dx, primary[i].dy, primary[i].dz};
-        if (dot3(nn, pd) > 0.0f) {
-            nn[0] = -nn[0];
-            nn[1] = -nn[1];
-            nn[2] = -nn[2];
-        }
-        Pcg32 rng(seed, r);
-        const float u1 = rng.uniform(), u2 = rng.uniform();
-        const float rad = std::sqrt(u1), phi = 6.2831853071795865f * u2;
-        const float lx = rad * std::cos(phi), ly = rad * std::sin(phi), lz = std::sqrt(std::max(0.0f, 1.0f - u1));
-        const float sgn = std::copysign(1.0f, nn[2]);
-        const float a = -1.0f / (sgn + nn[2]);
-        const float bb = nn[0] * nn[1] * a;
-        const float t[3] = {1.0f + sgn * nn[0] * nn[0] * a, sgn * bb, -sgn * nn[0]};
-        const float bt[3] = {bb, sgn + nn[1] * nn[1] * a, -nn[1]};
-        rt_ray ray;
-        ray.tmax = tmax;
-        ray.ox = pp[0] + nn[0] * 0.001f;
-        ray.oy = pp[1] + nn[1] * 0.001f;
-        ray.oz = pp[2] + nn[2] * 0.001f;
-        set_direction(ray, t[0] * lx + bt[0] * ly + nn[0] * lz, t[1] * lx + bt[1] * ly + nn[1] * lz,
-                      t[2] * lx + bt[2] * ly + nn[2] * lz);
-        out[r] = ray;
+        out[r] = diffuse_ray(pd, normal + (size_t)i * 3, position + (size_t)i * 3, seed, r, tmax);
     }
     return RT_OK;
 }

@@ -21,6 +21,7 @@
 #include "host_common.h"
 #include "rt_kernels.h"
 #include "rt_layout.h"
+#include "rt_shade.h"
 
 using namespace rt;
 
@@ -90,6 +91,7 @@ struct rt_scene {
     float* tris = nullptr;
     int32_t* triIds = nullptr;
     uint32_t* nodeEntry = nullptr;
+    uint8_t* leafLc = nullptr;  // getLeafLoopCount per leaf block (hit reconstruction)
     uint64_t bytes = 0;
     uint32_t blockSize = 256, stackWindow = 16;
     uint32_t cus = 0;
@@ -106,7 +108,8 @@ namespace {
 // reference's traversal relies on (contiguous children, loop counts within
 // the node/leaf memory).
 int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<float>& tris,
-            std::vector<int32_t>& triIds, std::vector<uint32_t>& entry, LayoutInfo& info) {
+            std::vector<int32_t>& triIds, std::vector<uint32_t>& entry, std::vector<uint8_t>& leafLc,
+            LayoutInfo& info) {
     const uint32_t M = d->node_memory, Lm = d->leaf_memory, G = d->gang, n = d->node_count;
     if (!(M == 4 || M == 8 || M == 12 || M == 16) || !(Lm == 4 || Lm == 8 || Lm == 12 || Lm == 16))
         return set_error(RT_ERR_UNSUPPORTED, "rt_upload_bvh: node/leaf memory must be 4, 8, 12 or 16");
@@ -211,11 +214,17 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<floa
     std::vector<uint32_t> blockK(nLeafBlocks, 0);
     for (uint32_t i = 0; i < n; ++i)
         if (isLeaf[i]) blockK[index[i]] = kOf[i];
+    leafLc.assign(nLeafBlocks, 0);
     for (uint64_t l = 0; l < nLeafBlocks; ++l) {
         float* dst = &tris[l * LS];
         const float* src = d->tri_points + l * block;
         std::memcpy(dst, src, block * sizeof(float));
         const uint32_t k = blockK[l];
+        // loop count as getLeafLoopCount reads it (fastNodeManager.h:1728-1746),
+        // kept for hit reconstruction when it matches the layout's stride
+        int lc = (int)Lm;
+        if (Lm != G && std::isnan(src[block - 1])) lc = (int)src[block - 2];
+        if (k != 0 && lc == (int)k) leafLc[l] = (uint8_t)lc;
         for (uint32_t c = 0; c < 3; ++c)
             for (uint32_t s = 0; s < k; ++s) {
                 const float p0 = src[c * k + s];
@@ -256,6 +265,7 @@ void free_scene(rt_scene* s) {
     if (s->tris) (void)hipFree(s->tris);
     if (s->triIds) (void)hipFree(s->triIds);
     if (s->nodeEntry) (void)hipFree(s->nodeEntry);
+    if (s->leafLc) (void)hipFree(s->leafLc);
     if (s->ownStream) (void)hipStreamDestroy(s->ownStream);
     (void)hipSetDevice(cur);
     delete s;
@@ -308,7 +318,7 @@ extern "C" int rt_scene_footprint(const rt_bvh_desc* d, uint64_t* bytes) {
         if (!std::isnan(d->bounds[(size_t)i * 6 * M])) ++inner;
     const uint64_t leaves = d->tri_float_count / (9ull * Lm);
     *bytes = inner * record_words(M) * 4ull + leaves * leaf_stride(Lm) * 4ull + leaves * Lm * 4ull +
-             (uint64_t)d->node_count * 4ull;
+             (uint64_t)d->node_count * 4ull + std::max<uint64_t>(leaves, 64);
     return RT_OK;
 }
 
@@ -317,10 +327,11 @@ extern "C" int rt_upload_bvh(const rt_bvh_desc* d, rt_scene_t* out) {
     std::vector<uint32_t> nodes, entry;
     std::vector<float> tris;
     std::vector<int32_t> triIds;
+    std::vector<uint8_t> leafLc;
     LayoutInfo info;
     int rc;
     try {
-        rc = convert(d, nodes, tris, triIds, entry, info);
+        rc = convert(d, nodes, tris, triIds, entry, leafLc, info);
     } catch (const std::bad_alloc&) {
         rc = set_error(RT_ERR_NOMEM, "rt_upload_bvh: out of host memory");
     }
@@ -342,7 +353,8 @@ extern "C" int rt_upload_bvh(const rt_bvh_desc* d, rt_scene_t* out) {
     if (!s->cus) s->cus = 256;
     if ((rc = dev_upload(&s->nodes, nodes, s->bytes)) != RT_OK || (rc = dev_upload(&s->tris, tris, s->bytes)) != RT_OK ||
         (rc = dev_upload(&s->triIds, triIds, s->bytes)) != RT_OK ||
-        (rc = dev_upload(&s->nodeEntry, entry, s->bytes)) != RT_OK) {
+        (rc = dev_upload(&s->nodeEntry, entry, s->bytes)) != RT_OK ||
+        (rc = dev_upload(&s->leafLc, leafLc, s->bytes)) != RT_OK) {
         free_scene(s);
         return rc;
     }
@@ -359,6 +371,16 @@ extern "C" int rt_set_launch(rt_scene_t s, uint32_t block_size, uint32_t stack_w
     std::lock_guard<std::mutex> lk(s->mu);
     if (block_size) s->blockSize = block_size;
     if (stack_window) s->stackWindow = stack_window;
+    return RT_OK;
+}
+
+extern "C" int rt_hit_surface_device(rt_scene_t s, const rt_ray* rays, const rt_hit* hits, uint32_t n,
+                                     float* normal_out, float* position_out, void* hip_stream) {
+    if (!s || (n && (!rays || !hits || !normal_out || !position_out)))
+        return set_error(RT_ERR_ARG, "rt_hit_surface_device: bad argument");
+    HIP_TRY(launch_surface(s->tris, s->leafLc, s->Lm, s->leaves, rays, hits, n, normal_out, position_out,
+                           static_cast<hipStream_t>(hip_stream)),
+            "surface kernel");
     return RT_OK;
 }
 

@@ -1,0 +1,16 @@
+// Device hit reconstruction (rt_shade.hip), launched by rt_hit_surface_device
+// with the scene's device arrays.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/rt.h"
+
+namespace rt {
+
+// tris: device leaf blocks (rt_layout.h); leafLc: loop count per leaf block
+// as getLeafLoopCount reads it (0 = unusable).
+hipError_t launch_surface(const float* tris, const uint8_t* leafLc, uint32_t Lm, uint32_t nLeaves,
+                          const rt_ray* rays, const rt_hit* hits, uint32_t n, float* nrm, float* pos,
+                          hipStream_t stream);
+
+}  // namespace rt

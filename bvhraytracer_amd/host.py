@@ -26,6 +26,10 @@ from ._lib import (COUNTS_DTYPE, HIT_DTYPE, RAY_DTYPE, RtBatchItem, RtBuildParam
                    RtCamera, RtServerConfig, RtServerStats, RtStats, check, lib, ptr)
 
 
+def _stream(stream):
+    return None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+
+
 def roundup(x, g):
     return int(math.ceil(x / float(g)) * g)
 
@@ -209,6 +213,21 @@ class Scene:
         check(lib().rt_trace(self._h, ptr(rays), n, flags, ptr(out),
                              ptr(counts) if counts is not None else None, s))
 
+    def surface_device(self, rays, hits, normal=None, position=None, stream=None):
+        """rt_hit_surface_device: hit reconstruction from the resident
+        triangles.  rays (n,10) float32, hits (n,6) int32 CUDA tensors;
+        returns (normal (n,3), position (n,3)) CUDA tensors."""
+        import torch
+
+        n = rays.shape[0]
+        if normal is None:
+            normal = torch.empty((n, 3), dtype=torch.float32, device=rays.device)
+        if position is None:
+            position = torch.empty((n, 3), dtype=torch.float32, device=rays.device)
+        check(lib().rt_hit_surface_device(self._h, ptr(rays), ptr(hits), n, ptr(normal), ptr(position),
+                                          _stream(stream)))
+        return normal, position
+
     def stats(self):
         s = RtStats()
         check(lib().rt_stats(self._h, ctypes.byref(s)))
@@ -358,6 +377,46 @@ def spawn_diffuse(primary, normal, position, hits, count, seed=3, tmax=222222.0)
     check(lib().rt_spawn_diffuse(ptr(primary), ptr(np.ascontiguousarray(normal, np.float32)),
                                  ptr(np.ascontiguousarray(position, np.float32)), ptr(hits),
                                  len(hits), int(seed), int(count), float(tmax), ptr(out)))
+    return out
+
+
+def camera_rays_device(pos, target, width, height, wg=16, up=(0.0, 1.0, 0.0), focal=0.866, out=None,
+                       stream=None):
+    """rt_camera_rays_device: primary rays in CameraFast tile order, made on
+    the GPU; returns a (width*height, 10) float32 CUDA tensor."""
+    import torch
+
+    cam = RtCamera((ctypes.c_float * 3)(*pos), (ctypes.c_float * 3)(*target),
+                   (ctypes.c_float * 3)(*up), float(focal))
+    if out is None:
+        out = torch.empty((width * height, 10), dtype=torch.float32, device="cuda")
+    check(lib().rt_camera_rays_device(ctypes.byref(cam), width, height, wg, ptr(out), _stream(stream)))
+    return out
+
+
+def spawn_shadow_device(normal, position, hits, light, tmax=1.0, out=None, stream=None):
+    """rt_spawn_shadow_device: one shadow ray per hit record (NaN origin for
+    a miss); returns an (n, 10) float32 CUDA tensor."""
+    import torch
+
+    n = hits.shape[0]
+    if out is None:
+        out = torch.empty((n, 10), dtype=torch.float32, device=hits.device)
+    lt = (ctypes.c_float * 3)(*light)
+    check(lib().rt_spawn_shadow_device(ptr(normal), ptr(position), ptr(hits), n, lt, float(tmax), ptr(out),
+                                       _stream(stream)))
+    return out
+
+
+def spawn_diffuse_device(primary, normal, position, hits, count, seed=3, tmax=222222.0, out=None, stream=None):
+    """rt_spawn_diffuse_device: the rays of spawn_diffuse, made on the GPU;
+    returns a (count, 10) float32 CUDA tensor."""
+    import torch
+
+    if out is None:
+        out = torch.empty((count, 10), dtype=torch.float32, device=hits.device)
+    check(lib().rt_spawn_diffuse_device(ptr(primary), ptr(normal), ptr(position), ptr(hits), hits.shape[0],
+                                        int(seed), int(count), float(tmax), ptr(out), _stream(stream)))
     return out
 
 

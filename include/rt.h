@@ -303,6 +303,39 @@ int rt_spawn_diffuse(const rt_ray* primary, const float* normal, const float* po
                      const rt_hit* hits, uint32_t n, uint64_t seed, uint32_t n_out,
                      float tmax, rt_ray* out);
 
+/* ---- Ray generation and hit shading on the GPU ----------------------------
+ * Device pointers only; each call enqueues its kernels on hip_stream (NULL =
+ * the default stream) and returns.  The arithmetic is shared with the host
+ * functions above, so the results are bit-identical to theirs. */
+
+/* Primary rays in CameraFast tile order, as rt_camera_rays
+ * (Camera::getRayTargetPosition, camera.h:91-98; cameraFast.h:318-342).
+ * out: width*height rays. */
+int rt_camera_rays_device(const rt_camera* cam, uint32_t width, uint32_t height, uint32_t wg,
+                          rt_ray* out, void* hip_stream);
+
+/* Hit reconstruction from the scene's resident triangles, as rt_hit_surface
+ * (getSurfaceNormalPosition, fastNodeManager.h:1699-1726); misses get NaN. */
+int rt_hit_surface_device(rt_scene_t s, const rt_ray* rays, const rt_hit* hits, uint32_t n,
+                          float* normal_out, float* position_out, void* hip_stream);
+
+/* Shadow rays toward a point light, one per input: ray i = rt_spawn_shadow's
+ * ray for hit i (FastRay(P + n*0.001, light, tmax), cameraFast.h:373,521).
+ * Where hit i is a miss, ray i has tmax = -1, which no box test accepts: it
+ * costs one root visit and is never occluded.  (The reference gives such
+ * pixels a NaN origin, cameraFast.h:522-527, and skips them in its stream
+ * any-hit; the per-ray traversal would accept every box for a NaN origin.) */
+int rt_spawn_shadow_device(const float* normal, const float* position, const rt_hit* hits, uint32_t n,
+                           const float light[3], float tmax, rt_ray* out, void* hip_stream);
+
+/* n_out bounce rays exactly as rt_spawn_diffuse (ray r from hit number
+ * r % hits, in pixel order); with no hit at all every ray gets tmax = -1
+ * (no hit, as above).
+ * Allocates its scratch stream-ordered (hipMallocAsync) on hip_stream. */
+int rt_spawn_diffuse_device(const rt_ray* primary, const float* normal, const float* position,
+                            const rt_hit* hits, uint32_t n, uint64_t seed, uint32_t n_out, float tmax,
+                            rt_ray* out, void* hip_stream);
+
 /* sahFactorConfigSse.txt lookup (RayTracer::loadSahFactor, rayTracer.cpp:880-925). */
 float rt_sah_factor(uint32_t leaf_size, uint32_t branching, uint32_t gang, const char* table_path);
 
