@@ -133,6 +133,50 @@ def bench_batched(rt, torch, args, log):
     return out
 
 
+def bench_frame(rt, torch, scene, bvh, n_bounce, log, reps=3):
+    """One bounce frame: 1920x1080 primaries -> closest hit -> hit surfaces ->
+    n_bounce diffuse rays -> closest hit.  device: every stage on the GPU on
+    one stream (rt_camera_rays_device, rt_hit_surface_device,
+    rt_spawn_diffuse_device, device-resident rt_trace); host-assisted: the
+    host makes the rays and surfaces and each trace copies rays in and hits
+    out.  Median of reps frames after one warm-up, wall clock."""
+    W, H, pos, tgt = 1920, 1080, (0.0, 5.0, -14.0), (0.0, 3.0, 0.0)
+    stream = torch.cuda.Stream()
+    n = W * H
+    d_h = torch.empty((n, 6), dtype=torch.int32, device="cuda")
+    d_bh = torch.empty((n_bounce, 6), dtype=torch.int32, device="cuda")
+
+    def device_frame():
+        with torch.cuda.stream(stream):
+            d_r = rt.camera_rays_device(pos, tgt, W, H, 16, stream=stream)
+            scene.intersect_device(d_r, d_h, stream=stream, sync=False)
+            d_n, d_p = scene.surface_device(d_r, d_h, stream=stream)
+            d_b = rt.spawn_diffuse_device(d_r, d_n, d_p, d_h, n_bounce, seed=3, stream=stream)
+            scene.intersect_device(d_b, d_bh, stream=stream, sync=False)
+        stream.synchronize()
+
+    def host_frame():
+        prim = rt.camera_rays(pos, tgt, W, H, 16)
+        hits = scene.intersect(prim)
+        nrm, p = bvh.surface(prim, hits)
+        bounce = rt.spawn_diffuse(prim, nrm, p, hits, n_bounce, seed=3)
+        return scene.intersect(bounce)
+
+    out = {}
+    for name, fn in (("device", device_frame), ("host_assisted", host_frame)):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        out[name + "_ms"] = round(float(np.median(ts)) * 1000.0, 3)
+    out["rays"] = n + n_bounce
+    out["device_mrays"] = round((n + n_bounce) / (out["device_ms"] / 1000.0) / 1e6, 2)
+    log(f"bounce frame: device {out['device_ms']} ms, host-assisted {out['host_assisted_ms']} ms")
+    return out
+
+
 def bench_serving(rt, args, log):
     """16 resident scenes (100k..2M tris log-spaced, seeds 200..215), Poisson
     arrivals at 200 req/s for args.serve_seconds (seed 5), rays per request
@@ -236,6 +280,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=2.0, help="target seconds per CPU repetition")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-frame", action="store_true", help="skip the bounce-frame leg")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (counter calibration)")
@@ -314,6 +359,7 @@ def main():
         pms.append(scene.stats()["last_kernel_ms"])
     primary_mrays = len(prim) / (float(np.median(pms)) / 1000.0) / 1e6
 
+    frame = None if args.no_frame else bench_frame(rt, torch, scene, bvh, n, log)
     batched = None if args.no_batch else bench_batched(rt, torch, args, log)
     serving = None if args.no_serve else bench_serving(rt, args, log)
 
@@ -361,6 +407,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
+        "bounce_frame": frame,
         "batched": batched,
         "serving": serving,
     }

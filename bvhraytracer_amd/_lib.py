@@ -76,6 +76,20 @@ class RtStats(ctypes.Structure):
     ]
 
 
+RT_HIST_DEPTHS = 64
+RT_HIST_FULL = 17
+
+
+class RtTraceHist(ctypes.Structure):
+    _fields_ = [
+        ("rays", ctypes.c_uint64), ("node_visits", ctypes.c_uint64), ("leaf_visits", ctypes.c_uint64),
+        ("aabb_tests", ctypes.c_uint64), ("aabb_hits", ctypes.c_uint64),
+        ("tri_tests", ctypes.c_uint64), ("tri_hits", ctypes.c_uint64),
+        ("node_depth", ctypes.c_uint64 * RT_HIST_DEPTHS), ("leaf_depth", ctypes.c_uint64 * RT_HIST_DEPTHS),
+        ("child_fullness", ctypes.c_uint64 * RT_HIST_FULL), ("leaf_fullness", ctypes.c_uint64 * RT_HIST_FULL),
+    ]
+
+
 class RtBatchItem(ctypes.Structure):
     _fields_ = [("scene", ctypes.c_void_p), ("rays", ctypes.c_void_p), ("out", ctypes.c_void_p),
                 ("counts", ctypes.c_void_p), ("n", ctypes.c_uint32), ("flags", ctypes.c_uint32)]
@@ -105,6 +119,7 @@ SIGNATURES = [
     ("rt_upload_bvh", ctypes.c_int, [ctypes.POINTER(RtBvhDesc), ctypes.POINTER(_P)]),
     ("rt_trace", ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P]),
     ("rt_stats", ctypes.c_int, [_P, ctypes.POINTER(RtStats)]),
+    ("rt_trace_histograms", ctypes.c_int, [_P, _P, _U32, _U32, _P, ctypes.POINTER(RtTraceHist), _P]),
     ("rt_set_launch", ctypes.c_int, [_P, _U32, _U32]),
     ("rt_set_schedule", ctypes.c_int, [_P, _U32]),
     ("rt_release", ctypes.c_int, [_P]),

@@ -48,6 +48,7 @@ struct Workspace {
     size_t spillBytes = 0;
     void* stage = nullptr;  // rays | out | counts for host-pointer calls
     size_t stageBytes = 0;
+    uint64_t* hist = nullptr;  // rt_trace_hist counters of the instrumented launch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
 };
@@ -92,6 +93,7 @@ struct rt_scene {
     int32_t* triIds = nullptr;
     uint32_t* nodeEntry = nullptr;
     uint8_t* leafLc = nullptr;  // getLeafLoopCount per leaf block (hit reconstruction)
+    uint8_t* leafPrims = nullptr;  // real triangles per leaf block (visit statistics)
     uint64_t bytes = 0;
     uint32_t blockSize = 256, stackWindow = 16;
     uint32_t cus = 0;
@@ -258,6 +260,7 @@ void free_scene(rt_scene* s) {
         if (w.work) (void)hipFree(w.work);
         if (w.spill) (void)hipFree(w.spill);
         if (w.stage) (void)hipFree(w.stage);
+        if (w.hist) (void)hipFree(w.hist);
         if (w.ev0) (void)hipEventDestroy(w.ev0);
         if (w.ev1) (void)hipEventDestroy(w.ev1);
     }
@@ -266,6 +269,7 @@ void free_scene(rt_scene* s) {
     if (s->triIds) (void)hipFree(s->triIds);
     if (s->nodeEntry) (void)hipFree(s->nodeEntry);
     if (s->leafLc) (void)hipFree(s->leafLc);
+    if (s->leafPrims) (void)hipFree(s->leafPrims);
     if (s->ownStream) (void)hipStreamDestroy(s->ownStream);
     (void)hipSetDevice(cur);
     delete s;
@@ -327,7 +331,7 @@ extern "C" int rt_upload_bvh(const rt_bvh_desc* d, rt_scene_t* out) {
     std::vector<uint32_t> nodes, entry;
     std::vector<float> tris;
     std::vector<int32_t> triIds;
-    std::vector<uint8_t> leafLc;
+    std::vector<uint8_t> leafLc, leafPrims;
     LayoutInfo info;
     int rc;
     try {
@@ -336,6 +340,13 @@ extern "C" int rt_upload_bvh(const rt_bvh_desc* d, rt_scene_t* out) {
         rc = set_error(RT_ERR_NOMEM, "rt_upload_bvh: out of host memory");
     }
     if (rc != RT_OK) return rc;
+    // real triangles per leaf block: the slots with a triangle id (padding is -1)
+    leafPrims.assign(triIds.size() / d->leaf_memory, 0);
+    for (size_t l = 0; l < leafPrims.size(); ++l) {
+        uint32_t c = 0;
+        for (uint32_t j = 0; j < d->leaf_memory; ++j) c += triIds[l * d->leaf_memory + j] >= 0 ? 1u : 0u;
+        leafPrims[l] = (uint8_t)c;
+    }
     rt_scene* s = new (std::nothrow) rt_scene;
     if (!s) return set_error(RT_ERR_NOMEM, "rt_upload_bvh: out of host memory");
     (void)hipGetDevice(&s->device);
@@ -354,7 +365,8 @@ extern "C" int rt_upload_bvh(const rt_bvh_desc* d, rt_scene_t* out) {
     if ((rc = dev_upload(&s->nodes, nodes, s->bytes)) != RT_OK || (rc = dev_upload(&s->tris, tris, s->bytes)) != RT_OK ||
         (rc = dev_upload(&s->triIds, triIds, s->bytes)) != RT_OK ||
         (rc = dev_upload(&s->nodeEntry, entry, s->bytes)) != RT_OK ||
-        (rc = dev_upload(&s->leafLc, leafLc, s->bytes)) != RT_OK) {
+        (rc = dev_upload(&s->leafLc, leafLc, s->bytes)) != RT_OK ||
+        (rc = dev_upload(&s->leafPrims, leafPrims, s->bytes)) != RT_OK) {
         free_scene(s);
         return rc;
     }
@@ -392,8 +404,28 @@ extern "C" int rt_set_schedule(rt_scene_t s, uint32_t leaf_ratio16) {
     return RT_OK;
 }
 
+namespace {
+int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, void* out, rt_counts* counts,
+               void* hip_stream, rt_trace_hist* hist);
+}
+
 extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, void* out,
                         rt_counts* counts, void* hip_stream) {
+    return trace_impl(s, rays, n, flags, out, counts, hip_stream, nullptr);
+}
+
+extern "C" int rt_trace_histograms(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, void* out,
+                                   rt_trace_hist* hist, void* hip_stream) {
+    if (!hist) return set_error(RT_ERR_ARG, "rt_trace_histograms: null histogram");
+    if (flags & (RT_STREAM | RT_COOP | RT_COUNTS | RT_NO_SYNC))
+        return set_error(RT_ERR_ARG, "rt_trace_histograms: only RT_CLOSEST/RT_ANY/RT_SAVE_DIST and RT_DEVICE_PTRS");
+    std::memset(hist, 0, sizeof(*hist));
+    return trace_impl(s, rays, n, flags, out, nullptr, hip_stream, hist);
+}
+
+namespace {
+int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, void* out, rt_counts* counts,
+               void* hip_stream, rt_trace_hist* hist) {
     if (!s) return set_error(RT_ERR_ARG, "rt_trace: null scene");
     if (n == 0) return RT_OK;
     if (!rays || !out) return set_error(RT_ERR_ARG, "rt_trace: null rays or output");
@@ -427,16 +459,17 @@ extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t f
 
     const bool stream = (flags & RT_STREAM) != 0;
     const bool coop = (flags & RT_COOP) != 0;
-    const uint32_t S = s->stackWindow;
+    const uint32_t S = hist ? kHistWindow : s->stackWindow;
     const uint32_t W = coop_width(s->M, s->Lm);
     const uint32_t coopDepth = (stack_capacity(s) + 63) / 64 * 64;
     if (coop && (size_t)(256 / W) * coopDepth * 4 > 65536)
         return set_error(RT_ERR_UNSUPPORTED, "rt_trace: tree too deep for the cooperative kernel's LDS stack");
-    uint32_t bd = (stream || coop) ? 256u : s->blockSize;
-    while (!stream && !coop && bd > 64 && trace_lds_bytes(s->M, mode, S, bd) > 65536) bd -= 64;
-    const int occ = std::max(1, stream ? stream_occupancy(s->M, mode, S)
-                                       : (coop ? coop_occupancy(W, mode, coopDepth)
-                                               : trace_occupancy(s->M, mode, S, bd)));
+    uint32_t bd = hist ? trace_hist_block(s->M, mode) : ((stream || coop) ? 256u : s->blockSize);
+    while (!stream && !coop && !hist && bd > 64 && trace_lds_bytes(s->M, mode, S, bd) > 65536) bd -= 64;
+    const int occ = std::max(1, hist ? trace_hist_occupancy(s->M, mode)
+                                     : stream ? stream_occupancy(s->M, mode, S)
+                                              : (coop ? coop_occupancy(W, mode, coopDepth)
+                                                      : trace_occupancy(s->M, mode, S, bd)));
     const uint64_t wantBlocks = (uint64_t)s->cus * (uint64_t)occ;
     const uint64_t raysPerBlock = coop ? bd / W : bd;
     const uint64_t needBlocks = ((uint64_t)n + raysPerBlock - 1) / raysPerBlock;
@@ -444,7 +477,8 @@ extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t f
     const uint32_t cap = stack_capacity(s);
     const uint32_t spillCap = cap > S ? cap - S : 0;
     const size_t slots = (size_t)grid * bd;
-    const size_t spillNeed = std::max<size_t>(64, slots * spillCap * 4 * (mode == kSaveDist ? 2 : 1));
+    const size_t spillNeed =
+        std::max<size_t>(64, slots * spillCap * 4 * (hist ? 3 : (mode == kSaveDist ? 2 : 1)));
     if ((rc = ensure(reinterpret_cast<void**>(&w->spill), w->spillBytes, spillNeed, "hipMalloc(spill)")) != RT_OK)
         return rc;
 
@@ -487,9 +521,17 @@ extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t f
     p.chunk = ray_chunk(n, grid, bd);
     p.boxesBounded = s->boxesBounded ? 1u : 0u;
     p.sched = s->sched;
+    constexpr size_t kHistWords = sizeof(rt_trace_hist) / 8;
+    if (hist) {
+        if (!w->hist) HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w->hist), sizeof(rt_trace_hist)), "hipMalloc(hist)");
+        HIP_TRY(launch_zero(reinterpret_cast<uint32_t*>(w->hist), (uint32_t)(kHistWords * 2), st), "zero kernel");
+        p.hist = w->hist;
+        p.leafPrims = s->leafPrims;
+    }
     HIP_TRY(launch_reset(w->work, st), "reset kernel");
     HIP_TRY(hipEventRecord(w->ev0, st), "hipEventRecord");
-    if (coop) HIP_TRY(launch_coop(W, mode, grid, p, st), "cooperative kernel launch");
+    if (hist) HIP_TRY(launch_trace_hist(s->M, mode, grid, p, st), "instrumented kernel launch");
+    else if (coop) HIP_TRY(launch_coop(W, mode, grid, p, st), "cooperative kernel launch");
     else if (stream) HIP_TRY(launch_stream(s->M, mode, S, grid, p, st), "stream kernel launch");
     else HIP_TRY(launch_trace(s->M, mode, S, bd, grid, p, st), "trace kernel launch");
     HIP_TRY(hipEventRecord(w->ev1, st), "hipEventRecord");
@@ -507,8 +549,18 @@ extern "C" int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t f
         HIP_TRY(hipMemcpy(&errs, w->work + kErrWord, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
         s->errorRays += errs;
     }
+    if (hist) {
+        HIP_TRY(hipMemcpy(hist, w->hist, sizeof(rt_trace_hist), hipMemcpyDeviceToHost), "hipMemcpy(hist)");
+        hist->rays = n;
+        hist->node_visits = hist->leaf_visits = 0;
+        for (int i = 0; i < RT_HIST_DEPTHS; ++i) {
+            hist->node_visits += hist->node_depth[i];
+            hist->leaf_visits += hist->leaf_depth[i];
+        }
+    }
     return RT_OK;
 }
+}  // namespace
 
 extern "C" int rt_stats(rt_scene_t s, rt_stats_t* o) {
     if (!s || !o) return set_error(RT_ERR_ARG, "rt_stats: null argument");

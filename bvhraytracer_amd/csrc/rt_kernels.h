@@ -64,6 +64,10 @@ struct KParams {
     uint32_t nodeCount;
     uint32_t chunk;  // rays reserved per wave per atomicAdd on the counter
     uint32_t boxesBounded;  // every tested box coordinate is within +-2^100
+    // visit statistics launch only: real triangles per leaf block, and the
+    // rt_trace_hist counters (uint64) the kernel adds to
+    const uint8_t* leafPrims;
+    uint64_t* hist;
     // per-ray kernel wave schedule: 0 = every lane steps each iteration;
     // r > 0 = one phase per iteration, the leaf phase when
     // 16 * leafLanes >= r * nodeLanes
@@ -101,5 +105,15 @@ hipError_t launch_coop(uint32_t W, int mode, uint32_t grid, const KParams& p, hi
 int coop_occupancy(uint32_t W, int mode, uint32_t depth);
 
 hipError_t launch_reset(uint32_t* work, hipStream_t stream);
+// Zeroes `words` 32-bit words with a kernel (no memset node if captured).
+hipError_t launch_zero(uint32_t* p, uint32_t words, hipStream_t stream);
+
+// Instrumented per-ray kernel (closest / any / save distance): blocks of
+// trace_hist_block() threads with a 16-entry LDS stack window; p.hist and
+// p.leafPrims set.
+constexpr uint32_t kHistWindow = 16;
+hipError_t launch_trace_hist(uint32_t M, int mode, uint32_t grid, const KParams& p, hipStream_t stream);
+uint32_t trace_hist_block(uint32_t M, int mode);
+int trace_hist_occupancy(uint32_t M, int mode);
 
 }  // namespace rt

@@ -140,6 +140,7 @@ struct SceneCtx {
     const float* tris;
     const int32_t* triIds;
     const uint32_t* nodeEntry;
+    const uint8_t* leafPrims;  // real triangles per leaf block (visit statistics)
     uint32_t rootEntry, G, Lm, nodeCount, M, recw, ow;
     bool boxesBounded;  // every tested box coordinate is within +-2^100
 };
@@ -152,6 +153,11 @@ struct StackCtx {
     float* spD;
     uint32_t cap;    // stack capacity (LDS window + spill)
     uint32_t stride;
+    // visit statistics only: depth of each stack entry (LDS window and
+    // spill, laid out like stE/spE) and the block's LDS counters
+    uint32_t* stDep;
+    uint32_t* spDep;
+    uint32_t* hist;
 };
 
 struct Trav {
@@ -165,6 +171,7 @@ struct Trav {
     uint32_t cN, cL, cNS, cLS;
     bool hasCur, occl, err, anyMode, rev, fast;
     int axis;
+    uint32_t curDepth;  // visit statistics: depth of `cur` (root = 0)
 };
 
 __device__ __forceinline__ void trav_init(Trav& t, const rt_ray& in, uint32_t rootEntry, bool anyMode,
@@ -185,6 +192,7 @@ __device__ __forceinline__ void trav_init(Trav& t, const rt_ray& in, uint32_t ro
     t.fast = boxesBounded && ray_fast(t.ray);
     t.cur = rootEntry;
     t.curDist = 0.0f;
+    t.curDepth = 0;
     t.hasCur = true;
     t.sp = 0;
     t.live = 0;
@@ -204,8 +212,23 @@ __device__ __forceinline__ uint32_t trav_peek(const Trav& t, const StackCtx& k) 
 
 __device__ __forceinline__ bool trav_done(const Trav& t) { return !t.hasCur && t.sp == 0; }
 
+// Visit statistics (rt_trace_histograms): LDS counter words of a block.
+constexpr uint32_t kHistDepthBins = 64, kHistFullBins = 17;
+enum HistWord : uint32_t {
+    kHwNodeDepth = 0,
+    kHwLeafDepth = kHwNodeDepth + kHistDepthBins,
+    kHwChildFull = kHwLeafDepth + kHistDepthBins,
+    kHwLeafFull = kHwChildFull + kHistFullBins,
+    kHwAabbTests = kHwLeafFull + kHistFullBins,
+    kHwAabbHits,
+    kHwTriTests,
+    kHwTriHits,
+    kHwWords
+};
+
 // One traversal step: take the next entry and visit it (node or leaf).
-template <int MT, int MODE, int S>
+// HIST adds the visit statistics (depth per stack entry, LDS counters).
+template <int MT, int MODE, int S, bool HIST = false>
 __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk, Trav& t) {
     const uint32_t M = MT ? (uint32_t)MT : sc.M;
     const uint32_t recw = MT ? record_words(MT) : sc.recw;
@@ -213,18 +236,22 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
     const uint32_t bd = sk.stride;
     uint32_t e;
     float edist = 0.0f;
+    uint32_t edep = 0;
     if (t.hasCur) {
         e = t.cur;
         edist = t.curDist;
+        if constexpr (HIST) edep = t.curDepth;
         t.hasCur = false;
     } else {
         --t.sp;
         const uint32_t slot = (t.sp & (S - 1)) * bd;
         e = sk.stE[slot];
         if (MODE == kSaveDist) edist = sk.stD[slot];
+        if constexpr (HIST) edep = sk.stDep[slot];
         if (t.sp >= S) {
             sk.stE[slot] = sk.spE[t.sp - S];
             if (MODE == kSaveDist) sk.stD[slot] = sk.spD[t.sp - S];
+            if constexpr (HIST) sk.stDep[slot] = sk.spDep[t.sp - S];
         }
     }
     if (MODE == kSaveDist && edist > t.ray.tmax) return;
@@ -238,6 +265,8 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
         const float* T = sc.tris + (size_t)idx * leaf_stride(sc.Lm);
         ++t.cL;
         t.cLS += k;
+        uint32_t prims = 0, triHits = 0;
+        if constexpr (HIST) prims = sc.leafPrims[idx];
         if (t.anyMode) {
             bool any = false;
             for (uint32_t s0 = 0; s0 < k; s0 += LEAF_CHUNK) {
@@ -247,8 +276,10 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
                 for (int j = 0; j < LEAF_CHUNK; ++j) {
                     float tt, b1, b2;
                     if (mt(a.v(0, j), a.v(1, j), a.v(2, j), a.v(3, j), a.v(4, j), a.v(5, j), a.v(6, j), a.v(7, j),
-                           a.v(8, j), t.ray, tt, b1, b2))
+                           a.v(8, j), t.ray, tt, b1, b2)) {
                         any = true;
+                        if constexpr (HIST) triHits += (s0 + (uint32_t)j < prims) ? 1u : 0u;
+                    }
                 }
             }
             if (any) {
@@ -272,6 +303,7 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
                     if (mt(a.v(0, j), a.v(1, j), a.v(2, j), a.v(3, j), a.v(4, j), a.v(5, j), a.v(6, j), a.v(7, j),
                            a.v(8, j), t.ray, tt, b1, b2)) {
                         const uint32_t s = s0 + (uint32_t)j;
+                        if constexpr (HIST) triHits += (s < prims) ? 1u : 0u;
                         const uint32_t g = 1u << (s & (sc.G - 1u));
                         if (tt < bt) {
                             bt = tt;
@@ -289,6 +321,12 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
                 t.hitEntry = e;
                 t.hitSlot = bidx;
             }
+        }
+        if constexpr (HIST) {
+            atomicAdd(&sk.hist[kHwLeafDepth + min(edep, kHistDepthBins - 1)], 1u);
+            atomicAdd(&sk.hist[kHwLeafFull + min(prims, kHistFullBins - 1)], 1u);
+            atomicAdd(&sk.hist[kHwTriTests], prims);
+            atomicAdd(&sk.hist[kHwTriHits], triHits);
         }
         return;
     }
@@ -323,6 +361,13 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
     if (t.fast) boxes(std::true_type{});
     else boxes(std::false_type{});
     t.live |= H;
+    if constexpr (HIST) {
+        const uint32_t cc = (h0.z >> 16) & 31u;
+        atomicAdd(&sk.hist[kHwNodeDepth + min(edep, kHistDepthBins - 1)], 1u);
+        atomicAdd(&sk.hist[kHwChildFull + min(cc, kHistFullBins - 1)], 1u);
+        atomicAdd(&sk.hist[kHwAabbTests], cc);
+        atomicAdd(&sk.hist[kHwAabbHits], (uint32_t)__popc(H & ((1u << cc) - 1u)));
+    }
     if (MODE != kSaveDist && H == 0) return;
     const uint32_t innerBase = h0.x, leafBase = h0.y;
     const uint32_t leafMask = h0.z & 0xFFFFu, c = (h0.z >> 16) & 31u;
@@ -357,9 +402,14 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
             }
             sk.stE[sl] = t.cur;
             if (MODE == kSaveDist) sk.stD[sl] = t.curDist;
+            if constexpr (HIST) {
+                if (t.sp >= S) sk.spDep[t.sp - S] = sk.stDep[sl];
+                sk.stDep[sl] = t.curDepth;
+            }
             ++t.sp;
         }
         t.cur = e;
+        if constexpr (HIST) t.curDepth = edep + 1u;
         if (MODE == kSaveDist) {
             t.curDist = sk.aabbD[s * bd];
             sk.aabbD[s * bd] = __builtin_nanf("");
@@ -472,6 +522,7 @@ __device__ __forceinline__ SceneCtx plain_scene(const KParams& p, uint32_t MT) {
     sc.G = p.gang;
     sc.Lm = p.Lm;
     sc.nodeCount = p.nodeCount;
+    sc.leafPrims = p.leafPrims;
     sc.boxesBounded = p.boxesBounded != 0;
     sc.M = MT;
     sc.recw = record_words(MT);
@@ -525,12 +576,9 @@ struct WaveQueue {
 // (BATCH: every lane carries the scene and item of its current ray; MT = 0
 // reads the node memory from the scene; MODE = kMixed picks closest/any per
 // item).
-// Register budget of the plain launch: at most 96 VGPRs, 5 waves per SIMD
-// (measured faster than the unconstrained 100-VGPR build at 4 waves; 6 waves
-// spills).  The batched launch keeps its per-lane scene context in registers
-// and stays at 4 waves.
-template <int MT, int MODE, int S, bool BATCH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BATCH ? 4 : 5))) k_trace(KParams p) {
+// Body of the per-ray kernel; HIST adds the visit statistics.
+template <int MT, int MODE, int S, bool BATCH, bool HIST>
+__device__ __forceinline__ void trace_body(const KParams& p) {
     extern __shared__ uint32_t smem[];
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
     const size_t gslot = (size_t)blockIdx.x * bd + tid;
@@ -543,6 +591,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BATCH 
     sk.spD = reinterpret_cast<float*>(p.spill + nslots * p.spillCap) + gslot * p.spillCap;
     sk.cap = S + p.spillCap;
     sk.stride = bd;
+    sk.stDep = nullptr;
+    sk.spDep = nullptr;
+    sk.hist = nullptr;
+    if constexpr (HIST) {
+        // LDS after the stack window (and the save-distance arrays): depth
+        // window, then the block's counters; spill: a third region
+        const uint32_t base = (MODE == kSaveDist) ? 2 * S * bd + (uint32_t)MT * bd : S * bd;
+        sk.stDep = smem + base + tid;
+        sk.spDep = p.spill + 2 * nslots * p.spillCap + gslot * p.spillCap;
+        sk.hist = smem + base + S * bd;
+        for (uint32_t w = tid; w < kHwWords; w += bd) sk.hist[w] = 0;
+        __syncthreads();
+    }
 
     SceneCtx sc = plain_scene(p, MT);
     const rt_ray* rays = p.rays;
@@ -607,11 +668,42 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BATCH 
             const uint32_t nn = (uint32_t)__popcll(__ballot(!leafNext));
             if (leafNext != (16u * nl >= p.sched * nn)) continue;
         }
-        trav_step<MT, MODE, S>(sc, sk, t);
+        trav_step<MT, MODE, S, HIST>(sc, sk, t);
         if (!trav_done(t)) continue;
         trav_finish(sc, t, hits, occluded, counts, r, p.work + kErrWord);
         has = false;
     }
+    if constexpr (HIST) {
+        // flush the block's counters into the rt_trace_hist layout
+        __syncthreads();
+        for (uint32_t w = tid; w < kHwWords; w += bd) {
+            const uint32_t v = sk.hist[w];
+            if (v == 0) continue;
+            uint32_t g;
+            if (w < kHwLeafDepth) g = 7 + (w - kHwNodeDepth);
+            else if (w < kHwChildFull) g = 7 + kHistDepthBins + (w - kHwLeafDepth);
+            else if (w < kHwLeafFull) g = 7 + 2 * kHistDepthBins + (w - kHwChildFull);
+            else if (w < kHwAabbTests) g = 7 + 2 * kHistDepthBins + kHistFullBins + (w - kHwLeafFull);
+            else g = 3 + (w - kHwAabbTests);
+            atomicAdd(reinterpret_cast<unsigned long long*>(p.hist) + g, (unsigned long long)v);
+        }
+    }
+}
+
+// Register budget of the plain launch: at most 96 VGPRs, 5 waves per SIMD
+// (measured faster than the unconstrained 100-VGPR build at 4 waves; 6 waves
+// spills).  The batched launch keeps its per-lane scene context in registers
+// and stays at 4 waves.
+template <int MT, int MODE, int S, bool BATCH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BATCH ? 4 : 5))) k_trace(KParams p) {
+    trace_body<MT, MODE, S, BATCH, false>(p);
+}
+
+// Instrumented per-ray kernel (rt_trace_histograms): same traversal, plus
+// visit counts by depth, child/leaf fullness and test/success counts.
+template <int MT, int MODE>
+__global__ void __launch_bounds__(256) k_trace_hist(KParams p) {
+    trace_body<MT, MODE, kHistWindow, false, true>(p);
 }
 
 // Ray-stream kernel: the GPU form of intersectWideAlternative
@@ -1002,6 +1094,10 @@ __global__ void k_reset(uint32_t* work) {
     for (uint32_t i = threadIdx.x; i < kWorkWords; i += blockDim.x) work[i] = 0;
 }
 
+__global__ void k_zero(uint32_t* p, uint32_t words) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
 template <int M, int MODE, int S>
 void* kernel_ptr() {
     return reinterpret_cast<void*>(&k_trace<M, MODE, S, false>);
@@ -1099,6 +1195,60 @@ hipError_t launch_trace(uint32_t M, int mode, uint32_t S, uint32_t bd, uint32_t 
 
 size_t trace_lds_bytes(uint32_t M, int mode, uint32_t S, uint32_t bd) { return lds_bytes(M, mode, S, bd); }
 
+namespace {
+
+template <int M>
+void* pick_hist_mode(int mode) {
+    switch (mode) {
+        case kClosest: return reinterpret_cast<void*>(&k_trace_hist<M, kClosest>);
+        case kAny: return reinterpret_cast<void*>(&k_trace_hist<M, kAny>);
+        case kSaveDist: return reinterpret_cast<void*>(&k_trace_hist<M, kSaveDist>);
+        default: return nullptr;
+    }
+}
+
+void* pick_hist(uint32_t M, int mode) {
+    switch (M) {
+        case 4: return pick_hist_mode<4>(mode);
+        case 8: return pick_hist_mode<8>(mode);
+        case 12: return pick_hist_mode<12>(mode);
+        case 16: return pick_hist_mode<16>(mode);
+        default: return nullptr;
+    }
+}
+
+}  // namespace
+
+size_t trace_hist_lds_bytes(uint32_t M, int mode, uint32_t bd) {
+    return lds_bytes(M, mode, kHistWindow, bd) + (size_t)kHistWindow * bd * 4 + (size_t)kHwWords * 4;
+}
+
+uint32_t trace_hist_block(uint32_t M, int mode) {
+    uint32_t bd = 256;
+    while (bd > 64 && trace_hist_lds_bytes(M, mode, bd) > 65536) bd -= 64;
+    return bd;
+}
+
+int trace_hist_occupancy(uint32_t M, int mode) {
+    void* fn = pick_hist(M, mode);
+    if (!fn) return 0;
+    const uint32_t bd = trace_hist_block(M, mode);
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(fn), (int)bd,
+                                                     trace_hist_lds_bytes(M, mode, bd)) != hipSuccess)
+        return 0;
+    return blocks;
+}
+
+hipError_t launch_trace_hist(uint32_t M, int mode, uint32_t grid, const KParams& p, hipStream_t stream) {
+    void* fn = pick_hist(M, mode);
+    if (!fn || !p.hist || !p.leafPrims) return hipErrorInvalidValue;
+    const uint32_t bd = trace_hist_block(M, mode);
+    KParams copy = p;
+    void* args[] = {&copy};
+    return hipLaunchKernel(fn, dim3(grid), dim3(bd), args, trace_hist_lds_bytes(M, mode, bd), stream);
+}
+
 int trace_occupancy(uint32_t M, int mode, uint32_t S, uint32_t bd) {
     void* fn = pick(M, mode, S);
     if (!fn) return 0;
@@ -1152,6 +1302,12 @@ int coop_occupancy(uint32_t W, int mode, uint32_t depth) {
                                                      (size_t)(256 / W) * depth * 4) != hipSuccess)
         return 0;
     return blocks;
+}
+
+hipError_t launch_zero(uint32_t* p, uint32_t words, hipStream_t stream) {
+    hipLaunchKernelGGL(k_zero, dim3(std::max(1u, std::min(1024u, (words + 255) / 256))), dim3(256), 0, stream, p,
+                       words);
+    return hipGetLastError();
 }
 
 hipError_t launch_reset(uint32_t* work, hipStream_t stream) {

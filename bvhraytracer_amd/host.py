@@ -213,6 +213,21 @@ class Scene:
         check(lib().rt_trace(self._h, ptr(rays), n, flags, ptr(out),
                              ptr(counts) if counts is not None else None, s))
 
+    def histograms(self, rays, mode="closest"):
+        """rt_trace_histograms on host rays: returns (out, stats) where out is
+        what intersect() returns and stats a dict of the visit counters
+        (numpy arrays for the histograms)."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        n = len(rays)
+        out = np.empty(n, np.uint8) if mode == "any" else np.empty(n, HIT_DTYPE)
+        h = _lib.RtTraceHist()
+        check(lib().rt_trace_histograms(self._h, ptr(rays), n, MODES[mode], ptr(out), ctypes.byref(h), None))
+        stats = {}
+        for f, _ in _lib.RtTraceHist._fields_:
+            v = getattr(h, f)
+            stats[f] = np.array(v[:], np.int64) if hasattr(v, "_length_") else int(v)
+        return out, stats
+
     def surface_device(self, rays, hits, normal=None, position=None, stream=None):
         """rt_hit_surface_device: hit reconstruction from the resident
         triangles.  rays (n,10) float32, hits (n,6) int32 CUDA tensors;

@@ -176,6 +176,57 @@ def write_perf(path, name, problem, gang, node_memory, leaf_memory, branching, l
         f.write(f"Triangle Count: {tri_count}\n")
 
 
+def _info_block(f, label, st):
+    rays = max(1, st["rays"])
+    f.write(f"{label} intersections node: {st['node_visits'] / rays}\n")
+    f.write(f"{label} aabb intersections: {st['aabb_tests'] / rays}\n")
+    f.write(f"{label} aabb success ration: {st['aabb_hits'] / max(1, st['aabb_tests'])}\n")
+    waste = (st["leaf_visits"] + st["node_visits"] - st["rays"]) / max(1, st["node_visits"] * st["branching"])
+    f.write(f"{label} waste factor: {1 - waste}\n")
+    f.write(f"{label} intersections leaf: {st['leaf_visits'] / rays}\n")
+    f.write(f"{label} primitive intersections: {st['tri_tests'] / rays}\n")
+    f.write(f"{label} primitive success ratio: {st['tri_hits'] / max(1, st['tri_tests'])}\n")
+
+
+def _trim(h):
+    nz = np.nonzero(h)[0]
+    return h[: nz[-1] + 1] if len(nz) else h[:1]
+
+
+def write_info(path, name, branching, leaf_size, primary, secondary):
+    """The _Info.txt of the reference's instrumented renderer
+    (cameraData.h:241-329), from rt_trace_histograms counters of a primary
+    and a secondary (shadow) batch: per-ray averages, success ratios, the
+    waste factor, child/leaf fullness and visits per depth, with the same
+    line keys, so the reference's Analysis/DataManager.py reads it."""
+    p = dict(primary, branching=branching)
+    s = dict(secondary, branching=branching)
+    with open(path, "w") as f:
+        f.write(f"scenario {name} with branching factor of {branching} and leafsize of {leaf_size}\n")
+        f.write("intersections counts are normalized per Ray\n\n")
+        _info_block(f, "primary", p)
+        f.write("\n")
+        _info_block(f, "secondary", s)
+        f.write("\n")
+        f.write("intersections with nodes with x children :\n")
+        cf = _trim(p["child_fullness"])
+        f.write(f"average child fullness: {float((cf * np.arange(len(cf))).sum() / max(1, cf.sum())):.6f}\n")
+        for i, v in enumerate(cf):
+            f.write(f"{i} : {v / max(1, p['rays'])}\n")
+        f.write("\n")
+        f.write("intersections with leaf nodes with x primitives :\n")
+        lf = _trim(p["leaf_fullness"])
+        f.write(f"average leaf fullness: {float((lf * np.arange(len(lf))).sum() / max(1, lf.sum())):.6f}\n")
+        for i, v in enumerate(lf):
+            f.write(f"{i} : {v / max(1, p['rays'])}\n")
+        for label, st, key in (("primary node", p, "node_depth"), ("primary leaf", p, "leaf_depth"),
+                               ("secondary node", s, "node_depth"), ("secondary leaf", s, "leaf_depth")):
+            f.write("\n")
+            f.write(f"{label} intersections at depth x :\n")
+            for i, v in enumerate(_trim(st[key])):
+                f.write(f"{i} : {v / max(1, st['rays'])}\n")
+
+
 SCENES = {
     "spheres": dict(make=lambda: spheres_scene(0), cam=((0.0, 4.0, 32.0), (0.0, 0.0, 0.0))),
     "interior": dict(make=lambda: interior_scene(2, 1_000_000), cam=((0.0, 5.0, -14.0), (0.0, 3.0, 0.0))),

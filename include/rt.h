@@ -148,6 +148,32 @@ typedef struct rt_stats_t {
 
 int rt_stats(rt_scene_t s, rt_stats_t* out);
 
+/* Visit statistics of one traced batch (the counters of the reference's
+ * instrumented renderer, cameraData.h:241-329, kept over this traversal):
+ * visits of inner nodes and leaves by depth (root = depth 0), visits by the
+ * visited node's child count and the visited leaf's triangle count, child
+ * boxes tested / hit, and triangles tested / passing the test.  Tests count
+ * real children and triangles (not the padded slots).  Totals, not per ray. */
+#define RT_HIST_DEPTHS 64  /* depths >= 63 share the last bin */
+#define RT_HIST_FULL 17
+typedef struct rt_trace_hist {
+    uint64_t rays;
+    uint64_t node_visits, leaf_visits;
+    uint64_t aabb_tests, aabb_hits;
+    uint64_t tri_tests, tri_hits;
+    uint64_t node_depth[RT_HIST_DEPTHS];
+    uint64_t leaf_depth[RT_HIST_DEPTHS];
+    uint64_t child_fullness[RT_HIST_FULL];
+    uint64_t leaf_fullness[RT_HIST_FULL];
+} rt_trace_hist;
+
+/* Traces like rt_trace (flags: RT_CLOSEST / RT_ANY / RT_SAVE_DIST,
+ * RT_DEVICE_PTRS) with the instrumented kernel and fills *hist (host memory;
+ * the call waits for the result).  `out` receives the same hits / occlusion
+ * flags as rt_trace. */
+int rt_trace_histograms(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, void* out,
+                        rt_trace_hist* hist, void* hip_stream);
+
 /* Overrides the launch configuration (block size 64, 128, 192 or 256,
  * LDS stack window 8/16/32 entries).  0 keeps the current value. */
 int rt_set_launch(rt_scene_t s, uint32_t block_size, uint32_t stack_window);

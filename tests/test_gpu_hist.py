@@ -1,0 +1,91 @@
+"""Visit statistics of the instrumented kernel (rt_trace_histograms): exact
+counts on hand-built trees whose visit sequences are worked out by hand,
+and on a real scene the totals agree with the per-ray counts of rt_trace
+(which the oracle pins) and the histograms are consistent with each other.
+Results of the instrumented launch equal rt_trace's."""
+import numpy as np
+import pytest
+
+from tests import handbvh
+from tests.helpers import assert_closest_equal, oracle_for
+
+pytestmark = pytest.mark.gpu
+
+
+def fray(o, d, tmax=222222.0):
+    import bvhraytracer_amd as rt
+
+    return rt.make_rays(np.float32([o]), np.float32([o]) + np.float32([d]), tmax)
+
+
+def scene_of(rt, arr):
+    a = rt.FastNodeArrays(arr["node_memory"], arr["leaf_memory"], arr["gang"], arr["bounds"], arr["base"],
+                          arr["order"], arr["child_type"], arr["tri_points"], arr["tri_ids"])
+    return rt.Scene(a)
+
+
+def test_hist_stale_tree(rt):
+    """stale_tree, ray from the origin along +x: root (3 real children; A
+    hit, far1/far2 missed; the zero padding box is hit but not a child), then
+    A (4 children, slots 0 and 1 hit) and leaves A0, A3 and A1 in that order
+    (A3 pushed by the root's stale slot-3 distance), one triangle each.  A0
+    passes at t = 0.25; A1's triangle (t = 0.375) then fails t <= tMax."""
+    scene = scene_of(rt, handbvh.stale_tree())
+    hits, st = scene.histograms(fray([0, 0, 0], [8, 0, 0]))
+    assert hits["slot"][0] == 0
+    assert st["rays"] == 1 and st["node_visits"] == 2 and st["leaf_visits"] == 3
+    assert st["aabb_tests"] == 7 and st["aabb_hits"] == 3
+    assert st["tri_tests"] == 3 and st["tri_hits"] == 1
+    assert list(st["node_depth"][:3]) == [1, 1, 0] and list(st["leaf_depth"][:3]) == [0, 0, 3]
+    assert st["child_fullness"][3] == 1 and st["child_fullness"][4] == 1 and st["child_fullness"].sum() == 2
+    assert st["leaf_fullness"][1] == 3 and st["leaf_fullness"].sum() == 3
+    # any-hit: the stale slot 3 pushes A's first leaf directly, which occludes
+    occ, st = scene.histograms(fray([0, 0, 0], [8, 0, 0]), mode="any")
+    assert occ[0] == 1 and st["node_visits"] == 1 and st["leaf_visits"] == 1
+    assert st["leaf_depth"][1] == 1 and st["tri_hits"] == 1
+
+
+def test_hist_quirk_tree(rt):
+    """quirk_tree(full=False) along +x: root with 3 children (slots 0 and 1
+    hold the same triangle, hit; slot 2 missed), both leaves visited."""
+    scene = scene_of(rt, handbvh.quirk_tree(False))
+    _, st = scene.histograms(fray([0, 0.25, 0.25], [8, 0, 0]))
+    assert st["node_visits"] == 1 and st["leaf_visits"] == 2
+    assert st["aabb_tests"] == 3 and st["aabb_hits"] == 2
+    assert st["tri_tests"] == 2 and st["tri_hits"] == 2
+    assert st["node_depth"][0] == 1 and st["leaf_depth"][1] == 2 and st["child_fullness"][3] == 1
+
+
+@pytest.mark.parametrize("mode", ["closest", "any", "save_distance"])
+@pytest.mark.parametrize("N,L", [(8, 8), (4, 16)])
+def test_hist_consistent_with_counts(rt, mode, N, L):
+    tris = rt.interior_scene(37, 40_000, 20)
+    bvh = rt.Bvh(tris, N, L)
+    scene = rt.Scene(bvh)
+    prim = rt.camera_rays((0.0, 5.0, -14.0), (0.0, 3.0, 0.0), 128, 96, 16)
+    if mode == "any":
+        h = scene.intersect(prim)
+        nrm, pos = bvh.surface(prim, h)
+        rays, _ = rt.spawn_shadow(nrm, pos, h, (0.0, 9.5, 0.0), 1.0)
+    else:
+        rays = prim
+    print(f"histograms {mode} N{N}L{L}: {len(rays)} rays", flush=True)
+    out, st = scene.histograms(rays, mode=mode)
+    ref, cnt = scene.intersect(rays, mode=mode, counts=True)
+    assert np.array_equal(out, ref)
+    assert st["rays"] == len(rays)
+    assert st["node_visits"] == cnt["nodes"].sum() and st["leaf_visits"] == cnt["leaves"].sum()
+    assert st["node_depth"].sum() == st["node_visits"] and st["leaf_depth"].sum() == st["leaf_visits"]
+    assert st["child_fullness"].sum() == st["node_visits"] and st["leaf_fullness"].sum() == st["leaf_visits"]
+    k = np.arange(17)
+    assert (st["child_fullness"] * k).sum() == st["aabb_tests"]
+    assert (st["leaf_fullness"] * k).sum() == st["tri_tests"]
+    # every ray starts at the root (an inner node); leaves lie below it
+    assert st["node_depth"][0] == len(rays) and st["leaf_depth"][0] == 0
+    assert st["child_fullness"][N + 1:].sum() == 0 and st["leaf_fullness"][L + 1:].sum() == 0
+    assert 0 < st["aabb_hits"] <= st["aabb_tests"] and 0 < st["tri_hits"] <= st["tri_tests"]
+    if mode != "any":
+        assert st["tri_hits"] >= (out["slot"] >= 0).sum()
+        r = oracle_for(bvh).trace(rays)
+        if mode == "closest":
+            assert_closest_equal(out, r["hits"], bvh, cnt, r["counts"], "hist")
