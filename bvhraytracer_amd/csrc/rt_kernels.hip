@@ -449,7 +449,18 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
         const bool pad0 = !t.rev && c < M && (live & 1u);
         if (pad0) live &= ~1u;
         uint32_t pm = 0;  // row positions q < c whose slot is live
-        for (uint32_t q = 0; q < c; ++q) pm |= ((live >> ((uint32_t)(row >> (4 * q)) & 15u)) & 1u) << q;
+        if constexpr (MT > 0) {
+            // unrolled over the node memory; positions >= c masked off
+            const uint32_t lo = (uint32_t)row, hi = (uint32_t)(row >> 32);
+#pragma unroll
+            for (uint32_t q = 0; q < (uint32_t)MT; ++q) {
+                const uint32_t slotq = __builtin_amdgcn_ubfe(q < 8 ? lo : hi, 4 * (q & 7u), 4);
+                pm |= __builtin_amdgcn_ubfe(live, slotq, 1) << q;
+            }
+            pm &= (1u << c) - 1u;
+        } else {
+            for (uint32_t q = 0; q < c; ++q) pm |= ((live >> ((uint32_t)(row >> (4 * q)) & 15u)) & 1u) << q;
+        }
         if (!fits((uint32_t)__popc(pm) + (pad0 ? 1u : 0u))) {
             t.err = true;
         } else {
@@ -457,10 +468,12 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
             // one loop for both directions (forward takes the lowest position
             // first, backward the highest), so lanes of either kind share
             // the iterations; row positions q < c hold real slots only
+            const uint32_t rowLo = (uint32_t)row, rowHi = (uint32_t)(row >> 32);
             while (pm) {
                 const uint32_t q = t.rev ? (uint32_t)__builtin_ctz(pm) : 31u - (uint32_t)__builtin_clz(pm);
                 pm &= ~(1u << q);
-                const uint32_t s = (uint32_t)(row >> (4 * q)) & 15u;
+                const uint32_t s = (MT > 0 && MT <= 8) ? __builtin_amdgcn_ubfe(rowLo, 4 * q, 4)
+                                                       : __builtin_amdgcn_ubfe(q < 8 ? rowLo : rowHi, 4 * (q & 7u), 4);
                 push(entry_real(s), s);
             }
         }
