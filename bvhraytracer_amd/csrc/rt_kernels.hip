@@ -60,11 +60,16 @@ __device__ __forceinline__ bool slab_fast(float bnx, float bny, float bnz, float
     return t0 <= t1;
 }
 
-// Per-ray condition for slab_fast (the box bound is checked per scene).
+// Per-ray condition for slab_fast (the box bound is checked per scene) and
+// for skipping a leaf's trailing all-zero triangles: with a finite direction
+// such a triangle has e1 = e2 = 0, so s1 = 0 and divisor = 0, and it never
+// passes the test (an infinite direction component would make the divisor
+// NaN, which passes).
 __device__ __forceinline__ bool ray_fast(const Ray& r) {
     constexpr float kMaxO = 0x1p100f, kMaxF = 3.40282347e38f;
     return fabsf(r.ox) <= kMaxO && fabsf(r.oy) <= kMaxO && fabsf(r.oz) <= kMaxO && fabsf(r.ix) <= kMaxF &&
-           fabsf(r.iy) <= kMaxF && fabsf(r.iz) <= kMaxF && r.tmax == r.tmax;
+           fabsf(r.iy) <= kMaxF && fabsf(r.iz) <= kMaxF && fabsf(r.dx) <= kMaxF && fabsf(r.dy) <= kMaxF &&
+           fabsf(r.dz) <= kMaxF && r.tmax == r.tmax;
 }
 
 // Moeller-Trumbore body of triIntersect / triAnyHit (rayTracerTemplate.ispc:49-75).
@@ -141,6 +146,7 @@ struct SceneCtx {
     const int32_t* triIds;
     const uint32_t* nodeEntry;
     const uint8_t* leafPrims;  // real triangles per leaf block (visit statistics)
+    const uint8_t* leafTest;   // slots to test per leaf block for finite directions
     uint32_t rootEntry, G, Lm, nodeCount, M, recw, ow;
     bool boxesBounded;  // every tested box coordinate is within +-2^100
 };
@@ -267,9 +273,13 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
         t.cLS += k;
         uint32_t prims = 0, triHits = 0;
         if constexpr (HIST) prims = sc.leafPrims[idx];
+        // slots to test: all k, or for a finite direction up to the last
+        // triangle that is not all zeros (chunk 0 is always tested, so its
+        // loads do not wait for this count)
+        const uint32_t kt = t.fast ? min(k, (uint32_t)sc.leafTest[idx]) : k;
         if (t.anyMode) {
             bool any = false;
-            for (uint32_t s0 = 0; s0 < k; s0 += LEAF_CHUNK) {
+            for (uint32_t s0 = 0; s0 == 0 || s0 < kt; s0 += LEAF_CHUNK) {
                 LeafChunk a;
                 a.load(T, k, s0);
 #pragma unroll
@@ -294,7 +304,7 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
             float bt = 1000000.0f;
             int bidx = -1;
             uint32_t gm = (1u << sc.G) - 1u;  // instances whose tStore == bt
-            for (uint32_t s0 = 0; s0 < k; s0 += LEAF_CHUNK) {
+            for (uint32_t s0 = 0; s0 == 0 || s0 < kt; s0 += LEAF_CHUNK) {
                 LeafChunk a;
                 a.load(T, k, s0);
 #pragma unroll
@@ -536,6 +546,7 @@ __device__ __forceinline__ SceneCtx plain_scene(const KParams& p, uint32_t MT) {
     sc.Lm = p.Lm;
     sc.nodeCount = p.nodeCount;
     sc.leafPrims = p.leafPrims;
+    sc.leafTest = p.leafTest;
     sc.boxesBounded = p.boxesBounded != 0;
     sc.M = MT;
     sc.recw = record_words(MT);
@@ -655,6 +666,7 @@ __device__ __forceinline__ void trace_body(const KParams& p) {
                     sc.Lm = bs.Lm;
                     sc.nodeCount = bs.nodeCount;
                     sc.boxesBounded = bs.boxesBounded != 0;
+                    sc.leafTest = bs.leafTest;
                     sc.M = bs.M;
                     sc.recw = record_words(bs.M);
                     sc.ow = order_words(bs.M);

@@ -220,16 +220,20 @@ def test_cooperative_edge_cases(rt):
 def _special_rays(prim, seed):
     """Ordinary primary rays mixed with rays whose slab operands can produce
     NaNs: infinite inverse direction (denormal or -0 direction part), NaN
-    tMax, an origin coordinate beyond 2^100."""
+    tMax, an origin coordinate beyond 2^100, and an infinite direction part
+    (inverse 0), for which the all-zero padding triangles of a leaf have a
+    NaN divisor and pass the test."""
     rng = np.random.default_rng(seed)
     sp = prim[rng.choice(len(prim), 4000)].copy()
-    k = np.arange(len(sp)) % 5
+    k = np.arange(len(sp)) % 6
     sp["dx"][k == 0] = np.float32(1e-40)
     sp["ix"][k == 0] = np.float32(np.inf)
     sp["tmax"][k == 1] = np.float32(np.nan)
     sp["ox"][k == 2] = np.float32(3e30)
     sp["dy"][k == 3] = np.float32(-0.0)
     sp["iy"][k == 3] = np.float32(-np.inf)
+    sp["dz"][k == 5] = np.float32(np.inf)
+    sp["iz"][k == 5] = np.float32(0.0)
     rays = np.concatenate([prim, sp])
     return rays[rng.permutation(len(rays))]
 
