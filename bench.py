@@ -77,7 +77,7 @@ def bench_batched(rt, torch, args, log):
     ntris = clamp_lognormal(rng, np.log(2e5), 1.0, 1e4, 2e6, 64)
     nrays = clamp_lognormal(rng, np.log(6.4e4), 1.2, 4096, 1 << 20, 64)
     t0 = time.time()
-    scenes = []
+    scenes, host_rays = [], []
     for i in range(64):
         tris = rt.interior_scene(100 + i, int(ntris[i]))
         bvh = rt.Bvh(tris, args.branching, args.leaf)
@@ -94,6 +94,7 @@ def bench_batched(rt, torch, args, log):
         d_r = torch.from_numpy(rays.view(np.float32).reshape(-1, 10)).cuda()
         d_h = torch.empty((len(rays), 6), dtype=torch.int32, device="cuda")
         scenes.append((bvh, sc, d_r, d_h))
+        host_rays.append((tris, rays))
     total = sum(x[2].shape[0] for x in scenes)
     log(f"batched workload: 64 scenes, {int(ntris.sum())} tris, {total} rays, setup {time.time() - t0:.1f}s")
     stream = torch.cuda.current_stream()
@@ -128,9 +129,52 @@ def bench_batched(rt, torch, args, log):
            "plain_mrays": round(res["plain"], 2), "batched_mrays": round(res["batched"], 2),
            "speedup": round(res["batched"] / res["plain"], 3), "identical_to_plain": bool(same),
            "n_l": f"N{args.branching}L{args.leaf}", "graph": True}
+    if args.batch_autotune:
+        out["autotuned"] = batched_autotuned(rt, torch, host_rays, scenes, outs, stream, total, args, log)
     for bvh, sc, d_r, d_h in scenes:
         sc.close()
     return out
+
+
+def batched_autotuned(rt, torch, host_rays, scenes, ref_outs, stream, total, args, log):
+    """The same 64 scenes with N and L chosen per scene by the autotuner
+    (candidates N4L4, N4L8, N8L4, N8L8 timed on 16384 of the scene's own rays),
+    traced in one batched graph launch that mixes the chosen node/leaf
+    memories.  Tuning and the rebuilds are ingest cost, outside the timing."""
+    from bvhraytracer_amd.autotune import Autotuner
+
+    t0 = time.time()
+    tuner = Autotuner(nl=((4, 4), (4, 8), (8, 4), (8, 8)), blocks=(256,), windows=(16,), schedules=(16,),
+                      sample=16384, reps=2)
+    tuned, choice = [], {}
+    for (tris, rays), (bvh0, sc0, d_r, d_h) in zip(host_rays, scenes):
+        best = tuner.tune_scene(tris, rays)
+        key = f"N{best['N']}L{best['L']}"
+        choice[key] = choice.get(key, 0) + 1
+        bvh = rt.Bvh(tris, best["N"], best["L"])
+        sc = rt.Scene(bvh)
+        tuned.append((bvh, sc, d_r, torch.empty_like(d_h)))
+    tune_s = time.time() - t0
+    batch = rt.Batch(64)
+    items = [(sc, d_r, o, "closest") for (bvh, sc, d_r, o) in tuned]
+    for _ in range(max(1, args.warmup)):
+        batch.trace(items, graph=True, stream=stream)
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(max(3, args.steps // 2)):
+        t = time.perf_counter()
+        batch.trace(items, graph=True, stream=stream)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+    mrays = total / float(np.median(ts)) / 1e6
+    # the hit triangles are the same whatever the tree, up to exact ties
+    same_tri = sum(int((o[:, 0] == r[:, 0]).sum().item()) for (b, s, d, o), r in zip(tuned, ref_outs)) / total
+    log(f"batched autotuned {mrays:.1f} Mrays/s, choices {choice}, tuning {tune_s:.1f}s")
+    batch.close()
+    for bvh, sc, d_r, o in tuned:
+        sc.close()
+    return {"batched_mrays": round(mrays, 2), "choices": choice, "tune_seconds": round(tune_s, 1),
+            "same_hit_triangle_fraction": round(same_tri, 6)}
 
 
 def bench_frame(rt, torch, scene, bvh, n_bounce, log, reps=3):
@@ -281,6 +325,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-frame", action="store_true", help="skip the bounce-frame leg")
+    ap.add_argument("--batch-autotune", action="store_true",
+                    help="also run the batched leg with N and L autotuned per scene (adds ~1 min of ingest)")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (counter calibration)")
