@@ -633,7 +633,7 @@ struct rt_batch {
     hipGraphNode_t traceNode = nullptr;
     hipKernelNodeParams traceNodeParams{};
     hipStream_t graphStream = nullptr;
-    uint32_t graphGrid = 0, graphBd = 0, graphS = 0, graphSpillCap = 0;
+    uint32_t graphGrid = 0, graphBd = 0, graphS = 0, graphSpillCap = 0, graphM = 0;
     uint32_t* graphSpill = nullptr;
     float lastMs = 0;
     uint64_t raysTraced = 0;
@@ -670,11 +670,12 @@ void free_batch(rt_batch* b) {
     delete b;
 }
 
-int capture_graph(rt_batch* b, hipStream_t st, const KParams& p, uint32_t grid, uint32_t bd, uint32_t S) {
+int capture_graph(rt_batch* b, hipStream_t st, const KParams& p, uint32_t grid, uint32_t bd, uint32_t S,
+                  uint32_t M) {
     drop_graph(b);
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipError_t e1 = launch_reset(p.work, st);
-    hipError_t e2 = launch_trace(0, kMixed, S, bd, grid, p, st);
+    hipError_t e2 = launch_trace(M, kMixed, S, bd, grid, p, st);
     hipGraph_t g = nullptr;
     hipError_t e3 = hipStreamEndCapture(st, &g);
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
@@ -704,6 +705,7 @@ int capture_graph(rt_batch* b, hipStream_t st, const KParams& p, uint32_t grid, 
     b->graphS = S;
     b->graphSpill = b->spill;
     b->graphSpillCap = p.spillCap;
+    b->graphM = M;
     return RT_OK;
 }
 
@@ -815,7 +817,11 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     if (used == 0) return RT_OK;
     const uint32_t S = b->stackWindow, bd = b->blockSize;
     const bool graph = (flags & RT_GRAPH) != 0;
-    const int occ = std::max(1, trace_occupancy(0, kMixed, S, bd));
+    // one node memory for the whole batch: the kernel specialised on it
+    uint32_t batchM = b->hScenes[0].M;
+    for (uint32_t i = 1; i < nScenes; ++i)
+        if (b->hScenes[i].M != batchM) batchM = 0;
+    const int occ = std::max(1, trace_occupancy(batchM, kMixed, S, bd));
     const uint64_t full = (uint64_t)b->cus * (uint64_t)occ;
     const uint32_t grid = graph ? (uint32_t)full
                                 : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(full, (total + bd - 1) / bd));
@@ -839,8 +845,8 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     p.sched = kDefaultSched;
     if (graph) {
         if (!b->exec || b->graphStream != st || b->graphGrid != grid || b->graphBd != bd || b->graphS != S ||
-            b->graphSpill != b->spill || b->graphSpillCap != spillCap) {
-            if ((rc = capture_graph(b, st, p, grid, bd, S)) != RT_OK) return rc;
+            b->graphSpill != b->spill || b->graphSpillCap != spillCap || b->graphM != batchM) {
+            if ((rc = capture_graph(b, st, p, grid, bd, S, batchM)) != RT_OK) return rc;
         } else {
             // re-arm: same graph, new kernel arguments (ray count, item count)
             hipKernelNodeParams kp = b->traceNodeParams;
@@ -857,7 +863,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     } else {
         HIP_TRY(launch_reset(b->work, st), "reset kernel");
         HIP_TRY(hipEventRecord(b->ev0, st), "hipEventRecord");
-        HIP_TRY(launch_trace(0, kMixed, S, bd, grid, p, st), "batched trace kernel");
+        HIP_TRY(launch_trace(batchM, kMixed, S, bd, grid, p, st), "batched trace kernel");
         HIP_TRY(hipEventRecord(b->ev1, st), "hipEventRecord");
     }
     b->pending = true;

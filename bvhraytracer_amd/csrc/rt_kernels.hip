@@ -1181,12 +1181,25 @@ void* pick_coop(uint32_t W, int mode) {
 
 size_t stream_lds(uint32_t S) { return (size_t)(10 + 11 + S + 2) * kStreamSlots * 4 + 16; }
 
-void* pick_batch(uint32_t S) {
+template <int M>
+void* pick_batch_s(uint32_t S) {
     switch (S) {
-        case 8: return reinterpret_cast<void*>(&k_trace<0, kMixed, 8, true>);
-        case 16: return reinterpret_cast<void*>(&k_trace<0, kMixed, 16, true>);
-        case 32: return reinterpret_cast<void*>(&k_trace<0, kMixed, 32, true>);
+        case 8: return reinterpret_cast<void*>(&k_trace<M, kMixed, 8, true>);
+        case 16: return reinterpret_cast<void*>(&k_trace<M, kMixed, 16, true>);
+        case 32: return reinterpret_cast<void*>(&k_trace<M, kMixed, 32, true>);
         default: return nullptr;
+    }
+}
+
+// Batched kernel: specialised on the node memory when every scene of the
+// batch has the same M, generic (M read per scene) otherwise (M = 0).
+void* pick_batch(uint32_t S, uint32_t M) {
+    switch (M) {
+        case 4: return pick_batch_s<4>(S);
+        case 8: return pick_batch_s<8>(S);
+        case 12: return pick_batch_s<12>(S);
+        case 16: return pick_batch_s<16>(S);
+        default: return pick_batch_s<0>(S);
     }
 }
 
@@ -1211,7 +1224,7 @@ void* pick_mode(int mode, uint32_t S) {
 }
 
 void* pick(uint32_t M, int mode, uint32_t S) {
-    if (mode == kMixed) return pick_batch(S);
+    if (mode == kMixed) return pick_batch(S, M);
     switch (M) {
         case 4: return pick_mode<4>(mode, S);
         case 8: return pick_mode<8>(mode, S);

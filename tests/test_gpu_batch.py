@@ -90,3 +90,29 @@ def test_batch_equals_plain(rt, scenes):
         ref = torch.empty((len(rays), 6), dtype=torch.int32, device="cuda")
         sc.intersect_device(r, ref)
         assert torch.equal(o, ref), f"scene {i}"
+
+
+def test_batch_uniform_node_memory(rt, scenes):
+    """Batches whose scenes share one node memory run the kernel specialised
+    on it; a graph captured for one node memory is re-captured when the next
+    batch mixes them.  Every result equals the plain launch's."""
+    import torch
+    batch = rt.Batch(16)
+    same_m = [(1, "closest"), (4, "any"), (4, "closest"), (1, "any")]  # N8L8 and N8L4: M = 8
+    mixed = [(0, "closest"), (1, "closest"), (2, "any"), (3, "closest")]
+    for rep, order in enumerate((same_m, mixed, same_m)):
+        print(f"uniform-M batch call {rep}: {len(order)} items", flush=True)
+        outs = run_batch(rt, scenes, order, True, batch)
+        for i, kind, o, c in outs:
+            bvh, sc, rays, shadow = scenes[i]
+            rr = rays if kind == "closest" else shadow
+            mode = kind
+            ref, rc = sc.intersect(rr, mode=mode, counts=True)
+            got = o.cpu().numpy()
+            if kind == "closest":
+                assert np.array_equal(np.ascontiguousarray(got).view(rt.HIT_DTYPE).reshape(-1), ref), f"scene {i}"
+            else:
+                assert np.array_equal(got, ref), f"scene {i}"
+            cc = c.cpu().numpy().view(rt.COUNTS_DTYPE).reshape(-1)
+            assert np.array_equal(cc["nodes"], rc["nodes"]) and np.array_equal(cc["leaves"], rc["leaves"])
+    batch.close()
