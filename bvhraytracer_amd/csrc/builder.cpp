@@ -65,6 +65,45 @@ inline float sah_leaf(size_t count, float area, float invArea, int leafTarget) {
     return (float)steps * area * invArea * (float)leafTarget;
 }
 
+// Stable sort of [first, last) on up to `threads` threads: chunks are
+// stable-sorted in parallel, then adjacent runs are merged pairwise with
+// std::merge (which takes equal keys from the left run first).  A stable sort
+// has exactly one result for a given order and comparator, so this equals
+// std::stable_sort bit for bit.
+template <class It, class Less>
+void parallel_stable_sort(It first, It last, Less less, int threads) {
+    using T = typename std::iterator_traits<It>::value_type;
+    const size_t n = (size_t)(last - first);
+    const size_t parts = std::min<size_t>((size_t)std::max(1, threads), 16);
+    if (parts < 2 || n < (1u << 17)) {
+        std::stable_sort(first, last, less);
+        return;
+    }
+    std::vector<size_t> cut(parts + 1);
+    for (size_t i = 0; i <= parts; ++i) cut[i] = n * i / parts;
+    {
+        std::vector<std::thread> ts;
+        for (size_t i = 0; i < parts; ++i)
+            ts.emplace_back([&, i] { std::stable_sort(first + cut[i], first + cut[i + 1], less); });
+        for (auto& t : ts) t.join();
+    }
+    std::vector<T> tmp(n);
+    T* src = &*first;
+    T* dst = tmp.data();
+    bool inTmp = false;
+    for (size_t width = 1; width < parts; width *= 2) {
+        std::vector<std::thread> ts;
+        for (size_t i = 0; i < parts; i += 2 * width) {
+            const size_t a = cut[i], m = cut[std::min(i + width, parts)], b = cut[std::min(i + 2 * width, parts)];
+            ts.emplace_back([=] { std::merge(src + a, src + m, src + m, src + b, dst + a, less); });
+        }
+        for (auto& t : ts) t.join();
+        std::swap(src, dst);
+        inTmp = !inTmp;
+    }
+    if (inTmp) std::copy(tmp.begin(), tmp.end(), first);
+}
+
 struct BNode {
     Box box;
     uint32_t begin = 0, end = 0;  // primitive range (empty for inner nodes)
@@ -81,10 +120,11 @@ struct Builder {
     int leafSplitOption;
     float sahFactor;
     std::atomic<int> spareThreads;
+    int threads;
 
     Builder(std::vector<Prim>& p, unsigned n, unsigned l, bool s, int o, float f, int threads)
         : prims(p), N(n), L(l), sortEachSplit(s), leafSplitOption(o), sahFactor(f),
-          spareThreads(threads - 1) {}
+          spareThreads(threads - 1), threads(threads) {}
 
     // chooseAxisAndSort (aabb.h:40-69)
     int choose_axis_and_sort(uint32_t b, uint32_t e) {
@@ -97,8 +137,12 @@ struct Builder {
             }
         int axis = max_dimension(mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]);
         auto less = [axis](const Prim& p1, const Prim& p2) { return p1.c[axis] < p2.c[axis]; };
-        if (!std::is_sorted(prims.begin() + b, prims.begin() + e, less))
-            std::stable_sort(prims.begin() + b, prims.begin() + e, less);
+        if (!std::is_sorted(prims.begin() + b, prims.begin() + e, less)) {
+            // large ranges are only sorted near the root, before the
+            // subtrees take the threads
+            if (e - b >= (1u << 17)) parallel_stable_sort(prims.begin() + b, prims.begin() + e, less, threads);
+            else std::stable_sort(prims.begin() + b, prims.begin() + e, less);
+        }
         return axis;
     }
 
