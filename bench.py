@@ -31,7 +31,7 @@ HBM_PEAK_GBPS = 8000.0
 # in KiB.  Keyed by (branching, leaf size, rays); the commands and the
 # per-dispatch values are in profiles/k_trace_N8L8_diffuse8M_hbm_traffic.md.
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
-MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 8868914, "write": 392396}}
+MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 8383360, "write": 397396}}
 METRIC = "Mrays/s per MI355X (closest-hit, N8L8) batched vs plain; p99 request latency"
 
 
