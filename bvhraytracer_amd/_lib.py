@@ -112,6 +112,11 @@ class RtServerStats(ctypes.Structure):
                 ("mean_requests_per_launch", ctypes.c_double)]
 
 
+# rt_step_stat
+STEP_DTYPE = np.dtype([("node_work", "<u4"), ("node_unique", "<u4"), ("leaf_work", "<u4"),
+                       ("leaf_unique", "<u4"), ("terminations", "<u4")])
+
+
 # (name, restype, argtypes) of every exported entry point in include/rt.h
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -120,6 +125,7 @@ SIGNATURES = [
     ("rt_trace", ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P]),
     ("rt_stats", ctypes.c_int, [_P, ctypes.POINTER(RtStats)]),
     ("rt_trace_histograms", ctypes.c_int, [_P, _P, _U32, _U32, _P, ctypes.POINTER(RtTraceHist), _P]),
+    ("rt_trace_steps", ctypes.c_int, [_P, _P, _U32, _U32, _U32, _U32, _P, _P, _P, _P]),
     ("rt_set_launch", ctypes.c_int, [_P, _U32, _U32]),
     ("rt_set_schedule", ctypes.c_int, [_P, _U32]),
     ("rt_release", ctypes.c_int, [_P]),

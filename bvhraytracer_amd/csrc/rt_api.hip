@@ -441,6 +441,103 @@ extern "C" int rt_trace_histograms(rt_scene_t s, const rt_ray* rays, uint32_t n,
     return trace_impl(s, rays, n, flags, out, nullptr, hip_stream, hist);
 }
 
+extern "C" int rt_trace_steps(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, uint32_t tile,
+                              uint32_t step_cap, rt_step_stat* out, uint32_t* steps, rt_counts* counts,
+                              void* hip_stream) {
+    if (!s) return set_error(RT_ERR_ARG, "rt_trace_steps: null scene");
+    if (flags & ~(RT_ANY | RT_DEVICE_PTRS))
+        return set_error(RT_ERR_ARG, "rt_trace_steps: only RT_CLOSEST/RT_ANY and RT_DEVICE_PTRS");
+    if (tile == 0 || tile > 1024) return set_error(RT_ERR_ARG, "rt_trace_steps: tile must be 1..1024 rays");
+    if (n % tile != 0) return set_error(RT_ERR_ARG, "rt_trace_steps: n must be a multiple of tile");
+    if (step_cap == 0) return set_error(RT_ERR_ARG, "rt_trace_steps: step_cap must be at least 1");
+    if (n == 0) return RT_OK;
+    if (!rays || !out || !steps) return set_error(RT_ERR_ARG, "rt_trace_steps: null rays, records or lengths");
+    if (n >= 0x80000000u) return set_error(RT_ERR_ARG, "rt_trace_steps: at most 2^31-1 rays per call");
+    const uint32_t tiles = n / tile;
+    if ((uint64_t)tiles * step_cap * sizeof(rt_step_stat) > (1ull << 34))
+        return set_error(RT_ERR_ARG, "rt_trace_steps: tiles x step_cap records exceed 16 GiB");
+    const int mode = (flags & RT_ANY) ? kAny : kClosest;
+    std::lock_guard<std::mutex> lk(s->mu);
+    int prevDev = 0;
+    (void)hipGetDevice(&prevDev);
+    if (prevDev != s->device) HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    struct Restore {
+        int dev, prev;
+        ~Restore() {
+            if (dev != prev) (void)hipSetDevice(prev);
+        }
+    } restore{s->device, prevDev};
+    hipStream_t st = reinterpret_cast<hipStream_t>(hip_stream);
+    if (!st) {
+        if (!s->ownStream) HIP_TRY(hipStreamCreateWithFlags(&s->ownStream, hipStreamNonBlocking), "hipStreamCreate");
+        st = s->ownStream;
+    }
+    Workspace* w = nullptr;
+    int rc = get_workspace(s, st, w);
+    if (rc != RT_OK) return rc;
+    const int occ = std::max(1, trace_steps_occupancy(s->M, mode, tile));
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)s->cus * occ, tiles));
+    const uint32_t cap = stack_capacity(s);
+    const uint32_t spillCap = cap > kStepWindow ? cap - kStepWindow : 0;
+    if ((rc = ensure(reinterpret_cast<void**>(&w->spill), w->spillBytes,
+                     std::max<size_t>(64, (size_t)grid * tile * spillCap * 4), "hipMalloc(spill)")) != RT_OK)
+        return rc;
+    const size_t rayBytes = (size_t)n * sizeof(rt_ray);
+    const size_t recBytes = (size_t)tiles * step_cap * sizeof(rt_step_stat);
+    const size_t lenBytes = (size_t)tiles * 4;
+    const size_t cntBytes = counts ? (size_t)n * sizeof(rt_counts) : 0;
+    const rt_ray* dRays = rays;
+    rt_step_stat* dOut = out;
+    uint32_t* dLen = steps;
+    rt_counts* dCounts = counts;
+    const bool dev = (flags & RT_DEVICE_PTRS) != 0;
+    if (!dev) {
+        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t outOff = up(rayBytes), lenOff = outOff + up(recBytes), cntOff = lenOff + up(lenBytes);
+        if ((rc = ensure(&w->stage, w->stageBytes, cntOff + cntBytes + 256, "hipMalloc(stage)")) != RT_OK) return rc;
+        char* base = static_cast<char*>(w->stage);
+        dRays = reinterpret_cast<const rt_ray*>(base);
+        dOut = reinterpret_cast<rt_step_stat*>(base + outOff);
+        dLen = reinterpret_cast<uint32_t*>(base + lenOff);
+        dCounts = counts ? reinterpret_cast<rt_counts*>(base + cntOff) : nullptr;
+        HIP_TRY(hipMemcpyAsync(const_cast<rt_ray*>(dRays), rays, rayBytes, hipMemcpyHostToDevice, st), "hipMemcpy(rays)");
+    }
+    KParams p{};
+    p.nodes = s->nodes;
+    p.tris = s->tris;
+    p.triIds = s->triIds;
+    p.nodeEntry = s->nodeEntry;
+    p.rays = dRays;
+    p.counts = dCounts;
+    p.spill = w->spill;
+    p.work = w->work;
+    p.n = n;
+    p.rootEntry = s->rootEntry;
+    p.spillCap = spillCap;
+    p.gang = s->G;
+    p.Lm = s->Lm;
+    p.nodeCount = s->nodeCount;
+    p.M = s->M;
+    p.boxesBounded = s->boxesBounded ? 1u : 0u;
+    p.leafTest = s->leafTest;
+    p.steps = dOut;
+    p.stepLen = dLen;
+    p.stepCap = step_cap;
+    HIP_TRY(launch_reset(w->work, st), "reset kernel");
+    HIP_TRY(launch_trace_steps(s->M, mode, tile, grid, p, st), "step statistics kernel launch");
+    if (!dev) {
+        HIP_TRY(hipMemcpyAsync(out, dOut, recBytes, hipMemcpyDeviceToHost, st), "hipMemcpy(records)");
+        HIP_TRY(hipMemcpyAsync(steps, dLen, lenBytes, hipMemcpyDeviceToHost, st), "hipMemcpy(lengths)");
+        if (cntBytes) HIP_TRY(hipMemcpyAsync(counts, dCounts, cntBytes, hipMemcpyDeviceToHost, st), "hipMemcpy(counts)");
+    }
+    HIP_TRY(hipStreamSynchronize(st), "step statistics kernel");
+    uint32_t errs = 0;
+    HIP_TRY(hipMemcpy(&errs, w->work + kErrWord, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
+    s->errorRays += errs;
+    s->raysTraced += n;
+    return RT_OK;
+}
+
 namespace {
 int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, void* out, rt_counts* counts,
                void* hip_stream, rt_trace_hist* hist) {

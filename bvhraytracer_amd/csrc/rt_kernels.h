@@ -79,6 +79,11 @@ struct KParams {
     // cooperative launch only: node memory and LDS stack entries per ray
     uint32_t M;
     uint32_t coopDepth;
+    // per-step tile statistics launch only: stepCap records per tile, and
+    // the record count of each tile
+    rt_step_stat* steps;
+    uint32_t* stepLen;
+    uint32_t stepCap;
     // batched launch only
     const BatchItem* items;
     const BatchScene* scenes;
@@ -119,5 +124,13 @@ constexpr uint32_t kHistWindow = 16;
 hipError_t launch_trace_hist(uint32_t M, int mode, uint32_t grid, const KParams& p, hipStream_t stream);
 uint32_t trace_hist_block(uint32_t M, int mode);
 int trace_hist_occupancy(uint32_t M, int mode);
+
+// Per-step tile statistics kernel (closest / any): blocks of `tile` threads,
+// one tile per block iteration, an 8-entry LDS stack window; p.steps,
+// p.stepLen and p.stepCap set, p.n a multiple of tile.
+constexpr uint32_t kStepWindow = 8;
+hipError_t launch_trace_steps(uint32_t M, int mode, uint32_t tile, uint32_t grid, const KParams& p,
+                              hipStream_t stream);
+int trace_steps_occupancy(uint32_t M, int mode, uint32_t tile);
 
 }  // namespace rt

@@ -751,6 +751,102 @@ __global__ void __launch_bounds__(256) k_trace_hist(KParams p) {
     trace_body<MT, MODE, kHistWindow, false, true>(p);
 }
 
+// Per-step workload of ray tiles (rt_trace_steps; the instrumented wide
+// traversal of compactNode.cpp:358-601).  One block per tile, one ray per
+// thread, blocks loop over tiles.  Each step every unfinished ray takes
+// exactly one visit; the block then counts node and leaf visits, distinct
+// entries (a visit is distinct if no lower thread visits the same entry in
+// this step: a broadcast LDS scan) and finished rays, and thread 0 writes
+// the step record.  Every loop bound is block-uniform: the tile loop, and
+// the step loop, which ends when no ray of the tile is unfinished.
+template <int MT, int MODE>
+__global__ void __launch_bounds__(1024) k_trace_steps(KParams p) {
+    extern __shared__ uint32_t smem[];
+    constexpr uint32_t S = kStepWindow;
+    const uint32_t tid = threadIdx.x, bd = blockDim.x;
+    uint32_t* keys = smem + S * bd;
+    uint32_t* ctr = keys + bd;  // node work, node unique, leaf work, leaf unique, terminations
+    StackCtx sk;
+    sk.stE = smem + tid;
+    sk.stD = nullptr;
+    sk.aabbD = nullptr;
+    sk.spE = p.spill + ((size_t)blockIdx.x * bd + tid) * p.spillCap;
+    sk.spD = nullptr;
+    sk.cap = S + p.spillCap;
+    sk.stride = bd;
+    sk.stDep = nullptr;
+    sk.spDep = nullptr;
+    sk.hist = nullptr;
+    const SceneCtx sc = plain_scene(p, MT);
+    const uint32_t tiles = p.n / bd;
+    const bool lane0 = __lane_id() == 0;
+    if (tid < 5) ctr[tid] = 0;
+    __syncthreads();
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t r = tile * bd + tid;
+        const rt_ray in = p.rays[r];
+        Trav t;
+        trav_init(t, in, sc.rootEntry, MODE == kAny, sc.boxesBounded);
+        const bool start = !(MODE == kAny && __builtin_isnan(in.ox));
+        if (!start) t.hasCur = false;
+        uint32_t* rec = reinterpret_cast<uint32_t*>(p.steps) + (size_t)tile * p.stepCap * 5u;
+        {
+            const uint32_t idle = (uint32_t)__popcll(__ballot(!start));
+            if (lane0) atomicAdd(&ctr[4], idle);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            rec[0] = bd - ctr[4];
+            rec[1] = 1;
+            rec[2] = 0;
+            rec[3] = 0;
+            rec[4] = ctr[4];
+            ctr[4] = 0;
+        }
+        uint32_t step = 1;
+        while (__syncthreads_or(!trav_done(t))) {
+            const bool active = !trav_done(t);
+            const uint32_t key = active ? trav_peek<S>(t, sk) : 0xFFFFFFFFu;
+            keys[tid] = key;
+            if (active) trav_step<MT, MODE, S>(sc, sk, t);
+            const bool fin = active && trav_done(t);
+            const bool leaf = (key & kEntryLeaf) != 0;
+            __syncthreads();
+            bool first = active;
+            for (uint32_t j = 0; j < bd; ++j) first = first && !(j < tid && keys[j] == key);
+            const uint32_t nN = (uint32_t)__popcll(__ballot(active && !leaf));
+            const uint32_t uN = (uint32_t)__popcll(__ballot(first && !leaf));
+            const uint32_t nL = (uint32_t)__popcll(__ballot(active && leaf));
+            const uint32_t uL = (uint32_t)__popcll(__ballot(first && leaf));
+            const uint32_t nT = (uint32_t)__popcll(__ballot(fin));
+            if (lane0) {
+                atomicAdd(&ctr[0], nN);
+                atomicAdd(&ctr[1], uN);
+                atomicAdd(&ctr[2], nL);
+                atomicAdd(&ctr[3], uL);
+                atomicAdd(&ctr[4], nT);
+            }
+            __syncthreads();
+            if (tid == 0) {
+                if (step < p.stepCap)
+                    for (int f = 0; f < 5; ++f) rec[(size_t)step * 5u + f] = ctr[f];
+                for (int f = 0; f < 5; ++f) ctr[f] = 0;
+            }
+            ++step;
+        }
+        if (tid == 0) p.stepLen[tile] = step;
+        if (p.counts) {
+            rt_counts c;
+            c.nodes = t.cN;
+            c.leaves = t.cL;
+            c.node_slots = t.cNS;
+            c.leaf_slots = t.cLS;
+            p.counts[r] = c;
+        }
+        if (t.err) atomicAdd(p.work + kErrWord, 1u);
+    }
+}
+
 // Ray-stream kernel: the GPU form of intersectWideAlternative
 // (fastNodeManager.h:835-1049) with continuous refill (the refill prototype,
 // :277-450).  A block keeps kStreamSlots rays whose traversal state lives
@@ -1296,6 +1392,51 @@ int trace_hist_occupancy(uint32_t M, int mode) {
                                                      trace_hist_lds_bytes(M, mode, bd)) != hipSuccess)
         return 0;
     return blocks;
+}
+
+namespace {
+
+template <int M>
+void* pick_steps_mode(int mode) {
+    switch (mode) {
+        case kClosest: return reinterpret_cast<void*>(&k_trace_steps<M, kClosest>);
+        case kAny: return reinterpret_cast<void*>(&k_trace_steps<M, kAny>);
+        default: return nullptr;
+    }
+}
+
+void* pick_steps(uint32_t M, int mode) {
+    switch (M) {
+        case 4: return pick_steps_mode<4>(mode);
+        case 8: return pick_steps_mode<8>(mode);
+        case 12: return pick_steps_mode<12>(mode);
+        case 16: return pick_steps_mode<16>(mode);
+        default: return nullptr;
+    }
+}
+
+}  // namespace
+
+size_t trace_steps_lds_bytes(uint32_t tile) { return ((size_t)kStepWindow * tile + tile + 8) * 4; }
+
+int trace_steps_occupancy(uint32_t M, int mode, uint32_t tile) {
+    void* fn = pick_steps(M, mode);
+    if (!fn) return 0;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(fn), (int)tile,
+                                                     trace_steps_lds_bytes(tile)) != hipSuccess)
+        return 0;
+    return blocks;
+}
+
+hipError_t launch_trace_steps(uint32_t M, int mode, uint32_t tile, uint32_t grid, const KParams& p,
+                              hipStream_t stream) {
+    void* fn = pick_steps(M, mode);
+    if (!fn || !p.steps || !p.stepLen || p.stepCap == 0 || tile == 0 || tile > 1024 || p.n % tile != 0)
+        return hipErrorInvalidValue;
+    KParams copy = p;
+    void* args[] = {&copy};
+    return hipLaunchKernel(fn, dim3(grid), dim3(tile), args, trace_steps_lds_bytes(tile), stream);
 }
 
 hipError_t launch_trace_hist(uint32_t M, int mode, uint32_t grid, const KParams& p, hipStream_t stream) {

@@ -228,6 +228,30 @@ class Scene:
             stats[f] = np.array(v[:], np.int64) if hasattr(v, "_length_") else int(v)
         return out, stats
 
+    def workgroup_steps(self, rays, tile, mode="closest", step_cap=256):
+        """rt_trace_steps on host rays: the per-step workload of tiles of
+        `tile` consecutive rays.  Returns (tiles, counts): tiles[i] is a
+        STEP_DTYPE array of tile i's records (record 0 = before the first
+        step), counts the visits of every ray (COUNTS_DTYPE).  Calls again
+        with a larger cap when a tile takes more than step_cap records."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        n = len(rays)
+        if mode not in ("closest", "any"):
+            raise ValueError("workgroup_steps: mode must be 'closest' or 'any'")
+        nt = n // tile if tile else 0
+        cnt = np.zeros(n, COUNTS_DTYPE)
+        while True:
+            out = np.zeros(max(1, nt * step_cap), _lib.STEP_DTYPE)
+            lens = np.zeros(max(1, nt), np.uint32)
+            check(lib().rt_trace_steps(self._h, ptr(rays), n, MODES[mode], int(tile), int(step_cap), ptr(out),
+                                       ptr(lens), ptr(cnt), None))
+            need = int(lens[:nt].max()) if nt else 0
+            if need <= step_cap:
+                break
+            step_cap = need
+        out = out[:nt * step_cap].reshape(nt, step_cap) if nt else out[:0]
+        return [out[i, :lens[i]].copy() for i in range(nt)], cnt
+
     def surface_device(self, rays, hits, normal=None, position=None, stream=None):
         """rt_hit_surface_device: hit reconstruction from the resident
         triangles.  rays (n,10) float32, hits (n,6) int32 CUDA tensors;

@@ -4,6 +4,9 @@
   RayTracer.run      the N/L sweep            (rayTracer.cpp:97-122 clamps, :568-576 loop)
   RayTracer.render   RayTracer::renderImage   (rayTracer.cpp:588-745, performance branch)
   CameraFast         CameraFast::renderImages / renderImage (src/cameraFast.h:49-582)
+  write_png          the _Perf.png of cameraFast.h:829-852 (tile order -> rows, RGBA PNG)
+  workgroup_analysis the wide render's per-step tile statistics and the four
+                     WorkGroup*.txt files of cameraData.h:458-560,660-1109
 
 Scenes are procedural (the reference loads glTF models that are not part of
 the repository); everything else keeps the reference's semantics: substring
@@ -12,7 +15,9 @@ and the `_Perf.txt` fields.  Traversal runs on the GPU through librt.so.
 """
 import math
 import os
+import struct
 import time
+import zlib
 
 import numpy as np
 
@@ -103,8 +108,9 @@ def _sphere_dirs(index, sample, normal):
     """Deterministic uniform directions flipped into the normal's hemisphere
     (the sampler shape of Camera::sampleHemisphere, camera.cpp:36-52, fed by
     a PCG-style hash instead of the platform std::hash)."""
-    h = (index.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)
-         + np.uint64(sample) * np.uint64(0xBF58476D1CE4E5B9) + np.uint64(1))
+    with np.errstate(over="ignore"):  # uint64 arithmetic wraps by design
+        s = np.uint64(sample) * np.uint64(0xBF58476D1CE4E5B9)
+    h = index.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15) + s + np.uint64(1)
     h ^= h >> np.uint64(31)
     h *= np.uint64(0x94D049BB133111EB)
     h ^= h >> np.uint64(29)
@@ -116,6 +122,61 @@ def _sphere_dirs(index, sample, normal):
     flip = (d * normal).sum(1) <= 0
     d[flip] = -d[flip]
     return d.astype(np.float32)
+
+
+def tile_order_to_rows(values, width, height, wg):
+    """Per-pixel values in CameraFast tile order (the order of camera_rays:
+    wg x wg tiles left to right, tile rows top to bottom, lanes row-major in a
+    tile) to a (height, width, ...) row-major image: the reordering before
+    cameraFast.h:833-849 saves the image, extended to a partial last tile
+    row (whose tiles hold height % wg rows)."""
+    values = np.asarray(values)
+    if width % wg:
+        raise ValueError("width must be a multiple of the workgroup size")
+    if len(values) != width * height:
+        raise ValueError(f"expected {width * height} pixels, got {len(values)}")
+    tiles_x, full_rows = width // wg, height // wg
+    h, w = np.meshgrid(np.arange(height, dtype=np.int64), np.arange(width, dtype=np.int64), indexing="ij")
+    lane = (h % wg) * wg + w % wg
+    k = ((h // wg) * tiles_x + w // wg) * wg * wg + lane
+    part = h >= full_rows * wg
+    if part.any():
+        rows_left = height - full_rows * wg
+        k[part] = (full_rows * tiles_x * wg * wg + (w[part] // wg) * rows_left * wg
+                   + (h[part] % wg) * wg + w[part] % wg)
+    return values[k]
+
+
+def _png_chunk(tag, data):
+    return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+
+
+def write_png(path, rgba):
+    """An 8-bit RGBA PNG of a (height, width, 4) uint8 image (lodepng::encode
+    of the reference's encodeTwoSteps, util.h:52-61, takes the same RGBA
+    buffer; the pixels decode identically, the compressed bytes differ)."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
+    if rgba.ndim != 3 or rgba.shape[2] != 4:
+        raise ValueError("expected a (height, width, 4) image")
+    height, width = rgba.shape[:2]
+    raw = b"".join(b"\x00" + rgba[r].tobytes() for r in range(height))  # filter type 0 per row
+    png = (b"\x89PNG\r\n\x1a\n"
+           + _png_chunk(b"IHDR", struct.pack(">IIBBBBB", width, height, 8, 6, 0, 0, 0))
+           + _png_chunk(b"IDAT", zlib.compress(raw, 6))
+           + _png_chunk(b"IEND", b""))
+    with open(path, "wb") as f:
+        f.write(png)
+
+
+def ambient_image(occluded, hit, samples):
+    """Gray value per pixel (cameraFast.h:404-439): a hit pixel is
+    (uint8)((1 - occluded / samples) * 255) in float32, a miss is 0.  With
+    no samples the reference converts NaN, which gives 0 on x86."""
+    img = np.zeros(len(hit), np.uint8)
+    if samples:
+        factor = np.float32(1) - occluded.astype(np.float32) / np.float32(samples)
+        img[hit] = (factor[hit] * np.float32(255)).astype(np.uint8)
+    return img
 
 
 class CameraFast:
@@ -143,15 +204,34 @@ class CameraFast:
             origin = pos[idx] + nrm[idx] * np.float32(0.001)
             for a in range(ambient_samples):
                 d = _sphere_dirs(idx, a, nrm[idx])
-                amb = make_rays(origin, origin + d, tmax=ambient_distance)
+                # FastRay::updateDirection (fastRay.cpp:22-38): the direction
+                # itself, not (origin + d) - origin
+                amb = make_rays(np.zeros_like(d), d, tmax=ambient_distance)
+                amb["ox"], amb["oy"], amb["oz"] = origin[:, 0], origin[:, 1], origin[:, 2]
                 occl[idx] += scene.intersect(amb, mode="any")
         total = time.perf_counter() - t0
-        image = ((1 - occl / max(1, ambient_samples)) * 255).astype(np.uint8) if ambient_samples else None
-        return total, t_primary, hits, image
+        return total, t_primary, hits, ambient_image(occl, hit, ambient_samples)
+
+    def save_image(self, path, name, cam, image):
+        """<path>/<name>_c<cam>_Perf.png (cameraFast.h:829-852), gray RGBA."""
+        g = tile_order_to_rows(image, self.width, self.height, self.wg)
+        rgba = np.empty(g.shape + (4,), np.uint8)
+        rgba[..., :3] = g[..., None]
+        rgba[..., 3] = 255
+        out = os.path.join(path, f"{name}_c{cam}_Perf.png")
+        write_png(out, rgba)
+        return out
 
     def render_images(self, scene, bvh, ambient_samples=1, ambient_distance=1.0, samples=5,
-                      mode="closest"):
+                      mode="closest", save_image=False, path=".", name="image"):
+        """With save_image, one untimed render per camera first writes its
+        _Perf.png (cameraFast.h:81-89)."""
         totals, primaries = [], []
+        if save_image:
+            os.makedirs(path, exist_ok=True)
+            for cam in range(len(self.positions)):
+                img = self.render_image(scene, bvh, cam, ambient_samples, ambient_distance, mode)[3]
+                self.save_image(path, name, cam, img)
         for cam in range(len(self.positions)):
             self.render_image(scene, bvh, cam, ambient_samples, ambient_distance, mode)  # warm-up
             res = sorted(self.render_image(scene, bvh, cam, ambient_samples, ambient_distance, mode)[:2]
@@ -227,6 +307,179 @@ def write_info(path, name, branching, leaf_size, primary, secondary):
                 f.write(f"{i} : {v / max(1, st['rays'])}\n")
 
 
+def _g(x):
+    """std::ostream << float / double with the default format (%g, 6
+    significant digits)."""
+    return "%g" % float(x)
+
+
+def _whisker_rows(tiles, wgsq):
+    """Per-tile (median, min, max, avg, sd) of the step at which each ray
+    finished (cameraData.h:703-779): the depth list expands the
+    terminations per step; avg and the variance are float32 as there (each
+    square comes from std::pow in double and is added to the float sum, one
+    rounding per ray, in ray order; vectorised over tiles)."""
+    med_id = int(np.float32(wgsq) * np.float32(0.5))
+    depth = np.zeros((len(tiles), wgsq), np.int64)
+    work = np.zeros(len(tiles), np.int64)
+    for i, rec in enumerate(tiles):
+        term = rec["terminations"].astype(np.int64)
+        dl = np.repeat(np.arange(len(term), dtype=np.int64), term)
+        if len(dl) < wgsq:
+            raise ValueError(f"tile {i}: {len(dl)} terminations, expected at least {wgsq}")
+        depth[i] = dl[:wgsq]
+        work[i] = int((np.arange(len(term)) * term).sum())
+    avg = (work.astype(np.float32) / np.float32(wgsq)).astype(np.float32)
+    var = np.zeros(len(tiles), np.float32)
+    for j in range(wgsq):
+        d = (depth[:, j].astype(np.float32) - avg).astype(np.float32).astype(np.float64)
+        var = (var.astype(np.float64) + d * d).astype(np.float32)
+    var = (var / np.float32(wgsq)).astype(np.float32)
+    sd = np.sqrt(var).astype(np.float32)
+    rows = [(int(depth[i, med_id]), int(depth[i, 0]), int(depth[i, wgsq - 1]), avg[i], sd[i])
+            for i in range(len(tiles))]
+    # std::sort by (median, max); the reference leaves ties unordered, here
+    # they keep the tile order
+    return sorted(rows, key=lambda r: (r[0], r[2]))
+
+
+def _step_averages(tiles, j, n_tiles_f, wgsq):
+    """One row half of _WorkGroupData.txt for step j (cameraData.h:846-949):
+    averages over the tiles still working at step j, unique counts over the
+    tiles with a nonzero unique count, terminations over all tiles."""
+    live = [t for t in tiles if len(t) > j]
+    acc = {f: np.float32(0) for f in ("node_work", "leaf_work", "node_unique", "leaf_unique", "terminations")}
+    for t in live:
+        for f in acc:
+            acc[f] = np.float32(acc[f] + np.float32(int(t[f][j])))
+    un = [int(t["node_unique"][j]) for t in live if t["node_unique"][j] != 0]
+    ul = [int(t["leaf_unique"][j]) for t in live if t["leaf_unique"][j] != 0]
+    cnt = np.float32(max(len(live), 1))
+    return (np.float32(acc["node_work"] / cnt), np.float32(acc["node_unique"] / np.float32(max(len(un), 1))),
+            np.float32(acc["leaf_work"] / cnt), np.float32(acc["leaf_unique"] / np.float32(max(len(ul), 1))),
+            np.float32(acc["terminations"] / n_tiles_f),
+            max(un) if un else 0, min(un) if un else 0, max(ul) if ul else 0, min(ul) if ul else 0)
+
+
+def write_workgroup_analysis(path, name, problem, camera_id, width, height, wg, primary, secondary,
+                             primary_visits, secondary_visits, wide_alternative=True):
+    """The workgroup analysis files of CameraData::workGroupAnalysisPerImage
+    (cameraData.h:660-1109) under path/WorkGroupSize_<wg>_Version_<0|1>/:
+    _PrimaryWorkGroupWiskerPlot.txt, _SecondaryWorkGroupWiskerPlot.txt,
+    _WorkGroupData.txt and _WorkGroupUniqueWork.txt.
+
+    primary / secondary: per full tile, the STEP_DTYPE records of
+    Scene.workgroup_steps (secondary: the records of every ambient sample
+    concatenated, as the reference appends them).  primary_visits /
+    secondary_visits: (nodes, leaves) per ray in tile order (secondary summed
+    over the samples).  Returns the four paths."""
+    wgsq = wg * wg
+    n_tiles = (width // wg) * (height // wg)
+    if len(primary) != n_tiles or len(secondary) != n_tiles:
+        raise ValueError(f"expected {n_tiles} tiles")
+    d = os.path.join(path, f"WorkGroupSize_{wg}_Version_{int(bool(wide_alternative))}")
+    os.makedirs(d, exist_ok=True)
+    stem = os.path.join(d, f"{name}{problem}_c{camera_id}")
+    out = [stem + "_PrimaryWorkGroupWiskerPlot.txt", stem + "_SecondaryWorkGroupWiskerPlot.txt",
+           stem + "_WorkGroupData.txt", stem + "_WorkGroupUniqueWork.txt"]
+    for fn, tiles in ((out[0], primary), (out[1], secondary)):
+        with open(fn, "w") as f:
+            f.write("median, min, max, lowerStdDeviation, upperStdDeviation\n")
+            for med, mn, mx, avg, sd in _whisker_rows(tiles, wgsq):
+                f.write(f"{med}, {mn}, {mx}, {_g(avg)}, {_g(sd)}\n")
+    # the reference's size loop tests the primary size twice, so the steps
+    # are those of the longest primary tile
+    max_size = max(len(t) for t in primary)
+    n_tiles_f = np.float32(np.float32(width // wg) * np.float32(height // wg))
+    with open(out[2], "w") as f:
+        f.write("stepId, avgPrimaryNodeWork, avgPrimaryNodeUnique, avgPrimaryLeafWork, avgPrimaryLeafUnique, "
+                "avgPrimaryRayTermination, primaryNodeWorkMax, primaryNodeWorkMin, primaryLeafWorkMax, "
+                "primaryLeafWorkMin, avgSecondaryNodeWork, avgSecondaryNodeUnique, avgSecondaryLeafWork, "
+                "avgSecondaryLeafUnique, avgSecondaryRayTermination, secondaryNodeWorkMax, secondaryNodeWorkMin, "
+                "secondaryLeafWorkMax, secondaryLeafWorkMin\n")
+        cum_p = cum_s = np.float32(0)
+        for j in range(max_size):
+            p = list(_step_averages(primary, j, n_tiles_f, wgsq))
+            s = list(_step_averages(secondary, j, n_tiles_f, wgsq))
+            if j:
+                p[4] = np.float32(p[4] + cum_p)
+                s[4] = np.float32(s[4] + cum_s)
+            cum_p, cum_s = p[4], s[4]
+            cells = [str(j)] + [_g(v) for v in p[:5]] + [str(v) for v in p[5:]] + \
+                [_g(v) for v in s[:5]] + [str(v) for v in s[5:]]
+            f.write(", ".join(cells) + "\n")
+    pn, pl = (np.asarray(a, np.int64) for a in primary_visits)
+    sn, sl = (np.asarray(a, np.int64) for a in secondary_visits)
+    with open(out[3], "w") as f:
+        f.write("loadedPrimaryNodes, loadedPrimaryLeafs, loadedPrimaryNodesMax, loadedPrimaryLeafsMax, "
+                "loadedPrimaryNodesMin, loadedPrimaryLeafsMin, loadedSecondaryNodes, loadedSecondaryLeafs, "
+                "loadedSecondaryNodesMax, loadedSecondaryLeafsMax, loadedSecondaryNodesMin, "
+                "loadedSecondaryLeafsMin, loadedWidePrimaryNodes, loadedWidePrimaryLeafs, "
+                "loadedWideSecondaryNodes, loadedWideSecondaryLeafs\n")
+
+        def mm(v):
+            # max from 0, min from 1000 (reset to 0 when it stays 1000)
+            mx, mn = max(0, int(v.max())), min(1000, int(v.min()))
+            return mx, (0 if mn == 1000 else mn)
+
+        for i in range(n_tiles):
+            sl_ = slice(i * wgsq, (i + 1) * wgsq)
+            cells = []
+            for nodes, leaves in ((pn[sl_], pl[sl_]), (sn[sl_], sl[sl_])):
+                nmx, nmn = mm(nodes)
+                lmx, lmn = mm(leaves)
+                cells += [_g(np.float32(nodes.sum())), _g(np.float32(leaves.sum())), str(nmx), str(lmx),
+                          str(nmn), str(lmn)]
+            cells += [_g(np.float32(int(primary[i]["node_unique"].sum()))),
+                      _g(np.float32(int(primary[i]["leaf_unique"].sum()))),
+                      _g(np.float32(int(secondary[i]["node_unique"].sum()))),
+                      _g(np.float32(int(secondary[i]["leaf_unique"].sum())))]
+            f.write(", ".join(cells) + "\n")
+    return out
+
+
+def workgroup_analysis(scene, bvh, pos, target, width, height, wg, ambient_samples, ambient_distance, path,
+                       name, problem, camera_id=0, wide_alternative=True):
+    """The wide render of CameraData::renderImage with doWorkGroupAnalysis
+    (cameraData.h:458-560): for every full wg x wg tile, the per-step
+    workload of its primary rays (closest hit) and of its ambient rays (any
+    hit; a pixel without a primary hit gets a NaN ray that is not started),
+    one batch per ambient sample; then the four analysis files.  Returns
+    (primary, secondary) per-tile records and the file paths."""
+    if ambient_samples < 1:
+        raise ValueError("the workgroup analysis needs at least one ambient sample")
+    wgsq = wg * wg
+    n_tiles = (width // wg) * (height // wg)
+    rays = camera_rays(pos, target, width, height, wg)[:n_tiles * wgsq]
+    primary, pc = scene.workgroup_steps(rays, wgsq, mode="closest")
+    hits = scene.intersect(rays)
+    hit = hits["slot"] >= 0
+    nrm, spos = bvh.surface(rays, hits)
+    idx = np.nonzero(hit)[0]
+    origin = spos[idx] + nrm[idx] * np.float32(0.001)
+    secondary = [[] for _ in range(n_tiles)]
+    sn = np.zeros(len(rays), np.int64)
+    sl = np.zeros(len(rays), np.int64)
+    for a in range(ambient_samples):
+        amb = np.zeros(len(rays), RAY_DTYPE)
+        for f in RAY_DTYPE.names:
+            amb[f] = np.float32(np.nan)
+        if len(idx):
+            d = _sphere_dirs(idx, a, nrm[idx])
+            sub = make_rays(np.zeros_like(d), d, tmax=ambient_distance)
+            sub["ox"], sub["oy"], sub["oz"] = origin[:, 0], origin[:, 1], origin[:, 2]
+            amb[idx] = sub
+        recs, c = scene.workgroup_steps(amb, wgsq, mode="any")
+        for i in range(n_tiles):
+            secondary[i].append(recs[i])
+        sn += c["nodes"]
+        sl += c["leaves"]
+    secondary = [np.concatenate(t) if t else np.zeros(0, primary[0].dtype) for t in secondary]
+    files = write_workgroup_analysis(path, name, problem, camera_id, width, height, wg, primary, secondary,
+                                     (pc["nodes"], pc["leaves"]), (sn, sl), wide_alternative)
+    return primary, secondary, files
+
+
 SCENES = {
     "spheres": dict(make=lambda: spheres_scene(0), cam=((0.0, 4.0, 32.0), (0.0, 0.0, 0.0))),
     "interior": dict(make=lambda: interior_scene(2, 1_000_000), cam=((0.0, 5.0, -14.0), (0.0, 3.0, 0.0))),
@@ -254,9 +507,16 @@ class RayTracer:
         cam = CameraFast(positions, targets, cfg.get("xRes", 1920), cfg.get("yRes", 1088),
                          cfg.get("workGroupSize", 16))
         mode = "save_distance" if cfg.get("saveDistance") else "closest"
-        total, primary = cam.render_images(scene, bvh, cfg.get("ambientSampleCount", 1), ambient, mode=mode)
+        total, primary = cam.render_images(scene, bvh, cfg.get("ambientSampleCount", 1), ambient, mode=mode,
+                                           save_image=cfg.get("saveImage", False), path=self.out_dir, name=name)
         os.makedirs(self.out_dir, exist_ok=True)
         problem = f"_b{branching}_l{leaf_size}"
+        if cfg.get("doWorkGroupAnalysis") and cfg.get("wideRender"):
+            # cameraData.h:121-133: the analysis needs the wide renderer
+            for c in range(len(positions)):
+                workgroup_analysis(scene, bvh, positions[c], targets[c], cam.width, cam.height, cam.wg,
+                                   max(1, cfg.get("ambientSampleCount", 1)), ambient, self.out_dir, name, problem,
+                                   camera_id=c, wide_alternative=cfg.get("wideAlternative", True))
         prefix = f"_mb{bvh.node_memory}_ml{bvh.leaf_memory}"
         write_perf(os.path.join(self.out_dir, f"{name}{problem}{prefix}_Perf.txt"), name, problem, self.gang,
                    bvh.node_memory, bvh.leaf_memory, branching, leaf_size, total, primary, 0.0,

@@ -174,6 +174,31 @@ typedef struct rt_trace_hist {
 int rt_trace_histograms(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, void* out,
                         rt_trace_hist* hist, void* hip_stream);
 
+/* Per-step workload of ray tiles: replaces the instrumented wide traversal
+ * CompactNodeManager::intersectWideAlternative (compactNode.cpp:358-601),
+ * whose per-workgroup vectors cameraData.h:660-1109 writes out.  The rays
+ * form tiles of `tile` consecutive rays (a camera workgroup of wg x wg
+ * pixels in CameraFast tile order); n must be a multiple of `tile`.
+ * Within a tile every unfinished ray takes exactly one visit per step (one
+ * node or one leaf, in its own depth-first order, as rt_trace with RT_CLOSEST
+ * or RT_ANY traverses it), and each step records how many rays visit a node
+ * or a leaf, how many distinct nodes and leaves they visit, and how many rays
+ * finish.  Record 0 is the state before the first step: node_work = rays
+ * started, node_unique = 1, leaf_* = 0, terminations = rays not started.
+ * With RT_ANY a ray whose origin x is NaN (a pixel without a primary hit)
+ * is not started, as in the reference; with RT_CLOSEST every ray starts.
+ * out: (n / tile) x step_cap records; steps[i] = records of tile i (its
+ * longest ray's visits + 1).  A tile with more than step_cap records
+ * writes the first step_cap and reports its full count in steps[i]: call
+ * again with step_cap = max(steps).  counts (optional) receives each ray's
+ * visits (zero for a ray not started).  flags: RT_CLOSEST or RT_ANY, plus
+ * RT_DEVICE_PTRS; the call waits for the result.  tile <= 1024. */
+typedef struct rt_step_stat {
+    uint32_t node_work, node_unique, leaf_work, leaf_unique, terminations;
+} rt_step_stat;
+int rt_trace_steps(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, uint32_t tile,
+                   uint32_t step_cap, rt_step_stat* out, uint32_t* steps, rt_counts* counts, void* hip_stream);
+
 /* Overrides the launch configuration (block size 64, 128, 192 or 256,
  * LDS stack window 8/16/32 entries).  0 keeps the current value. */
 int rt_set_launch(rt_scene_t s, uint32_t block_size, uint32_t stack_window);

@@ -59,6 +59,7 @@ def lib():
             "orc_mt_f64": (ctypes.c_int, [_P, _P, _P, _P, _P]),
             "orc_bruteforce_closest": (ctypes.c_int32, [_P, ctypes.c_uint32, _P, _P]),
             "orc_bruteforce_any": (ctypes.c_int, [_P, ctypes.c_uint32, _P]),
+            "orc_visit_sequences": (None, [_P, _P, ctypes.c_uint32, ctypes.c_int, _P, ctypes.c_uint32, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -100,6 +101,18 @@ class Oracle:
         used = lib().orc_trace(ctypes.byref(self.s), _p(rays), n, mode, int(stream), tile, threads,
                                _p(hits), _p(occ), _p(cnt), _p(diag))
         return {"hits": hits, "occluded": occ, "counts": cnt, "diag": diag, "threads": used}
+
+    def visit_sequences(self, rays, mode=ORC_CLOSEST, cap=4096):
+        """(ids, lens): ids[i, :lens[i]] are the FastNode ids ray i visits in
+        order (leaves | 1 << 31).  Raises if a ray visits more than cap."""
+        rays = np.ascontiguousarray(rays).view(RAY_DTYPE)
+        n = len(rays)
+        ids = np.zeros((n, cap), np.uint32)
+        lens = np.zeros(n, np.uint32)
+        lib().orc_visit_sequences(ctypes.byref(self.s), _p(rays), n, mode, _p(ids), cap, _p(lens))
+        if n and int(lens.max()) > cap:
+            raise ValueError(f"a ray takes {int(lens.max())} visits, more than cap {cap}")
+        return ids, lens
 
     def wide(self, rays, flags=0):
         rays = np.array(rays, copy=True).view(RAY_DTYPE)
@@ -150,3 +163,37 @@ def bruteforce_closest(tris, ray):
 def bruteforce_any(tris, ray):
     tris = np.ascontiguousarray(tris, np.float32)
     return lib().orc_bruteforce_any(_p(tris), tris.shape[0], _p(ray))
+
+
+STEP_DTYPE = np.dtype([("node_work", "<u4"), ("node_unique", "<u4"), ("leaf_work", "<u4"),
+                       ("leaf_unique", "<u4"), ("terminations", "<u4")])
+
+
+def tile_step_records(ids, lens, tile, started):
+    """Per-step records of tiles of `tile` consecutive rays, as the counting
+    of CompactNodeManager::intersectWideAlternative (compactNode.cpp:358-601)
+    produces them when every unfinished ray takes one visit per step in its
+    own per-ray order (ids, lens from Oracle.visit_sequences).  Record 0 is
+    the state before the loop (:419-424): rays started, 1, 0, 0, rays not
+    started; record t counts the rays whose t-th visit is a node / a leaf,
+    the distinct ids among them (:429-435,509-519 std::set) and the rays
+    whose last visit is the t-th (:474-477,573-584).  A ray not started
+    (started[i] false) takes no visit."""
+    n = len(lens)
+    out = []
+    for i in range(n // tile):
+        sl = slice(i * tile, (i + 1) * tile)
+        st = np.asarray(started[sl], bool)
+        v = np.where(st, lens[sl].astype(np.int64), 0)
+        steps = int(v.max()) if len(v) else 0
+        rec = np.zeros(steps + 1, STEP_DTYPE)
+        rec[0] = (int(st.sum()), 1, 0, 0, int((~st).sum()))
+        tid = ids[sl]
+        for t in range(1, steps + 1):
+            act = v >= t
+            col = tid[act, t - 1]
+            leaf = (col >> np.uint32(31)) != 0
+            rec[t] = (int((~leaf).sum()), len(np.unique(col[~leaf])), int(leaf.sum()),
+                      len(np.unique(col[leaf])), int((v == t).sum()))
+        out.append(rec)
+    return out

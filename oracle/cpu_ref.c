@@ -176,7 +176,18 @@ static inline int is_leaf(const orc_bvh* b, uint32_t id) {
     return isnan(b->bounds[(size_t)id * 6 * b->M]);
 }
 
-static inline void count_add(orc_counts* c, int leaf, int k) {
+/* Visit recorder of orc_visit_sequences: FastNode ids in visit order,
+ * leaves with bit 31 set; len counts every visit, ids keeps the first cap. */
+typedef struct orc_seq {
+    uint32_t* ids;
+    uint32_t cap, len;
+} orc_seq;
+
+static inline void count_add(orc_counts* c, orc_seq* q, uint32_t id, int leaf, int k) {
+    if (q) {
+        if (q->len < q->cap) q->ids[q->len] = id | (leaf ? 0x80000000u : 0u);
+        q->len++;
+    }
     if (!c) return;
     if (leaf) {
         c->leaves++;
@@ -188,8 +199,8 @@ static inline void count_add(orc_counts* c, int leaf, int k) {
 }
 
 /* FastNodeManager::intersect (fastNodeManager.h:1328-1418) */
-int orc_intersect(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_index, int32_t* tri_index,
-                  orc_counts* counts, uint32_t* diag) {
+static int intersect_impl(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_index, int32_t* tri_index,
+                          orc_counts* counts, uint32_t* diag, orc_seq* q) {
     uint32_t stack[ORC_STACK_CAP];
     int sp = 1, result = 0;
     uint32_t dg = 0;
@@ -209,7 +220,7 @@ int orc_intersect(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_index, int32_t*
         }
         if (is_leaf(b, id)) {
             const int k = orc_leaf_loop_count(b, b->base[id]);
-            count_add(counts, 1, k);
+            count_add(counts, q, id, 1, k);
             const int r = orc_tri_intersect(b->tri, b->base[id], k, ri, (int)b->G);
             if (r != -1) {
                 result = 1;
@@ -218,7 +229,7 @@ int orc_intersect(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_index, int32_t*
             }
         } else {
             const int k = orc_node_loop_count(b, id);
-            count_add(counts, 0, k);
+            count_add(counts, q, id, 0, k);
             const float* bd = b->bounds + (size_t)id * 6 * M;
             if (orc_aabb_intersect(bd, k, dist, ri)) {
                 const int8_t* ord = b->order + (size_t)id * 3 * M + (size_t)code * M;
@@ -245,8 +256,8 @@ done:
 /* FastNodeManager::intersectSaveDistance (fastNodeManager.h:1232-1326):
  * entries carry the child's t0; pops with t0 > tMax are skipped uncounted;
  * children are pushed whether or not the node test reported a hit. */
-int orc_intersect_save_distance(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_index,
-                                int32_t* tri_index, orc_counts* counts, uint32_t* diag) {
+static int save_distance_impl(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_index, int32_t* tri_index,
+                              orc_counts* counts, uint32_t* diag, orc_seq* q) {
     uint32_t stack[ORC_STACK_CAP];
     float sdist[ORC_STACK_CAP];
     int sp = 1, result = 0;
@@ -270,7 +281,7 @@ int orc_intersect_save_distance(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_i
         }
         if (is_leaf(b, id)) {
             const int k = orc_leaf_loop_count(b, b->base[id]);
-            count_add(counts, 1, k);
+            count_add(counts, q, id, 1, k);
             const int r = orc_tri_intersect(b->tri, b->base[id], k, ri, (int)b->G);
             if (r != -1) {
                 result = 1;
@@ -279,7 +290,7 @@ int orc_intersect_save_distance(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_i
             }
         } else {
             const int k = orc_node_loop_count(b, id);
-            count_add(counts, 0, k);
+            count_add(counts, q, id, 0, k);
             const float* bd = b->bounds + (size_t)id * 6 * M;
             (void)orc_aabb_intersect(bd, k, dist, ri);
             const int8_t* ord = b->order + (size_t)id * 3 * M + (size_t)code * M;
@@ -307,7 +318,7 @@ done:
 /* FastNodeManager::intersectSecondary (fastNodeManager.h:1420-1493):
  * any-hit; children pushed in slot order 0..M-1; stops at the first leaf
  * whose triAnyHit is true. */
-int orc_intersect_secondary(const orc_bvh* b, orc_ray* ray, orc_counts* counts, uint32_t* diag) {
+static int secondary_impl(const orc_bvh* b, orc_ray* ray, orc_counts* counts, uint32_t* diag, orc_seq* q) {
     uint32_t stack[ORC_STACK_CAP];
     int sp = 1;
     uint32_t dg = 0;
@@ -325,14 +336,14 @@ int orc_intersect_secondary(const orc_bvh* b, orc_ray* ray, orc_counts* counts, 
         }
         if (is_leaf(b, id)) {
             const int k = orc_leaf_loop_count(b, b->base[id]);
-            count_add(counts, 1, k);
+            count_add(counts, q, id, 1, k);
             if (orc_tri_anyhit(b->tri, b->base[id], k, ri)) {
                 result = 1;
                 break;
             }
         } else {
             const int k = orc_node_loop_count(b, id);
-            count_add(counts, 0, k);
+            count_add(counts, q, id, 0, k);
             const float* bd = b->bounds + (size_t)id * 6 * M;
             if (orc_aabb_intersect(bd, k, dist, ri)) {
                 for (uint32_t i = 0; i < M; ++i) {
@@ -352,6 +363,39 @@ int orc_intersect_secondary(const orc_bvh* b, orc_ray* ray, orc_counts* counts, 
 done:
     if (diag) *diag |= dg;
     return result;
+}
+
+int orc_intersect(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_index, int32_t* tri_index,
+                  orc_counts* counts, uint32_t* diag) {
+    return intersect_impl(b, ray, leaf_index, tri_index, counts, diag, NULL);
+}
+
+int orc_intersect_save_distance(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_index,
+                                int32_t* tri_index, orc_counts* counts, uint32_t* diag) {
+    return save_distance_impl(b, ray, leaf_index, tri_index, counts, diag, NULL);
+}
+
+int orc_intersect_secondary(const orc_bvh* b, orc_ray* ray, orc_counts* counts, uint32_t* diag) {
+    return secondary_impl(b, ray, counts, diag, NULL);
+}
+
+/* Visit sequence of each ray under the per-ray traversal of `mode`: the
+ * order in which a lockstep wide traversal (compactNode.cpp:358-601, one
+ * visit per ray per step) takes the ray's visits.  ids: n x cap; lens[i] =
+ * visits of ray i (ids keeps the first cap). */
+void orc_visit_sequences(const orc_bvh* b, const orc_ray* rays, uint32_t n, int mode, uint32_t* ids,
+                         uint32_t cap, uint32_t* lens) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long i = 0; i < (long)n; ++i) {
+        orc_ray r = rays[i];
+        orc_seq q = {ids + (size_t)i * cap, cap, 0};
+        uint32_t leaf = 0;
+        int32_t slot = -1;
+        if (mode == ORC_ANY) (void)secondary_impl(b, &r, NULL, NULL, &q);
+        else if (mode == ORC_SAVE_DIST) (void)save_distance_impl(b, &r, &leaf, &slot, NULL, NULL, &q);
+        else (void)intersect_impl(b, &r, &leaf, &slot, NULL, NULL, &q);
+        lens[i] = q.len;
+    }
 }
 
 /* Stream traversal state shared by the two wide forms. */
@@ -434,7 +478,7 @@ void orc_intersect_wide(const orc_bvh* b, orc_ray* rays, uint32_t count, uint32_
                 if (diag) diag[r] |= ORC_DIAG_BOGUS;
             } else {
                 const int k = orc_node_loop_count(b, id);
-                if (counts) count_add(&counts[r], 0, k);
+                if (counts) count_add(&counts[r], NULL, 0, 0, k);
                 if (orc_aabb_intersect(b->bounds + (size_t)id * 6 * M, k, dist, ri)) {
                     const uint32_t axis = (flags & ORC_WIDE_AXIS_BUG) ? (major[i] >> 1) : (major[r] >> 1);
                     const int8_t* ord = b->order + (size_t)id * 3 * M + (size_t)axis * M;
@@ -469,7 +513,7 @@ void orc_intersect_wide(const orc_bvh* b, orc_ray* rays, uint32_t count, uint32_
                 if (diag) diag[r] |= ORC_DIAG_BOGUS;
             } else {
                 const int k = orc_leaf_loop_count(b, b->base[id]);
-                if (counts) count_add(&counts[r], 1, k);
+                if (counts) count_add(&counts[r], NULL, 0, 1, k);
                 const int res = orc_tri_intersect(b->tri, b->base[id], k, ri, (int)b->G);
                 if (res != -1) {
                     leaf_index[r] = b->base[id];
@@ -526,7 +570,7 @@ void orc_intersect_secondary_wide(const orc_bvh* b, orc_ray* rays, uint32_t coun
                 if (diag) diag[r] |= ORC_DIAG_BOGUS;
             } else {
                 const int k = orc_node_loop_count(b, id);
-                if (counts) count_add(&counts[r], 0, k);
+                if (counts) count_add(&counts[r], NULL, 0, 0, k);
                 if (orc_aabb_intersect(b->bounds + (size_t)id * 6 * M, k, dist, ri)) {
                     for (uint32_t c = 0; c < M; ++c) {
                         if (!isnan(dist[c])) {
@@ -557,7 +601,7 @@ void orc_intersect_secondary_wide(const orc_bvh* b, orc_ray* rays, uint32_t coun
                 if (diag) diag[r] |= ORC_DIAG_BOGUS;
             } else {
                 const int k = orc_leaf_loop_count(b, b->base[id]);
-                if (counts) count_add(&counts[r], 1, k);
+                if (counts) count_add(&counts[r], NULL, 0, 1, k);
                 hit = orc_tri_anyhit(b->tri, b->base[id], k, ri);
             }
             if (hit) {

@@ -30,6 +30,9 @@
  *   save_distance   same hit as intersect (brute force), fewer/equal visits
  *   secondary       brute-force any-hit, slot-order push on hand-built BVHs
  *   wide / secondary_wide   per-ray == stream (emulation flags off)
+ *   visit_sequences length == nodes + leaves of the counts, root first,
+ *                   every later id a child of an earlier visited node, the
+ *                   hit leaf visited; hand-built BVH with a known sequence
  */
 #ifndef BVHRT_ORACLE_CPU_REF_H
 #define BVHRT_ORACLE_CPU_REF_H
@@ -93,6 +96,12 @@ int orc_intersect(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_index, int32_t*
 int orc_intersect_save_distance(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_index,
                                 int32_t* tri_index, orc_counts* counts, uint32_t* diag);
 int orc_intersect_secondary(const orc_bvh* b, orc_ray* ray, orc_counts* counts, uint32_t* diag);
+
+/* visit sequences: per ray, the FastNode ids it visits in order under the
+ * per-ray traversal of `mode` (ORC_*), leaves with bit 31 set; ids is
+ * n x cap, lens[i] the visits of ray i (the first cap are kept) */
+void orc_visit_sequences(const orc_bvh* b, const orc_ray* rays, uint32_t n, int mode, uint32_t* ids,
+                         uint32_t cap, uint32_t* lens);
 
 /* stream forms over `count` rays (count <= 4096) */
 void orc_intersect_wide(const orc_bvh* b, orc_ray* rays, uint32_t count, uint32_t flags,
