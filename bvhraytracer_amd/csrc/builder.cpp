@@ -47,6 +47,16 @@ struct Box {
             mx[a] = gmax(mx[a], p.bmax[a]);
         }
     }
+    // union with a box built by add() from the same initial corners: gmin /
+    // gmax of two non-NaN values (add() never takes a NaN in), and the
+    // earlier box wins ties, so folding chunk boxes in order equals adding
+    // the primitives one by one
+    void merge(const Box& o) {
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = gmin(mn[a], o.mn[a]);
+            mx[a] = gmax(mx[a], o.mx[a]);
+        }
+    }
     // Aabb::getSurfaceArea (aabb.h:130-134)
     float area() const {
         float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
@@ -65,6 +75,19 @@ inline float sah_leaf(size_t count, float area, float invArea, int leafTarget) {
     return (float)steps * area * invArea * (float)leafTarget;
 }
 
+// Ranges at least this long are swept, scanned and sorted on several threads
+// (only near the root: below it the subtrees already run in parallel).
+constexpr size_t kParallelRange = 1u << 17;
+
+// fn(k) for k in [0, parts): parts - 1 new threads, the caller runs part 0.
+template <class F>
+void run_parts(size_t parts, F fn) {
+    std::vector<std::thread> ts;
+    for (size_t k = 1; k < parts; ++k) ts.emplace_back([&fn, k] { fn(k); });
+    fn(0);
+    for (auto& t : ts) t.join();
+}
+
 // Stable sort of [first, last) on up to `threads` threads: chunks are
 // stable-sorted in parallel, then adjacent runs are merged pairwise with
 // std::merge (which takes equal keys from the left run first).  A stable sort
@@ -75,7 +98,7 @@ void parallel_stable_sort(It first, It last, Less less, int threads) {
     using T = typename std::iterator_traits<It>::value_type;
     const size_t n = (size_t)(last - first);
     const size_t parts = std::min<size_t>((size_t)std::max(1, threads), 16);
-    if (parts < 2 || n < (1u << 17)) {
+    if (parts < 2 || n < kParallelRange) {
         std::stable_sort(first, last, less);
         return;
     }
@@ -126,46 +149,94 @@ struct Builder {
         : prims(p), N(n), L(l), sortEachSplit(s), leafSplitOption(o), sahFactor(f),
           spareThreads(threads - 1), threads(threads) {}
 
-    // chooseAxisAndSort (aabb.h:40-69)
+    size_t parts_for(size_t n) const {
+        return (n >= kParallelRange && threads > 1) ? (size_t)std::min(threads, 16) : 1;
+    }
+
+    // chooseAxisAndSort (aabb.h:40-69).  The centroid bounds and the
+    // is-sorted test are exact in chunks (see Box::merge; sortedness is
+    // per chunk plus the chunk seams).
     int choose_axis_and_sort(uint32_t b, uint32_t e) {
-        float mn[3] = {222222.0f, 222222.0f, 222222.0f};
-        float mx[3] = {-222222.0f, -222222.0f, -222222.0f};
-        for (uint32_t i = b; i < e; ++i)
-            for (int a = 0; a < 3; ++a) {
-                mn[a] = gmin(mn[a], prims[i].c[a]);
-                mx[a] = gmax(mx[a], prims[i].c[a]);
-            }
-        int axis = max_dimension(mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]);
+        const size_t n = e - b, parts = parts_for(n);
+        std::vector<Box> cb(parts);
+        run_parts(parts, [&](size_t k) {
+            Box& x = cb[k];
+            for (size_t i = b + n * k / parts; i < b + n * (k + 1) / parts; ++i)
+                for (int a = 0; a < 3; ++a) {
+                    x.mn[a] = gmin(x.mn[a], prims[i].c[a]);
+                    x.mx[a] = gmax(x.mx[a], prims[i].c[a]);
+                }
+        });
+        Box c;
+        for (const Box& x : cb) c.merge(x);
+        int axis = max_dimension(c.mx[0] - c.mn[0], c.mx[1] - c.mn[1], c.mx[2] - c.mn[2]);
         auto less = [axis](const Prim& p1, const Prim& p2) { return p1.c[axis] < p2.c[axis]; };
-        if (!std::is_sorted(prims.begin() + b, prims.begin() + e, less)) {
+        std::vector<char> ok(parts, 1);
+        run_parts(parts, [&](size_t k) {
+            const size_t lo = b + n * k / parts, hi = b + n * (k + 1) / parts;
+            ok[k] = std::is_sorted(prims.begin() + lo, prims.begin() + hi, less) &&
+                    (hi == e || !less(prims[hi], prims[hi - 1]));
+        });
+        if (std::find(ok.begin(), ok.end(), 0) != ok.end()) {
             // large ranges are only sorted near the root, before the
             // subtrees take the threads
-            if (e - b >= (1u << 17)) parallel_stable_sort(prims.begin() + b, prims.begin() + e, less, threads);
+            if (parts > 1) parallel_stable_sort(prims.begin() + b, prims.begin() + e, less, threads);
             else std::stable_sort(prims.begin() + b, prims.begin() + e, less);
         }
         return axis;
     }
 
     // computerBestSplit(Sort) (aabb.cpp:4-69).  Returns the split position.
+    // Long ranges run the two sweeps in chunks: the box entering a chunk is
+    // the fold of the chunks before it (after it, for the right sweep), which
+    // equals the running box of the one-by-one sweep (Box::merge), so every
+    // metric[m] is bit-identical; the first minimum is taken per chunk and
+    // then across chunks in order.
     uint32_t best_split(uint32_t b, uint32_t e, float invArea, float& cost) {
         if (sortEachSplit) choose_axis_and_sort(b, e);
-        const size_t size = e - b;
-        std::vector<float> metric(size - 1, 0.0f);
-        Box left;  // sweepRight reads *(++primitiveEnd): prims b+1 .. b+m+1
-        for (size_t m = 0; m + 1 < size; ++m) {
-            left.add(prims[b + m + 1]);
-            metric[m] += sah_leaf(m + 1, left.area(), invArea, (int)L);
+        const size_t size = e - b, cnt = size - 1;
+        std::vector<float> metric(cnt, 0.0f);
+        const size_t parts = parts_for(size);
+        auto lo = [&](size_t k) { return cnt * k / parts; };
+        std::vector<Box> lstart(parts), rstart(parts);
+        if (parts > 1) {
+            std::vector<Box> chunk(parts);
+            run_parts(parts, [&](size_t k) {
+                for (size_t m = lo(k); m < lo(k + 1); ++m) chunk[k].add(prims[b + m + 1]);
+            });
+            Box acc;
+            for (size_t k = 0; k < parts; ++k) {
+                lstart[k] = acc;
+                acc.merge(chunk[k]);
+            }
+            acc = Box();
+            for (size_t k = parts; k-- > 0;) {
+                rstart[k] = acc;
+                acc.merge(chunk[k]);
+            }
         }
-        Box right;  // sweepLeft reads *(--primitiveBegin): prims b+m+1 .. e-1
-        for (size_t m = size - 1; m-- > 0;) {
-            right.add(prims[b + m + 1]);
-            metric[m] += sah_leaf(size - 1 - m, right.area(), invArea, (int)L);
-        }
-        size_t best = 0;  // std::min_element: first minimum
-        for (size_t m = 1; m < metric.size(); ++m)
-            if (metric[m] < metric[best]) best = m;
-        cost = metric[best];
-        return b + (uint32_t)best + 1;
+        std::vector<size_t> best(parts, 0);
+        run_parts(parts, [&](size_t k) {
+            Box left = lstart[k];  // sweepRight reads *(++primitiveEnd): prims b+1 .. b+m+1
+            for (size_t m = lo(k); m < lo(k + 1); ++m) {
+                left.add(prims[b + m + 1]);
+                metric[m] += sah_leaf(m + 1, left.area(), invArea, (int)L);
+            }
+            Box right = rstart[k];  // sweepLeft reads *(--primitiveBegin): prims b+m+1 .. e-1
+            for (size_t m = lo(k + 1); m-- > lo(k);) {
+                right.add(prims[b + m + 1]);
+                metric[m] += sah_leaf(size - 1 - m, right.area(), invArea, (int)L);
+            }
+            size_t bm = lo(k);  // std::min_element: first minimum
+            for (size_t m = lo(k) + 1; m < lo(k + 1); ++m)
+                if (metric[m] < metric[bm]) bm = m;
+            best[k] = bm;
+        });
+        size_t bi = best[0];
+        for (size_t k = 1; k < parts; ++k)
+            if (lo(k) < lo(k + 1) && metric[best[k]] < metric[bi]) bi = best[k];
+        cost = metric[bi];
+        return b + (uint32_t)bi + 1;
     }
 
     void bounds_of(BNode& n) {
@@ -267,7 +338,7 @@ struct Builder {
     }
 };
 
-void flatten(const BNode& root, const std::vector<Prim>& prims, const float* tris, unsigned G,
+void flatten(const BNode& root, const std::vector<Prim>& prims, const float* tris, unsigned G, int threads,
              HostBvh& out) {
     // customTreeOrder (fastNodeManager.h:1675-1688): root, then every node's
     // children as one block, recursively in child order.  Entering a node
@@ -299,63 +370,75 @@ void flatten(const BNode& root, const std::vector<Prim>& prims, const float* tri
     const uint32_t M = out.M, Lm = out.Lm;
     const size_t nn = nodes.size();
     auto id_of = [](const BNode* p) { return p->fid; };
+    // leaf blocks are appended in node order: leaf ordinal of each node
+    std::vector<uint32_t> leafOrd(nn, 0);
+    uint32_t leaves = 0;
+    for (size_t i = 0; i < nn; ++i)
+        if (nodes[i]->end > nodes[i]->begin) leafOrd[i] = leaves++;
     out.bounds.assign(nn * 6 * M, 0.0f);
     out.base.assign(nn, 0);
     out.order.assign(nn * 3 * M, 0);
     out.childType.assign(nn, 0);
-    out.tri.clear();
-    out.triIds.clear();
+    out.tri.assign((size_t)leaves * 9 * Lm, 0.0f);
+    out.triIds.assign((size_t)leaves * Lm, -1);
     const float qnan = std::numeric_limits<float>::quiet_NaN();
-    double depthSum = 0;
-    uint32_t leaves = 0, triCount = 0, maxDepth = 0;
-    for (size_t i = 0; i < nn; ++i) {
-        const BNode& n = *nodes[i];
-        float* bd = &out.bounds[i * 6 * M];
-        const size_t cc = n.children.size();
-        if (cc > 0) {
-            out.base[i] = id_of(n.children[0].get());
-            const uint32_t nrs = (uint32_t)(((cc - 1) / G + 1) * G);
-            for (int a = 0; a < 3; ++a)
-                for (size_t j = 0; j < cc; ++j) {
-                    bd[a * nrs + j] = n.children[j]->box.mn[a];
-                    bd[nrs * 3 + a * nrs + j] = n.children[j]->box.mx[a];
-                }
-            if (nrs != M) {
-                bd[6 * M - 2] = (float)nrs;
-                bd[6 * M - 1] = qnan;
-            }
-            uint16_t ct = 0;
-            for (size_t j = 0; j < cc; ++j)
-                if (!n.children[j]->children.empty()) ct |= (uint16_t)(1u << j);
-            out.childType[i] = ct;
-            for (int a = 0; a < 3; ++a)
-                for (uint32_t p = 0; p < M; ++p)
-                    out.order[i * 3 * M + a * M + p] =
-                        (p < n.order[0].size()) ? n.order[a][p] : (int8_t)0;
-        }
-        const uint32_t pc = n.end - n.begin;
-        if (pc > 0) {
-            const uint32_t pBegin = (uint32_t)out.tri.size();
-            const uint32_t lrs = ((pc - 1) / G + 1) * G;
-            out.tri.resize(pBegin + (size_t)9 * Lm, 0.0f);
-            for (int v = 0; v < 3; ++v)
+    // nodes are written independently: ranges of the node array on threads
+    const size_t parts = (nn >= (1u << 15) && threads > 1) ? (size_t)std::min(threads, 16) : 1;
+    std::vector<double> depthSums(parts, 0.0);
+    std::vector<uint32_t> triCounts(parts, 0), maxDepths(parts, 0);
+    run_parts(parts, [&](size_t k) {
+        for (size_t i = nn * k / parts; i < nn * (k + 1) / parts; ++i) {
+            const BNode& n = *nodes[i];
+            float* bd = &out.bounds[i * 6 * M];
+            const size_t cc = n.children.size();
+            if (cc > 0) {
+                out.base[i] = id_of(n.children[0].get());
+                const uint32_t nrs = (uint32_t)(((cc - 1) / G + 1) * G);
                 for (int a = 0; a < 3; ++a)
-                    for (uint32_t j = 0; j < pc; ++j)
-                        out.tri[pBegin + (v * 3 + a) * lrs + j] =
-                            tris[(size_t)prims[n.begin + j].id * 9 + v * 3 + a];
-            if (lrs != Lm) {
-                out.tri[pBegin + Lm * 9 - 2] = (float)lrs;
-                out.tri[pBegin + Lm * 9 - 1] = qnan;
+                    for (size_t j = 0; j < cc; ++j) {
+                        bd[a * nrs + j] = n.children[j]->box.mn[a];
+                        bd[nrs * 3 + a * nrs + j] = n.children[j]->box.mx[a];
+                    }
+                if (nrs != M) {
+                    bd[6 * M - 2] = (float)nrs;
+                    bd[6 * M - 1] = qnan;
+                }
+                uint16_t ct = 0;
+                for (size_t j = 0; j < cc; ++j)
+                    if (!n.children[j]->children.empty()) ct |= (uint16_t)(1u << j);
+                out.childType[i] = ct;
+                for (int a = 0; a < 3; ++a)
+                    for (uint32_t p = 0; p < M; ++p)
+                        out.order[i * 3 * M + a * M + p] = (p < n.order[0].size()) ? n.order[a][p] : (int8_t)0;
             }
-            bd[0] = qnan;
-            out.base[i] = pBegin;
-            for (uint32_t j = 0; j < Lm; ++j)
-                out.triIds.push_back(j < pc ? (int32_t)prims[n.begin + j].id : -1);
-            depthSum += n.depth;
-            ++leaves;
-            triCount += pc;
-            maxDepth = std::max<uint32_t>(maxDepth, n.depth);
+            const uint32_t pc = n.end - n.begin;
+            if (pc > 0) {
+                const size_t pBegin = (size_t)leafOrd[i] * 9 * Lm;
+                const uint32_t lrs = ((pc - 1) / G + 1) * G;
+                for (int v = 0; v < 3; ++v)
+                    for (int a = 0; a < 3; ++a)
+                        for (uint32_t j = 0; j < pc; ++j)
+                            out.tri[pBegin + (v * 3 + a) * lrs + j] = tris[(size_t)prims[n.begin + j].id * 9 + v * 3 + a];
+                if (lrs != Lm) {
+                    out.tri[pBegin + Lm * 9 - 2] = (float)lrs;
+                    out.tri[pBegin + Lm * 9 - 1] = qnan;
+                }
+                bd[0] = qnan;
+                out.base[i] = (uint32_t)pBegin;
+                for (uint32_t j = 0; j < pc; ++j)
+                    out.triIds[(size_t)leafOrd[i] * Lm + j] = (int32_t)prims[n.begin + j].id;
+                depthSums[k] += n.depth;
+                triCounts[k] += pc;
+                maxDepths[k] = std::max<uint32_t>(maxDepths[k], n.depth);
+            }
         }
+    });
+    double depthSum = 0;  // integer depths: exact in any order
+    uint32_t triCount = 0, maxDepth = 0;
+    for (size_t k = 0; k < parts; ++k) {
+        depthSum += depthSums[k];
+        triCount += triCounts[k];
+        maxDepth = std::max(maxDepth, maxDepths[k]);
     }
     out.averageDepth = leaves ? (float)(depthSum / leaves) : 0.0f;
     out.triangleCount = triCount;
@@ -400,7 +483,7 @@ int build_bvh(const float* tris, uint32_t n, const rt_build_params& p, HostBvh& 
     root.end = n;
     b.bounds_of(root);
     b.recurse(root);
-    flatten(root, prims, tris, G, out);
+    flatten(root, prims, tris, G, threads, out);
     return RT_OK;
 }
 

@@ -97,12 +97,17 @@ def test_invariants_spheres(rt):
 
 
 def test_deterministic_across_threads(rt):
-    tris = rt.interior_scene(4, 120_000)
+    """300k triangles: the ranges near the root pass 2^17, so the chunked
+    centroid scan, sort, SAH sweeps and the threaded flattening all run; the
+    arrays must equal the one-thread build byte for byte."""
+    tris = rt.interior_scene(4, 300_000)
     a = rt.Bvh(tris, 8, 8, threads=1)
-    b = rt.Bvh(tris, 8, 8, threads=8)
-    for f in ("bounds", "base", "order", "child_type", "tri_points", "tri_ids"):
-        x, y = getattr(a, f), getattr(b, f)
-        assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), f
+    for th in (3, 8):
+        b = rt.Bvh(tris, 8, 8, threads=th)
+        for f in ("bounds", "base", "order", "child_type", "tri_points", "tri_ids"):
+            x, y = getattr(a, f), getattr(b, f)
+            assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), (th, f)
+        assert a.average_depth == b.average_depth and a.max_depth == b.max_depth
 
 
 def small_tri(x, s):
