@@ -195,6 +195,22 @@ extern "C" int rt_spawn_shadow(const float* normal, const float* position, const
     return RT_OK;
 }
 
+// Ambient rays (ambient_ray) of one sample for every hit pixel, compacted.
+extern "C" int rt_spawn_ambient(const float* normal, const float* position, const rt_hit* hits, uint32_t n,
+                                uint32_t sample, float dist, rt_ray* out, uint32_t* src_index, uint32_t* n_out) {
+    if (!n_out || (n && (!normal || !position || !hits || !out)))
+        return set_error(RT_ERR_ARG, "rt_spawn_ambient: bad argument");
+    uint32_t k = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (hits[i].slot < 0) continue;
+        out[k] = ambient_ray(i, sample, normal + (size_t)i * 3, position + (size_t)i * 3, dist);
+        if (src_index) src_index[k] = i;
+        ++k;
+    }
+    *n_out = k;
+    return RT_OK;
+}
+
 // Cosine-distributed bounce rays (diffuse_ray) cycling over the hit pixels:
 // ray r starts at hit number r % hits.
 extern "C" int rt_spawn_diffuse(const rt_ray* primary, const float* normal, const float* position,

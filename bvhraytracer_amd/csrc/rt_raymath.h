@@ -222,4 +222,49 @@ RT_HD rt_ray diffuse_ray(const float primaryDir[3], const float normal[3], const
     return ray;
 }
 
+// Ambient-occlusion direction number `sample` of pixel `index`, the role of
+// Camera::getAmbientDirection + sampleHemisphere (camera.cpp:21-52): a
+// direction uniform on the unit sphere, negated when its dot product with
+// the surface normal is <= 0.  The reference hashes (index, sample) with
+// std::hash, whose value is implementation defined; here (u, v) come from a
+// fixed 64-bit mix of index and sample (SplitMix64's finaliser constants),
+// 24 bits each.  The sphere point is z = 1 - 2u, r = sqrt(1 - z^2),
+// (x, y) = r (cos 2 pi v, sin 2 pi v), the same distribution as the
+// reference's phi = acos(1 - 2u), theta = 2 pi v form.
+RT_HD void ambient_dir(uint64_t index, uint32_t sample, const float normal[3], float d[3]) {
+    uint64_t h = index * 0x9E3779B97F4A7C15ull + (uint64_t)sample * 0xBF58476D1CE4E5B9ull + 1ull;
+    h ^= h >> 31;
+    h *= 0x94D049BB133111EBull;
+    h ^= h >> 29;
+    const float u = (float)(uint32_t)(h >> 40) * (1.0f / 16777216.0f);
+    const float v = (float)(uint32_t)((h >> 16) & 0xFFFFFFull) * (1.0f / 16777216.0f);
+    const float z = 1.0f - 2.0f * u;
+    const float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float sv, cv;
+    sincos_turn(v, &sv, &cv);
+    d[0] = r * cv;
+    d[1] = r * sv;
+    d[2] = z;
+    if (dot3(d, normal) <= 0.0f) {
+        d[0] = -d[0];
+        d[1] = -d[1];
+        d[2] = -d[2];
+    }
+}
+
+// Ambient ray of the performance renderer (cameraFast.h:395-418): origin
+// P + n*0.001, FastRay::updateDirection with the ambient direction itself
+// (fastRay.cpp:22-38), tMax = ambientDistance.
+RT_HD rt_ray ambient_ray(uint64_t index, uint32_t sample, const float nrm[3], const float pos[3], float dist) {
+    float d[3];
+    ambient_dir(index, sample, nrm, d);
+    rt_ray r;
+    r.tmax = dist;
+    r.ox = pos[0] + nrm[0] * 0.001f;
+    r.oy = pos[1] + nrm[1] * 0.001f;
+    r.oz = pos[2] + nrm[2] * 0.001f;
+    set_direction(r, d[0], d[1], d[2]);
+    return r;
+}
+
 }  // namespace rt

@@ -409,6 +409,20 @@ def spawn_shadow(normal, position, hits, light, tmax=1.0):
     return out[:k.value].copy(), src[:k.value].copy()
 
 
+def spawn_ambient(normal, position, hits, sample, dist):
+    """rt_spawn_ambient: ambient-occlusion rays of one sample for every hit
+    pixel (compacted); returns (rays, source pixel index)."""
+    hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)
+    n = len(hits)
+    out = np.empty(n, RAY_DTYPE)
+    src = np.empty(n, np.uint32)
+    k = ctypes.c_uint32()
+    check(lib().rt_spawn_ambient(ptr(np.ascontiguousarray(normal, np.float32)),
+                                 ptr(np.ascontiguousarray(position, np.float32)), ptr(hits), n, int(sample),
+                                 float(dist), ptr(out), ptr(src), ctypes.byref(k)))
+    return out[:k.value].copy(), src[:k.value].copy()
+
+
 def spawn_diffuse(primary, normal, position, hits, count, seed=3, tmax=222222.0):
     primary = np.ascontiguousarray(primary, dtype=RAY_DTYPE)
     hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)

@@ -13,7 +13,6 @@ the repository); everything else keeps the reference's semantics: substring
 key matching, `#` comments, the 1 warm-up + 5 timed renders median protocol,
 and the `_Perf.txt` fields.  Traversal runs on the GPU through librt.so.
 """
-import math
 import os
 import struct
 import time
@@ -21,8 +20,7 @@ import zlib
 
 import numpy as np
 
-from .host import (Bvh, Scene, camera_rays, interior_scene, load_sah_factor, make_rays,
-                   spawn_shadow, spheres_scene)
+from .host import (Bvh, Scene, camera_rays, interior_scene, load_sah_factor, spawn_ambient, spheres_scene)
 from ._lib import RAY_DTYPE
 
 INT_KEYS = ["minLeafSize", "maxLeafSize", "leafStep", "minBranch", "maxBranch", "branchStep",
@@ -104,26 +102,6 @@ def sweep(cfg):
             yield leaf, branch
 
 
-def _sphere_dirs(index, sample, normal):
-    """Deterministic uniform directions flipped into the normal's hemisphere
-    (the sampler shape of Camera::sampleHemisphere, camera.cpp:36-52, fed by
-    a PCG-style hash instead of the platform std::hash)."""
-    with np.errstate(over="ignore"):  # uint64 arithmetic wraps by design
-        s = np.uint64(sample) * np.uint64(0xBF58476D1CE4E5B9)
-    h = index.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15) + s + np.uint64(1)
-    h ^= h >> np.uint64(31)
-    h *= np.uint64(0x94D049BB133111EB)
-    h ^= h >> np.uint64(29)
-    u = ((h >> np.uint64(40)).astype(np.float64)) / float(1 << 24)
-    v = (((h >> np.uint64(16)) & np.uint64(0xFFFFFF)).astype(np.float64)) / float(1 << 24)
-    phi = np.arccos(1 - 2 * u)
-    theta = 2 * math.pi * v
-    d = np.stack([np.sin(phi) * np.cos(theta), np.sin(phi) * np.sin(theta), np.cos(phi)], 1)
-    flip = (d * normal).sum(1) <= 0
-    d[flip] = -d[flip]
-    return d.astype(np.float32)
-
-
 def tile_order_to_rows(values, width, height, wg):
     """Per-pixel values in CameraFast tile order (the order of camera_rays:
     wg x wg tiles left to right, tile rows top to bottom, lanes row-major in a
@@ -200,15 +178,9 @@ class CameraFast:
         hit = hits["slot"] >= 0
         if ambient_samples and hit.any():
             nrm, pos = bvh.surface(rays, hits)
-            idx = np.nonzero(hit)[0]
-            origin = pos[idx] + nrm[idx] * np.float32(0.001)
             for a in range(ambient_samples):
-                d = _sphere_dirs(idx, a, nrm[idx])
-                # FastRay::updateDirection (fastRay.cpp:22-38): the direction
-                # itself, not (origin + d) - origin
-                amb = make_rays(np.zeros_like(d), d, tmax=ambient_distance)
-                amb["ox"], amb["oy"], amb["oz"] = origin[:, 0], origin[:, 1], origin[:, 2]
-                occl[idx] += scene.intersect(amb, mode="any")
+                amb, src = spawn_ambient(nrm, pos, hits, a, ambient_distance)
+                occl[src] += scene.intersect(amb, mode="any")
         total = time.perf_counter() - t0
         return total, t_primary, hits, ambient_image(occl, hit, ambient_samples)
 
@@ -453,22 +425,17 @@ def workgroup_analysis(scene, bvh, pos, target, width, height, wg, ambient_sampl
     rays = camera_rays(pos, target, width, height, wg)[:n_tiles * wgsq]
     primary, pc = scene.workgroup_steps(rays, wgsq, mode="closest")
     hits = scene.intersect(rays)
-    hit = hits["slot"] >= 0
     nrm, spos = bvh.surface(rays, hits)
-    idx = np.nonzero(hit)[0]
-    origin = spos[idx] + nrm[idx] * np.float32(0.001)
     secondary = [[] for _ in range(n_tiles)]
     sn = np.zeros(len(rays), np.int64)
     sl = np.zeros(len(rays), np.int64)
     for a in range(ambient_samples):
+        # a pixel without a hit gets a NaN ray (cameraData.h:495-499)
         amb = np.zeros(len(rays), RAY_DTYPE)
         for f in RAY_DTYPE.names:
             amb[f] = np.float32(np.nan)
-        if len(idx):
-            d = _sphere_dirs(idx, a, nrm[idx])
-            sub = make_rays(np.zeros_like(d), d, tmax=ambient_distance)
-            sub["ox"], sub["oy"], sub["oz"] = origin[:, 0], origin[:, 1], origin[:, 2]
-            amb[idx] = sub
+        sub, src = spawn_ambient(nrm, spos, hits, a, ambient_distance)
+        amb[src] = sub
         recs, c = scene.workgroup_steps(amb, wgsq, mode="any")
         for i in range(n_tiles):
             secondary[i].append(recs[i])

@@ -17,9 +17,11 @@
  * failure.  No C++ exception crosses this boundary.
  *
  * Host-side entry points (rt_scene_*, rt_build*, rt_camera_*, rt_spawn_*,
- * rt_config_*) restate the reference's host code that produces the path's
+ * rt_sah_factor) restate the reference's host code that produces the path's
  * inputs: the SAH builder, the FastNode flattener, the camera and the
- * config.txt / sahFactorConfigSse.txt readers.
+ * sahFactorConfigSse.txt reader.  The renderer driver above them
+ * (config.txt, the N/L sweep, CameraFast, the reports) is the C++ layer in
+ * rt_renderer.hpp.
  */
 #ifndef BVHRT_RT_H
 #define BVHRT_RT_H
@@ -347,6 +349,15 @@ int rt_hit_surface(rt_host_bvh_t h, const rt_ray* rays, const rt_hit* hits, uint
 int rt_spawn_shadow(const float* normal, const float* position, const rt_hit* hits,
                     uint32_t n, const float light[3], float tmax, rt_ray* out,
                     uint32_t* src_index, uint32_t* n_out);
+
+/* Ambient-occlusion rays of one sample (the performance renderer's ambient
+ * loop, cameraFast.h:395-418, Camera::getAmbientDirection, camera.cpp:21-52):
+ * for every hit pixel i, in pixel order, FastRay from P + n*0.001 with a
+ * hashed direction uniform on the sphere, flipped into the normal's
+ * hemisphere (the hash is this library's, not std::hash), tMax = dist.
+ * Writes *n_out rays; src_index[k] = pixel of ray k (optional). */
+int rt_spawn_ambient(const float* normal, const float* position, const rt_hit* hits, uint32_t n,
+                     uint32_t sample, float dist, rt_ray* out, uint32_t* src_index, uint32_t* n_out);
 
 /* n_out cosine-distributed bounce rays cycling over the hits (PCG32 stream
  * per ray index); the normal is face-forwarded against the primary ray. */

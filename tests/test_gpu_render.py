@@ -17,7 +17,7 @@ POS, TGT = (0.0, 4.0, 32.0), (0.0, 0.0, 0.0)
 
 
 def test_render_image_png_matches_oracle(rt, tmp_path):
-    from bvhraytracer_amd.raytracer import CameraFast, _sphere_dirs, ambient_image, tile_order_to_rows
+    from bvhraytracer_amd.raytracer import CameraFast, ambient_image, tile_order_to_rows
 
     W, H, wg, samples, dist = 160, 120, 16, 3, 4.0
     print(f"render {W}x{H} wg {wg} samples {samples}", flush=True)
@@ -38,15 +38,12 @@ def test_render_image_png_matches_oracle(rt, tmp_path):
     hit = ref["slot"] >= 0
     assert 0 < hit.sum() < len(hit)
     nrm, pos = bvh.surface(rays, ref)
-    idx = np.nonzero(hit)[0]
-    origin = pos[idx] + nrm[idx] * np.float32(0.001)
     occl = np.zeros(len(rays), np.uint16)
     for a in range(samples):
-        d = _sphere_dirs(idx, a, nrm[idx])
-        amb = rt.make_rays(np.zeros_like(d), d, tmax=dist)
-        amb["ox"], amb["oy"], amb["oz"] = origin[:, 0], origin[:, 1], origin[:, 2]
-        occl[idx] += orc.trace(amb, mode=oracle.ORC_ANY)["occluded"].astype(np.uint16)
-    assert 0 < occl.sum() < samples * len(idx)
+        amb, src = rt.spawn_ambient(nrm, pos, ref, a, dist)
+        assert np.array_equal(src, np.nonzero(hit)[0])
+        occl[src] += orc.trace(amb, mode=oracle.ORC_ANY)["occluded"].astype(np.uint16)
+    assert 0 < occl.sum() < samples * hit.sum()
     want = tile_order_to_rows(ambient_image(occl, hit, samples), W, H, wg)
     assert np.array_equal(got[..., 0], want)
     assert np.array_equal(got[..., 1], want) and np.array_equal(got[..., 2], want)

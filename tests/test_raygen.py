@@ -174,3 +174,50 @@ def test_hit_surface_model(rt):
     miss["slot"] = -1
     nm, pm = bvh.surface(rays[:4], miss[:4])
     assert np.isnan(nm).all() and np.isnan(pm).all()
+
+
+def _mix(index, sample):
+    """The (u, v) hash of the ambient directions: a 64-bit mix with
+    SplitMix64's finaliser constants, 24 bits each."""
+    h = (index * 0x9E3779B97F4A7C15 + sample * 0xBF58476D1CE4E5B9 + 1) & M64
+    h ^= h >> 31
+    h = (h * 0x94D049BB133111EB) & M64
+    h ^= h >> 29
+    return (h >> 40) * 2.0 ** -24, ((h >> 16) & 0xFFFFFF) * 2.0 ** -24
+
+
+def test_ambient_rays_model(rt):
+    """rt_spawn_ambient against a float64 model of the uniform sphere point
+    (z = 1 - 2u, r = sqrt(1 - z^2), angle 2 pi v) flipped into the normal's
+    hemisphere; origin P + n*0.001, direction used as is (updateDirection),
+    tMax = dist; misses get no ray.  Over many samples the directions are
+    uniform on the hemisphere: mean cosine 1/2."""
+    rng = np.random.default_rng(5)
+    n = 3000
+    nrm = rng.normal(size=(n, 3)).astype(np.float32)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    pos = rng.uniform(-5, 5, size=(n, 3)).astype(np.float32)
+    hits = np.zeros(n, rt.HIT_DTYPE)
+    hits["slot"] = np.where(rng.random(n) < 0.75, 0, -1)
+    for sample in (0, 3):
+        rays, src = rt.spawn_ambient(nrm, pos, hits, sample, 2.5)
+        assert np.array_equal(src, np.nonzero(hits["slot"] >= 0)[0])
+        got = _dirs(rays)
+        for k, i in enumerate(src[:400]):
+            u, v = _mix(int(i), sample)
+            z = 1.0 - 2.0 * u
+            r = math.sqrt(max(0.0, 1.0 - z * z))
+            d = np.array([r * math.cos(2 * math.pi * v), r * math.sin(2 * math.pi * v), z])
+            if np.dot(d, nrm[i].astype(np.float64)) <= 0:
+                d = -d
+            assert np.abs(got[k] - d).max() < 2e-6, (i, got[k], d)
+        o = pos[src].astype(np.float64) + nrm[src].astype(np.float64) * 0.001
+        assert np.abs(_origins(rays) - o).max() < 1e-5
+        assert np.all(rays["tmax"] == np.float32(2.5))
+        assert np.allclose(np.linalg.norm(got, axis=1), 1.0, atol=1e-6)
+        cosv = np.einsum("ij,ij->i", got, nrm[src].astype(np.float64))
+        assert cosv.min() >= 0 and abs(cosv.mean() - 0.5) < 0.02
+    # (0, 0) by hand: h = 1 -> 1 * 0x94D049BB133111EB, then h ^= h >> 29
+    h = 0x94D049BB133111EB
+    h ^= h >> 29
+    assert _mix(0, 0) == ((h >> 40) * 2.0 ** -24, ((h >> 16) & 0xFFFFFF) * 2.0 ** -24)
