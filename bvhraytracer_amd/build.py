@@ -1,7 +1,8 @@
-"""Builds librt.so (host C++ + HIP for gfx950) in-tree.
+"""Builds librt.so (host C++ + HIP for gfx950) and the bvhrt program in-tree.
 
 Host sources are compiled with g++, HIP sources with hipcc, and the shared
-library is linked with hipcc.  Every float path is built with
+library is linked with hipcc; bvhrt (the C++ renderer's command line) links
+librt.so from its own directory.  Every float path is built with
 -ffp-contract=off: the reference kernels (ISPC, sse4-i32x4) have no FMA, and
 bit-exact parity depends on the same operation sequence.
 """
@@ -14,9 +15,10 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librt.so")
+CLI = os.path.join(HERE, "bvhrt")
 OBJ = os.path.join(HERE, "_build")
 
-HOST_SOURCES = ["scene.cpp", "builder.cpp", "host_api.cpp"]
+HOST_SOURCES = ["scene.cpp", "builder.cpp", "host_api.cpp", "renderer.cpp"]
 HIP_SOURCES = ["rt_kernels.hip", "rt_api.hip", "rt_serve.hip", "rt_shade.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -54,8 +56,14 @@ def build(verbose=False, out=OUT, obj_dir=OBJ, defines=()):
                 print(text)
     objs = [os.path.join(obj_dir, s + ".o") for s in HOST_SOURCES + HIP_SOURCES]
     tmp = out + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-pthread"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-soname,librt.so", "-o", tmp, *objs, "-pthread",
+          "-lz"])
     os.replace(tmp, out)
+    if out == OUT:
+        # the program finds librt.so next to itself ($ORIGIN), never the system librt
+        tmp = CLI + ".tmp"
+        _run(["g++", *CXXFLAGS, "-o", tmp, os.path.join(CSRC, "bvhrt_main.cpp"), out, "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmp, CLI)
     return out
 
 

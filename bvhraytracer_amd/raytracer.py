@@ -146,6 +146,16 @@ def write_png(path, rgba):
         f.write(png)
 
 
+def ambient_distance(tris):
+    """ambientDistance of RayTracer::renderImage (rayTracer.cpp:607): cbrt of
+    the root box volume (Aabb::getVolume, aabb.h:136-140) / 10, in float32,
+    the root box being the triangles' bounds."""
+    pts = np.asarray(tris, np.float32).reshape(-1, 3)
+    d = pts.max(0) - pts.min(0)
+    vol = np.float32(np.float32(d[0] * d[1]) * d[2])
+    return float(np.float32(np.cbrt(vol) / np.float32(10)))
+
+
 def ambient_image(occluded, hit, samples):
     """Gray value per pixel (cameraFast.h:404-439): a hit pixel is
     (uint8)((1 - occluded / samples) * 255) in float32, a miss is 0.  With
@@ -220,11 +230,13 @@ def write_perf(path, name, problem, gang, node_memory, leaf_memory, branching, l
     with open(path, "w") as f:
         f.write(f"scenario {name} with branching factor of {branching} and leafsize of {leaf_size}\n")
         f.write(f"with a gang size of {gang}, a node meory of {node_memory}, and a leaf memory of {leaf_memory}\n")
-        f.write(f"Raytracer total time: {total}\n")
-        f.write(f"Time for all rays (SUM): {ray_sum}\n")
-        f.write(f"Time for triangle intersections (SUM): {tri_sum}\n")
-        f.write(f"Time all rays(sum) - triangle(sum): {ray_sum - tri_sum}\n")
-        f.write(f"Average BVH depth: {avg_depth}\n")
+        # the reference streams floats: %g with 6 significant digits
+        total, ray_sum, tri_sum = np.float32(total), np.float32(ray_sum), np.float32(tri_sum)
+        f.write(f"Raytracer total time: {_g(total)}\n")
+        f.write(f"Time for all rays (SUM): {_g(ray_sum)}\n")
+        f.write(f"Time for triangle intersections (SUM): {_g(tri_sum)}\n")
+        f.write(f"Time all rays(sum) - triangle(sum): {_g(np.float32(ray_sum - tri_sum))}\n")
+        f.write(f"Average BVH depth: {_g(np.float32(avg_depth))}\n")
         f.write(f"Triangle Count: {tri_count}\n")
 
 
@@ -468,9 +480,7 @@ class RayTracer:
         bvh = Bvh(tris, branching, leaf_size, gang=self.gang, sort_each_split=cfg.get("sortEachSplit", True),
                   leaf_split_option=cfg.get("leafSplitOption", 1), sah_factor=sah)
         scene = Scene(bvh)
-        # ambientDistance = cbrt(root volume) / 10 (rayTracer.cpp:607)
-        lo, hi = tris.reshape(-1, 3).min(0), tris.reshape(-1, 3).max(0)
-        ambient = float(np.cbrt(np.prod((hi - lo).astype(np.float64))) / 10.0)
+        ambient = ambient_distance(tris)
         cam = CameraFast(positions, targets, cfg.get("xRes", 1920), cfg.get("yRes", 1088),
                          cfg.get("workGroupSize", 16))
         mode = "save_distance" if cfg.get("saveDistance") else "closest"
@@ -485,7 +495,11 @@ class RayTracer:
                                    max(1, cfg.get("ambientSampleCount", 1)), ambient, self.out_dir, name, problem,
                                    camera_id=c, wide_alternative=cfg.get("wideAlternative", True))
         prefix = f"_mb{bvh.node_memory}_ml{bvh.leaf_memory}"
-        write_perf(os.path.join(self.out_dir, f"{name}{problem}{prefix}_Perf.txt"), name, problem, self.gang,
+        # cameraFast.h:134-147: <path>/WorkGroupSize_<wg>_<Normal|Wide|WideOld>/
+        kind = ("Wide" if cfg.get("wideAlternative", True) else "WideOld") if cfg.get("wideRender") else "Normal"
+        perf_dir = os.path.join(self.out_dir, f"WorkGroupSize_{cam.wg}_{kind}")
+        os.makedirs(perf_dir, exist_ok=True)
+        write_perf(os.path.join(perf_dir, f"{name}{problem}{prefix}_Perf.txt"), name, problem, self.gang,
                    bvh.node_memory, bvh.leaf_memory, branching, leaf_size, total, primary, 0.0,
                    bvh.average_depth, bvh.triangle_count)
         scene.close()

@@ -69,3 +69,29 @@ def test_errors_are_reported_not_raised_across_the_abi(rt):
     h = ctypes.c_void_p()
     rc = lib.rt_upload_bvh(ctypes.byref(arr.desc), ctypes.byref(h))
     assert rc == -5 and b"leaf blocks" in lib.rt_last_error()
+
+
+def test_cpp_layer_compiles_and_links(tmp_path, rt):
+    """include/rt_renderer.hpp compiles on its own and its functions resolve
+    in librt.so (config parsing and the sweep run without a GPU)."""
+    src = tmp_path / "use.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <fstream>
+#include "rt_renderer.hpp"
+int main(int argc, char** argv) {
+  { std::ofstream f(argv[1]); f << "minLeafSize = 4\nmaxLeafSize = 8\nminBranch = 4\nmaxBranch = 8\n"; }
+  rt::RenderConfig c;
+  if (!rt::read_config(argv[1], c)) return 3;
+  for (auto& p : rt::sweep(c)) std::printf("%d,%d ", p.first, p.second);
+  std::printf("%d %zu\n", rt::ambient_gray(true, 1, 3), sizeof(rt_step_stat));
+  try { rt::tile_order_to_rows(std::vector<uint8_t>(5), 4, 1, 2); } catch (const rt::Error&) { std::printf("error\n"); }
+  return 0;
+}
+''')
+    exe = tmp_path / "use"
+    lib = os.path.join(ROOT, "bvhraytracer_amd", "librt.so")
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), lib,
+                    "-Wl,-rpath," + os.path.dirname(lib), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe), str(tmp_path / "config.txt")], capture_output=True, text=True, check=True)
+    assert out.stdout.split("\n")[:2] == ["4,4 4,8 8,4 8,8 169 20", "error"]

@@ -61,7 +61,8 @@ def test_raytracer_sweep_writes_reports(rt, tmp_path):
     assert set(res) == {("spheres", 4, 4), ("spheres", 4, 8)}
     files = sorted(os.listdir(out))
     assert "spheres_c0_Perf.png" in files
-    assert sum(f.endswith("_Perf.txt") for f in files) == 2
+    perf = sorted(os.listdir(out / "WorkGroupSize_16_Normal"))
+    assert perf == ["spheres_b4_l4_mb4_ml4_Perf.txt", "spheres_b4_l8_mb4_ml8_Perf.txt"]
     img = decode_png((out / "spheres_c0_Perf.png").read_bytes())
     assert img.shape == (48, 64, 4) and img[..., 0].max() > 0
 

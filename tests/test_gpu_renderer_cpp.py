@@ -1,0 +1,67 @@
+"""The C++ renderer (bvhrt: RayTracer + CameraFast over the C ABI) on the
+GPU: a config.txt performance run writes the reference's _Perf.txt files
+and the _Perf.png, and the image equals the Python driver's render of the
+same BVH pixel for pixel."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.test_image import decode_png
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BVHRT = os.path.join(ROOT, "bvhraytracer_amd", "bvhrt")
+
+CONFIG = """xRes = 64
+yRes = 48
+workGroupSize = 16
+minLeafSize = 4
+maxLeafSize = 4
+leafStep = 4
+minBranch = 4
+maxBranch = 8
+branchStep = 4
+ambientSampleCount = 2
+saveImage = true
+doPerformanceTest = true
+scenario = 0
+"""
+
+
+def test_bvhrt_performance_run(rt, tmp_path):
+    from bvhraytracer_amd.raytracer import CameraFast, ambient_distance, tile_order_to_rows
+
+    cfg = tmp_path / "config.txt"
+    cfg.write_text(CONFIG)
+    out = tmp_path / "perf"
+    print("bvhrt spheres 64x48 N4,N8 L4", flush=True)
+    r = subprocess.run([BVHRT, str(cfg), str(out)], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    assert len(lines) == 2 and lines[0].startswith("scenario 0 N4 L4") and lines[1].startswith("scenario 0 N8 L4")
+
+    tris = rt.spheres_scene(0)
+    perf_dir = out / "WorkGroupSize_16_Normal"
+    assert sorted(os.listdir(perf_dir)) == ["spheres_b4_l4_mb4_ml4_Perf.txt", "spheres_b8_l4_mb8_ml4_Perf.txt"]
+    for N in (4, 8):
+        bvh = rt.Bvh(tris, N, 4, sah_factor=rt.load_sah_factor(4, N, 4))
+        text = (perf_dir / f"spheres_b{N}_l4_mb{N}_ml4_Perf.txt").read_text().splitlines()
+        assert text[0] == f"scenario spheres with branching factor of {N} and leafsize of 4"
+        assert text[1] == f"with a gang size of 4, a node meory of {N}, and a leaf memory of 4"
+        assert text[6] == "Average BVH depth: %g" % np.float32(bvh.average_depth)
+        assert text[7] == "Triangle Count: 100002"
+        assert float(text[2].split(": ")[1]) > 0
+
+    # the PNG is written per (N, L) under the same name: the last one is N8 L4
+    got = decode_png((out / "spheres_c0_Perf.png").read_bytes())
+    bvh = rt.Bvh(tris, 8, 4, sah_factor=rt.load_sah_factor(4, 8, 4))
+    scene = rt.Scene(bvh)
+    cam = CameraFast([(0.0, 4.0, 32.0)], [(0.0, 0.0, 0.0)], 64, 48, 16)
+    img = cam.render_image(scene, bvh, 0, 2, ambient_distance(tris))[3]
+    want = tile_order_to_rows(img, 64, 48, 16)
+    assert 0 < (want > 0).sum() < want.size
+    assert np.array_equal(got[..., 0], want) and np.all(got[..., 3] == 255)
+    scene.close()
