@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <filesystem>
 #include <fstream>
 #include <iostream>
@@ -193,6 +194,288 @@ void FastNodeManager::getSurfaceNormalPosition(const std::vector<rt_ray>& rays, 
     positions.resize(rays.size() * 3);
     check(rt_hit_surface(host_, rays.data(), hits.data(), (uint32_t)rays.size(), normals.data(), positions.data()),
           "rt_hit_surface");
+}
+
+void FastNodeManager::histograms(const std::vector<rt_ray>& rays, bool anyHit, rt_trace_hist& hist,
+                                 std::vector<rt_hit>* hits) {
+    if (hits) hits->resize(rays.size());
+    if (rays.empty()) return;
+    rt_trace_hist h{};
+    std::vector<uint8_t> out(rays.size() * (anyHit ? 1 : sizeof(rt_hit)));
+    check(rt_trace_histograms(scene_, rays.data(), (uint32_t)rays.size(), anyHit ? RT_ANY : RT_CLOSEST, out.data(), &h,
+                              nullptr),
+          "rt_trace_histograms");
+    if (hits && !anyHit) std::memcpy(hits->data(), out.data(), out.size());
+    uint64_t* dst = reinterpret_cast<uint64_t*>(&hist);
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(&h);
+    for (size_t i = 0; i < sizeof(rt_trace_hist) / 8; ++i) dst[i] += src[i];
+}
+
+std::vector<std::vector<rt_step_stat>> FastNodeManager::workgroupSteps(const std::vector<rt_ray>& rays, unsigned tile,
+                                                                       bool anyHit, TileVisits* visits) {
+    if (tile == 0 || rays.size() % tile != 0) throw Error("workgroupSteps: the ray count must be a multiple of tile");
+    const size_t tiles = rays.size() / tile;
+    std::vector<rt_counts> cnt(rays.size());
+    std::vector<uint32_t> lens(std::max<size_t>(1, tiles));
+    uint32_t cap = 256;
+    std::vector<rt_step_stat> rec;
+    for (;;) {
+        rec.assign(std::max<size_t>(1, tiles * cap), rt_step_stat{});
+        check(rt_trace_steps(scene_, rays.data(), (uint32_t)rays.size(), anyHit ? RT_ANY : RT_CLOSEST, tile, cap,
+                             rec.data(), lens.data(), cnt.data(), nullptr),
+              "rt_trace_steps");
+        uint32_t need = 0;
+        for (size_t i = 0; i < tiles; ++i) need = std::max(need, lens[i]);
+        if (need <= cap) break;
+        cap = need;
+    }
+    std::vector<std::vector<rt_step_stat>> out(tiles);
+    for (size_t i = 0; i < tiles; ++i) out[i].assign(rec.begin() + i * cap, rec.begin() + i * cap + lens[i]);
+    if (visits) {
+        visits->nodes.resize(rays.size());
+        visits->leaves.resize(rays.size());
+        for (size_t r = 0; r < rays.size(); ++r) {
+            visits->nodes[r] = cnt[r].nodes;
+            visits->leaves[r] = cnt[r].leaves;
+        }
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+
+namespace {
+
+void info_block(std::ofstream& f, const char* label, const rt_trace_hist& h, unsigned branching) {
+    const float factor = 1 / (float)h.rays;
+    f << label << " intersections node: " << h.node_visits * factor << std::endl;
+    f << label << " aabb intersections: " << h.aabb_tests * factor << std::endl;
+    f << label << " aabb success ration: " << h.aabb_hits / (float)h.aabb_tests << std::endl;
+    const float waste = (h.leaf_visits + h.node_visits - h.rays) / (float)(h.node_visits * branching);
+    f << label << " waste factor: " << 1 - waste << std::endl;
+    f << label << " intersections leaf: " << h.leaf_visits * factor << std::endl;
+    f << label << " primitive intersections: " << h.tri_tests * factor << std::endl;
+    f << label << " primitive success ratio: " << h.tri_hits / (float)h.tri_tests << std::endl;
+}
+
+void fullness(std::ofstream& f, const char* title, const char* avgKey, const uint64_t* counts, unsigned size,
+              float factor) {
+    std::vector<uint64_t> c(size, 0);
+    for (unsigned i = 0; i < size && i < RT_HIST_FULL; ++i) c[i] = counts[i];
+    float sum = 0;
+    for (size_t i = 0; i < c.size(); ++i) sum += c[i] * i;
+    int total = 0;  // std::accumulate(..., 0) sums into an int
+    for (uint64_t v : c) total = (int)(total + v);
+    sum /= total;
+    f << title << std::endl;
+    f << avgKey << std::to_string(sum) << std::endl;
+    for (size_t i = 0; i < c.size(); ++i) f << i << " : " << c[i] * factor << std::endl;
+}
+
+}  // namespace
+
+void write_info(const std::string& path, const std::string& name, unsigned branching, unsigned leafSize,
+                unsigned bvhDepth, const rt_trace_hist& p, const rt_trace_hist& s) {
+    std::ofstream f(path);
+    if (!f.is_open()) throw Error("write_info: cannot open " + path);
+    f << "scenario " << name << " with branching factor of " << branching << " and leafsize of " << leafSize
+      << std::endl;
+    f << "intersections counts are normalized per Ray" << std::endl << std::endl;
+    info_block(f, "primary", p, branching);
+    f << std::endl;
+    info_block(f, "secondary", s, branching);
+    f << std::endl;
+    const float factor = 1 / (float)p.rays;
+    fullness(f, "intersections with nodes with x children :", "average child fullness: ", p.child_fullness,
+             branching + 1, factor);
+    f << std::endl;
+    fullness(f, "intersections with leaf nodes with x primitives :", "average leaf fullness: ", p.leaf_fullness,
+             leafSize + 1, factor);
+    const std::pair<const char*, const uint64_t*> sections[] = {{"primary node", p.node_depth},
+                                                                {"primary leaf", p.leaf_depth},
+                                                                {"secondary node", s.node_depth},
+                                                                {"secondary leaf", s.leaf_depth}};
+    for (int k = 0; k < 4; ++k) {
+        const float fac = 1 / (float)(k < 2 ? p.rays : s.rays);
+        f << std::endl;
+        f << sections[k].first << " intersections at depth x :" << std::endl;
+        for (unsigned i = 0; i < bvhDepth; ++i)
+            f << i << " : " << (i < RT_HIST_DEPTHS ? sections[k].second[i] : 0ull) * fac << std::endl;
+    }
+}
+
+namespace {
+
+struct Whisker {
+    int median, min, max;
+    float avg, sd;
+};
+
+// cameraData.h:703-779
+std::vector<Whisker> whisker_rows(const std::vector<std::vector<rt_step_stat>>& tiles, unsigned wgsq) {
+    const int medianId = (int)(wgsq * 0.5f);
+    std::vector<Whisker> rows;
+    for (const auto& t : tiles) {
+        std::vector<int> depth;
+        int workSum = 0;
+        for (size_t tId = 0; tId < t.size(); ++tId) {
+            workSum += (int)tId * (int)t[tId].terminations;
+            for (uint32_t j = 0; j < t[tId].terminations; ++j) depth.push_back((int)tId);
+        }
+        if (depth.size() < wgsq) throw Error("workgroup analysis: a tile has fewer terminations than rays");
+        const float avg = workSum / (float)wgsq;
+        float variance = 0;
+        for (unsigned j = 0; j < wgsq; ++j) variance += std::pow(depth[j] - avg, 2);
+        variance /= (int)wgsq;
+        rows.push_back({depth[medianId], depth[0], depth[wgsq - 1], avg, std::sqrt(variance)});
+    }
+    std::stable_sort(rows.begin(), rows.end(), [](const Whisker& a, const Whisker& b) {
+        return a.median == b.median ? a.max < b.max : a.median < b.median;
+    });
+    return rows;
+}
+
+struct StepAvg {
+    float nodeWork = 0, nodeUnique = 0, leafWork = 0, leafUnique = 0, terminated = 0;
+    uint32_t nuMax = 0, nuMin = 0, luMax = 0, luMin = 0;
+};
+
+// one half of a _WorkGroupData.txt row (cameraData.h:846-949)
+StepAvg step_average(const std::vector<std::vector<rt_step_stat>>& tiles, size_t j, unsigned wgsq, float tilesF) {
+    StepAvg a;
+    int count = 0, nuCount = 0, luCount = 0;
+    uint32_t nuMax = 0, nuMin = wgsq + 1, luMax = 0, luMin = wgsq + 1;
+    for (const auto& t : tiles) {
+        if (t.size() <= j) continue;
+        ++count;
+        a.nodeWork += t[j].node_work;
+        a.leafWork += t[j].leaf_work;
+        a.nodeUnique += t[j].node_unique;
+        a.leafUnique += t[j].leaf_unique;
+        a.terminated += t[j].terminations;
+        if (t[j].node_unique != 0) {
+            ++nuCount;
+            nuMax = std::max(t[j].node_unique, nuMax);
+            nuMin = std::min(t[j].node_unique, nuMin);
+        }
+        if (t[j].leaf_unique != 0) {
+            ++luCount;
+            luMax = std::max(t[j].leaf_unique, luMax);
+            luMin = std::min(t[j].leaf_unique, luMin);
+        }
+    }
+    count = std::max(count, 1);
+    nuCount = std::max(nuCount, 1);
+    luCount = std::max(luCount, 1);
+    a.nuMax = nuMax;
+    a.nuMin = nuMin == wgsq + 1 ? 0 : nuMin;
+    a.luMax = luMax;
+    a.luMin = luMin == wgsq + 1 ? 0 : luMin;
+    a.nodeWork /= (float)count;
+    a.leafWork /= (float)count;
+    a.nodeUnique /= (float)nuCount;
+    a.leafUnique /= (float)luCount;
+    a.terminated /= tilesF;
+    return a;
+}
+
+}  // namespace
+
+std::vector<std::string> write_workgroup_analysis(const std::string& path, const std::string& name,
+                                                  const std::string& problem, int cameraId, unsigned width,
+                                                  unsigned height, unsigned wg,
+                                                  const std::vector<std::vector<rt_step_stat>>& primary,
+                                                  const std::vector<std::vector<rt_step_stat>>& secondary,
+                                                  const TileVisits& pv, const TileVisits& sv, bool wideAlternative) {
+    const unsigned wgsq = wg * wg;
+    const size_t tiles = (size_t)(width / wg) * (height / wg);
+    if (primary.size() != tiles || secondary.size() != tiles)
+        throw Error("write_workgroup_analysis: expected " + std::to_string(tiles) + " tiles");
+    if (pv.nodes.size() < tiles * wgsq || pv.leaves.size() < tiles * wgsq || sv.nodes.size() < tiles * wgsq ||
+        sv.leaves.size() < tiles * wgsq)
+        throw Error("write_workgroup_analysis: one visit count per ray of the full tiles");
+    const std::string dir = path + "/WorkGroupSize_" + std::to_string(wg) + "_Version_" + (wideAlternative ? "1" : "0");
+    std::error_code ec;
+    std::filesystem::create_directories(dir, ec);
+    const std::string stem = dir + "/" + name + problem + "_c" + std::to_string(cameraId);
+    const std::vector<std::string> files = {stem + "_PrimaryWorkGroupWiskerPlot.txt",
+                                            stem + "_SecondaryWorkGroupWiskerPlot.txt", stem + "_WorkGroupData.txt",
+                                            stem + "_WorkGroupUniqueWork.txt"};
+    auto open = [](const std::string& fn) {
+        std::ofstream f(fn);
+        if (!f.is_open()) throw Error("write_workgroup_analysis: cannot open " + fn);
+        return f;
+    };
+    for (int k = 0; k < 2; ++k) {
+        std::ofstream f = open(files[k]);
+        f << "median, min, max, lowerStdDeviation, upperStdDeviation" << std::endl;
+        for (const Whisker& w : whisker_rows(k == 0 ? primary : secondary, wgsq))
+            f << w.median << ", " << w.min << ", " << w.max << ", " << w.avg << ", " << w.sd << std::endl;
+    }
+    {
+        // the reference's size loop tests the primary size twice, so the steps
+        // are those of the longest primary tile
+        size_t maxSize = 0;
+        for (const auto& t : primary) maxSize = std::max(maxSize, t.size());
+        const float tilesF = (float)(width / wg) * (height / wg);
+        std::ofstream f = open(files[2]);
+        f << "stepId, avgPrimaryNodeWork, avgPrimaryNodeUnique, avgPrimaryLeafWork, avgPrimaryLeafUnique, "
+             "avgPrimaryRayTermination, primaryNodeWorkMax, primaryNodeWorkMin, primaryLeafWorkMax, "
+             "primaryLeafWorkMin, avgSecondaryNodeWork, avgSecondaryNodeUnique, avgSecondaryLeafWork, "
+             "avgSecondaryLeafUnique, avgSecondaryRayTermination, secondaryNodeWorkMax, secondaryNodeWorkMin, "
+             "secondaryLeafWorkMax, secondaryLeafWorkMin"
+          << std::endl;
+        float cumP = 0, cumS = 0;
+        for (size_t j = 0; j < maxSize; ++j) {
+            StepAvg p = step_average(primary, j, wgsq, tilesF), s = step_average(secondary, j, wgsq, tilesF);
+            if (j != 0) {
+                p.terminated += cumP;
+                s.terminated += cumS;
+            }
+            cumP = p.terminated;
+            cumS = s.terminated;
+            f << j << ", " << p.nodeWork << ", " << p.nodeUnique << ", " << p.leafWork << ", " << p.leafUnique << ", "
+              << p.terminated << ", " << p.nuMax << ", " << p.nuMin << ", " << p.luMax << ", " << p.luMin << ", "
+              << s.nodeWork << ", " << s.nodeUnique << ", " << s.leafWork << ", " << s.leafUnique << ", "
+              << s.terminated << ", " << s.nuMax << ", " << s.nuMin << ", " << s.luMax << ", " << s.luMin << std::endl;
+        }
+    }
+    {
+        std::ofstream f = open(files[3]);
+        f << "loadedPrimaryNodes, loadedPrimaryLeafs, loadedPrimaryNodesMax, loadedPrimaryLeafsMax, "
+             "loadedPrimaryNodesMin, loadedPrimaryLeafsMin, loadedSecondaryNodes, loadedSecondaryLeafs, "
+             "loadedSecondaryNodesMax, loadedSecondaryLeafsMax, loadedSecondaryNodesMin, loadedSecondaryLeafsMin, "
+             "loadedWidePrimaryNodes, loadedWidePrimaryLeafs, loadedWideSecondaryNodes, loadedWideSecondaryLeafs"
+          << std::endl;
+        for (size_t i = 0; i < tiles; ++i) {
+            float sums[4] = {0, 0, 0, 0};
+            int mx[4] = {0, 0, 0, 0}, mn[4] = {1000, 1000, 1000, 1000};
+            const std::vector<uint32_t>* v[4] = {&pv.nodes, &pv.leaves, &sv.nodes, &sv.leaves};
+            for (unsigned j = 0; j < wgsq; ++j)
+                for (int k = 0; k < 4; ++k) {
+                    const int x = (int)(*v[k])[i * wgsq + j];
+                    sums[k] += x;
+                    mx[k] = std::max(x, mx[k]);
+                    mn[k] = std::min(x, mn[k]);
+                }
+            for (int k = 0; k < 4; ++k)
+                if (mn[k] == 1000) mn[k] = 0;
+            int wide[4] = {0, 0, 0, 0};
+            for (const auto& r : primary[i]) {
+                wide[0] += (int)r.node_unique;
+                wide[1] += (int)r.leaf_unique;
+            }
+            for (const auto& r : secondary[i]) {
+                wide[2] += (int)r.node_unique;
+                wide[3] += (int)r.leaf_unique;
+            }
+            f << sums[0] << ", " << sums[1] << ", " << mx[0] << ", " << mx[1] << ", " << mn[0] << ", " << mn[1] << ", "
+              << sums[2] << ", " << sums[3] << ", " << mx[2] << ", " << mx[3] << ", " << mn[2] << ", " << mn[3] << ", "
+              << (float)wide[0] << ", " << (float)wide[1] << ", " << (float)wide[2] << ", " << (float)wide[3]
+              << std::endl;
+        }
+    }
+    return files;
 }
 
 // ---------------------------------------------------------------------------
@@ -382,6 +665,85 @@ std::pair<float, float> CameraFast::renderImages(bool saveImage, FastNodeManager
 
 // ---------------------------------------------------------------------------
 
+CameraData::CameraData(std::string path, std::string name, std::string problem, int workGroupSize, bool wideRender,
+                       std::vector<Vec3> positions, std::vector<Vec3> lookCenters, unsigned width, unsigned height,
+                       Vec3 upward, float focalLength)
+    : path_(std::move(path)), name_(std::move(name)), problem_(std::move(problem)), wg_(workGroupSize),
+      wideRender_(wideRender), positions_(std::move(positions)), targets_(std::move(lookCenters)), width_(width),
+      height_(height), up_(upward), focal_(focalLength) {
+    if (positions_.size() != targets_.size()) throw Error("CameraData: one look center per camera position");
+    if (wg_ <= 0 || width_ % (unsigned)wg_ != 0) throw Error("CameraData: width must be a multiple of workGroupSize");
+}
+
+std::string CameraData::renderImages(FastNodeManager& m, unsigned ambientSampleCount, float ambientDistance,
+                                     bool doWorkGroupAnalysis, bool wideAlternative) {
+    if (positions_.empty()) throw Error("CameraData: no camera");
+    std::error_code ec;
+    std::filesystem::create_directories(path_, ec);
+    rt_trace_hist hp{}, hs{};
+    const size_t n = (size_t)width_ * height_;
+    const unsigned wgsq = (unsigned)(wg_ * wg_);
+    const size_t tiles = (size_t)(width_ / wg_) * (height_ / wg_);
+    for (size_t c = 0; c < positions_.size(); ++c) {
+        rt_camera cam{};
+        for (int a = 0; a < 3; ++a) {
+            cam.pos[a] = positions_[c][a];
+            cam.target[a] = targets_[c][a];
+            cam.up[a] = up_[a];
+        }
+        cam.focal = focal_;
+        std::vector<rt_ray> rays(n);
+        check(rt_camera_rays(&cam, width_, height_, (uint32_t)wg_, rays.data()), "rt_camera_rays");
+        std::vector<rt_hit> hits;
+        m.histograms(rays, false, hp, &hits);
+        std::vector<float> nrm, pos;
+        m.getSurfaceNormalPosition(rays, hits, nrm, pos);
+        std::vector<rt_ray> amb(n);
+        std::vector<uint32_t> src(n);
+        for (unsigned a = 0; a < ambientSampleCount; ++a) {
+            uint32_t k = 0;
+            check(rt_spawn_ambient(nrm.data(), pos.data(), hits.data(), (uint32_t)n, a, ambientDistance, amb.data(),
+                                   src.data(), &k),
+                  "rt_spawn_ambient");
+            m.histograms(std::vector<rt_ray>(amb.begin(), amb.begin() + k), true, hs);
+        }
+        if (wideRender_ && doWorkGroupAnalysis) {
+            if (ambientSampleCount == 0) throw Error("CameraData: the workgroup analysis needs ambient samples");
+            const std::vector<rt_ray> full(rays.begin(), rays.begin() + tiles * wgsq);
+            TileVisits pv, sv;
+            const auto primary = m.workgroupSteps(full, wgsq, false, &pv);
+            std::vector<std::vector<rt_step_stat>> secondary(tiles);
+            sv.nodes.assign(full.size(), 0);
+            sv.leaves.assign(full.size(), 0);
+            rt_ray nanRay;
+            std::fill_n(reinterpret_cast<float*>(&nanRay), 10, std::nanf(""));
+            for (unsigned a = 0; a < ambientSampleCount; ++a) {
+                // a pixel without a hit gets a NaN ray (cameraData.h:495-499)
+                uint32_t k = 0;
+                check(rt_spawn_ambient(nrm.data(), pos.data(), hits.data(), (uint32_t)full.size(), a, ambientDistance,
+                                       amb.data(), src.data(), &k),
+                      "rt_spawn_ambient");
+                std::vector<rt_ray> dense(full.size(), nanRay);
+                for (uint32_t i = 0; i < k; ++i) dense[src[i]] = amb[i];
+                TileVisits v;
+                const auto recs = m.workgroupSteps(dense, wgsq, true, &v);
+                for (size_t t = 0; t < tiles; ++t) secondary[t].insert(secondary[t].end(), recs[t].begin(), recs[t].end());
+                for (size_t r = 0; r < full.size(); ++r) {
+                    sv.nodes[r] += v.nodes[r];
+                    sv.leaves[r] += v.leaves[r];
+                }
+            }
+            write_workgroup_analysis(path_, name_, problem_, (int)c, width_, height_, (unsigned)wg_, primary, secondary,
+                                     pv, sv, wideAlternative);
+        }
+    }
+    const std::string info = path_ + "/" + name_ + problem_ + "_Info.txt";
+    write_info(info, name_, m.branchingFactor, m.leafSize, m.desc().max_depth + 1, hp, hs);
+    return info;
+}
+
+// ---------------------------------------------------------------------------
+
 std::vector<float> scenario_triangles(int scenario, std::string& name, Vec3& pos, Vec3& target) {
     float* tris = nullptr;
     uint32_t n = 0;
@@ -426,16 +788,26 @@ float RayTracer::renderImage(unsigned branchingFactor, unsigned leafSize, unsign
     FastNodeManager m(tris, branchingFactor, leafSize, gangSize, sahFactor, cfg_.sortEachSplit, cfg_.leafSplitOption);
     const float ambientDistance = ambient_distance(tris);
     const std::string prefix = "_mb" + std::to_string(m.nodeMemory) + "_ml" + std::to_string(m.leafMemory);
-    CameraFast cam(path_, name, problem, prefix, cfg_.workGroupSize, cfg_.saveDistance, cfg_.wideRender,
-                   cameraPositions, cameraTargets, (unsigned)cfg_.xRes, (unsigned)cfg_.yRes);
-    return cam.renderImages(cfg_.saveImage, m, (unsigned)std::max(0, cfg_.ambientSampleCount), ambientDistance, true,
-                            cfg_.wideAlternative)
-        .first;
+    float total = 0;
+    if (cfg_.doPerformanceTest) {
+        CameraFast cam(path_, name, problem, prefix, cfg_.workGroupSize, cfg_.saveDistance, cfg_.wideRender,
+                       cameraPositions, cameraTargets, (unsigned)cfg_.xRes, (unsigned)cfg_.yRes);
+        total = cam.renderImages(cfg_.saveImage, m, (unsigned)std::max(0, cfg_.ambientSampleCount), ambientDistance,
+                                 true, cfg_.wideAlternative)
+                    .first;
+    }
+    if (cfg_.renderAnalysisImage) {
+        CameraData cam(path_, name, problem, cfg_.workGroupSize, cfg_.wideRender, cameraPositions, cameraTargets,
+                       (unsigned)cfg_.xRes, (unsigned)cfg_.yRes);
+        cam.renderImages(m, (unsigned)std::max(0, cfg_.ambientSampleCount), ambientDistance, cfg_.doWorkGroupAnalysis,
+                         cfg_.wideAlternative);
+    }
+    return total;
 }
 
 std::vector<std::tuple<int, int, int, float>> RayTracer::run() {
     std::vector<std::tuple<int, int, int, float>> out;
-    if (!cfg_.doPerformanceTest) return out;
+    if (!cfg_.doPerformanceTest && !cfg_.renderAnalysisImage) return out;
     std::vector<int> scenarios = cfg_.scenarios;
     if (scenarios.empty()) scenarios.push_back(0);
     for (int sc : scenarios) {

@@ -7,6 +7,7 @@
   write_png          the _Perf.png of cameraFast.h:829-852 (tile order -> rows, RGBA PNG)
   workgroup_analysis the wide render's per-step tile statistics and the four
                      WorkGroup*.txt files of cameraData.h:458-560,660-1109
+  analysis_render    CameraData's analysis render: _Info.txt (+ workgroup files)
 
 Scenes are procedural (the reference loads glTF models that are not part of
 the repository); everything else keeps the reference's semantics: substring
@@ -240,55 +241,80 @@ def write_perf(path, name, problem, gang, node_memory, leaf_memory, branching, l
         f.write(f"Triangle Count: {tri_count}\n")
 
 
-def _info_block(f, label, st):
-    rays = max(1, st["rays"])
-    f.write(f"{label} intersections node: {st['node_visits'] / rays}\n")
-    f.write(f"{label} aabb intersections: {st['aabb_tests'] / rays}\n")
-    f.write(f"{label} aabb success ration: {st['aabb_hits'] / max(1, st['aabb_tests'])}\n")
-    waste = (st["leaf_visits"] + st["node_visits"] - st["rays"]) / max(1, st["node_visits"] * st["branching"])
-    f.write(f"{label} waste factor: {1 - waste}\n")
-    f.write(f"{label} intersections leaf: {st['leaf_visits'] / rays}\n")
-    f.write(f"{label} primitive intersections: {st['tri_tests'] / rays}\n")
-    f.write(f"{label} primitive success ratio: {st['tri_hits'] / max(1, st['tri_tests'])}\n")
+def _info_block(f, label, st, branching):
+    """One counter block of _Info.txt (cameraData.h:247-265), float32 as
+    there: factor = 1 / (float)rays, uint64 counts converted to float."""
+    f32 = np.float32
+    factor = f32(1) / f32(st["rays"])
+
+    def per_ray(v):
+        return _g(f32(f32(int(v)) * factor))
+
+    def ratio(a, b):
+        return _g(f32(f32(int(a)) / f32(int(b))))
+
+    f.write(f"{label} intersections node: {per_ray(st['node_visits'])}\n")
+    f.write(f"{label} aabb intersections: {per_ray(st['aabb_tests'])}\n")
+    f.write(f"{label} aabb success ration: {ratio(st['aabb_hits'], st['aabb_tests'])}\n")
+    waste = f32(f32(int(st["leaf_visits"] + st["node_visits"] - st["rays"])) / f32(int(st["node_visits"]) * branching))
+    f.write(f"{label} waste factor: {_g(f32(1) - waste)}\n")
+    f.write(f"{label} intersections leaf: {per_ray(st['leaf_visits'])}\n")
+    f.write(f"{label} primitive intersections: {per_ray(st['tri_tests'])}\n")
+    f.write(f"{label} primitive success ratio: {ratio(st['tri_hits'], st['tri_tests'])}\n")
 
 
-def _trim(h):
-    nz = np.nonzero(h)[0]
-    return h[: nz[-1] + 1] if len(nz) else h[:1]
+def _fullness(f, title, avg_key, counts, size, factor):
+    """A fullness section (cameraData.h:270-292): the average is a float sum
+    of count * i over the counts' int total, written with std::to_string."""
+    f32 = np.float32
+    c = [int(v) for v in counts[:size]] + [0] * max(0, size - len(counts))
+    acc = f32(0)
+    for i, v in enumerate(c):
+        acc = f32(acc + f32(v * i))
+    total = sum(c) & 0xFFFFFFFF  # std::accumulate(..., 0) sums into an int
+    total = total - (1 << 32) if total >= 1 << 31 else total
+    acc = f32(acc / f32(total))
+    f.write(f"{title}\n")
+    f.write("%s%f\n" % (avg_key, float(acc)))
+    for i, v in enumerate(c):
+        f.write(f"{i} : {_g(f32(f32(v) * factor))}\n")
 
 
-def write_info(path, name, branching, leaf_size, primary, secondary):
+def write_info(path, name, branching, leaf_size, primary, secondary, bvh_depth=None):
     """The _Info.txt of the reference's instrumented renderer
-    (cameraData.h:241-329), from rt_trace_histograms counters of a primary
-    and a secondary (shadow) batch: per-ray averages, success ratios, the
-    waste factor, child/leaf fullness and visits per depth, with the same
-    line keys, so the reference's Analysis/DataManager.py reads it."""
-    p = dict(primary, branching=branching)
-    s = dict(secondary, branching=branching)
+    (cameraData.h:241-329), from rt_trace_histograms counters of the primary
+    rays and of the secondary (ambient) rays: per-ray averages, success
+    ratios, the waste factor, child fullness 0..N and leaf fullness 0..L of
+    the primary rays, and visits per depth 0..bvh_depth-1 (bvhDepth, the
+    tree depth + 1), in float32 with the reference's line keys and number
+    formats, so its Analysis/DataManager.py reads it.  Every ray, test and
+    visit count must be nonzero (the reference divides by them)."""
+    f32 = np.float32
+    if bvh_depth is None:
+        nz = np.nonzero(np.asarray(primary["node_depth"]) + np.asarray(primary["leaf_depth"]))[0]
+        bvh_depth = int(nz[-1]) + 1 if len(nz) else 1
     with open(path, "w") as f:
         f.write(f"scenario {name} with branching factor of {branching} and leafsize of {leaf_size}\n")
         f.write("intersections counts are normalized per Ray\n\n")
-        _info_block(f, "primary", p)
+        _info_block(f, "primary", primary, branching)
         f.write("\n")
-        _info_block(f, "secondary", s)
+        _info_block(f, "secondary", secondary, branching)
         f.write("\n")
-        f.write("intersections with nodes with x children :\n")
-        cf = _trim(p["child_fullness"])
-        f.write(f"average child fullness: {float((cf * np.arange(len(cf))).sum() / max(1, cf.sum())):.6f}\n")
-        for i, v in enumerate(cf):
-            f.write(f"{i} : {v / max(1, p['rays'])}\n")
+        factor = f32(1) / f32(primary["rays"])
+        _fullness(f, "intersections with nodes with x children :", "average child fullness: ",
+                  primary["child_fullness"], branching + 1, factor)
         f.write("\n")
-        f.write("intersections with leaf nodes with x primitives :\n")
-        lf = _trim(p["leaf_fullness"])
-        f.write(f"average leaf fullness: {float((lf * np.arange(len(lf))).sum() / max(1, lf.sum())):.6f}\n")
-        for i, v in enumerate(lf):
-            f.write(f"{i} : {v / max(1, p['rays'])}\n")
-        for label, st, key in (("primary node", p, "node_depth"), ("primary leaf", p, "leaf_depth"),
-                               ("secondary node", s, "node_depth"), ("secondary leaf", s, "leaf_depth")):
+        _fullness(f, "intersections with leaf nodes with x primitives :", "average leaf fullness: ",
+                  primary["leaf_fullness"], leaf_size + 1, factor)
+        for label, st, key in (("primary node", primary, "node_depth"), ("primary leaf", primary, "leaf_depth"),
+                               ("secondary node", secondary, "node_depth"),
+                               ("secondary leaf", secondary, "leaf_depth")):
+            fac = f32(1) / f32(st["rays"])
             f.write("\n")
             f.write(f"{label} intersections at depth x :\n")
-            for i, v in enumerate(_trim(st[key])):
-                f.write(f"{i} : {v / max(1, st['rays'])}\n")
+            for i in range(bvh_depth):
+                v = int(st[key][i]) if i < len(st[key]) else 0
+                f.write(f"{i} : {_g(f32(f32(v) * fac))}\n")
 
 
 def _g(x):
@@ -459,6 +485,41 @@ def workgroup_analysis(scene, bvh, pos, target, width, height, wg, ambient_sampl
     return primary, secondary, files
 
 
+def _hist_add(a, b):
+    if a is None:
+        return {k: (np.array(v, np.int64) if hasattr(v, "__len__") else int(v)) for k, v in b.items()}
+    for k, v in b.items():
+        a[k] = a[k] + (np.asarray(v, np.int64) if hasattr(v, "__len__") else int(v))
+    return a
+
+
+def analysis_render(scene, bvh, positions, targets, width, height, wg, ambient_samples, ambient_distance, path,
+                    name, problem, wide_render=False, workgroup_analysis_on=False, wide_alternative=True):
+    """CameraData's analysis renderer (cameraData.h:84-560) on the GPU: per
+    camera the primary rays (closest hit) and ambient_samples ambient rays per
+    hit through the instrumented traversal (rt_trace_histograms), their
+    counters summed over the cameras into <path>/<name><problem>_Info.txt;
+    with wide_render and workgroup_analysis_on the workgroup files per camera.
+    Returns the _Info.txt path."""
+    os.makedirs(path, exist_ok=True)
+    hp = hs = None
+    for c in range(len(positions)):
+        rays = camera_rays(positions[c], targets[c], width, height, wg)
+        hits, st = scene.histograms(rays)
+        hp = _hist_add(hp, st)
+        nrm, pos = bvh.surface(rays, hits)
+        for a in range(ambient_samples):
+            amb, _ = spawn_ambient(nrm, pos, hits, a, ambient_distance)
+            if len(amb):
+                hs = _hist_add(hs, scene.histograms(amb, mode="any")[1])
+        if wide_render and workgroup_analysis_on:
+            workgroup_analysis(scene, bvh, positions[c], targets[c], width, height, wg, ambient_samples,
+                               ambient_distance, path, name, problem, camera_id=c, wide_alternative=wide_alternative)
+    info = os.path.join(path, f"{name}{problem}_Info.txt")
+    write_info(info, name, bvh.branching, bvh.leaf_size, hp, hs, bvh_depth=bvh.max_depth + 1)
+    return info
+
+
 SCENES = {
     "spheres": dict(make=lambda: spheres_scene(0), cam=((0.0, 4.0, 32.0), (0.0, 0.0, 0.0))),
     "interior": dict(make=lambda: interior_scene(2, 1_000_000), cam=((0.0, 5.0, -14.0), (0.0, 3.0, 0.0))),
@@ -483,17 +544,22 @@ class RayTracer:
         ambient = ambient_distance(tris)
         cam = CameraFast(positions, targets, cfg.get("xRes", 1920), cfg.get("yRes", 1088),
                          cfg.get("workGroupSize", 16))
-        mode = "save_distance" if cfg.get("saveDistance") else "closest"
-        total, primary = cam.render_images(scene, bvh, cfg.get("ambientSampleCount", 1), ambient, mode=mode,
-                                           save_image=cfg.get("saveImage", False), path=self.out_dir, name=name)
         os.makedirs(self.out_dir, exist_ok=True)
         problem = f"_b{branching}_l{leaf_size}"
-        if cfg.get("doWorkGroupAnalysis") and cfg.get("wideRender"):
-            # cameraData.h:121-133: the analysis needs the wide renderer
-            for c in range(len(positions)):
-                workgroup_analysis(scene, bvh, positions[c], targets[c], cam.width, cam.height, cam.wg,
-                                   max(1, cfg.get("ambientSampleCount", 1)), ambient, self.out_dir, name, problem,
-                                   camera_id=c, wide_alternative=cfg.get("wideAlternative", True))
+        samples = max(0, cfg.get("ambientSampleCount", 1))
+        if cfg.get("renderAnalysisImage"):
+            # rayTracer.cpp:695-742; the workgroup analysis needs the wide
+            # renderer (cameraData.h:121-133)
+            analysis_render(scene, bvh, positions, targets, cam.width, cam.height, cam.wg, samples, ambient,
+                            self.out_dir, name, problem, wide_render=cfg.get("wideRender", False),
+                            workgroup_analysis_on=cfg.get("doWorkGroupAnalysis", False),
+                            wide_alternative=cfg.get("wideAlternative", True))
+        if not cfg.get("doPerformanceTest", True):
+            scene.close()
+            return 0.0
+        mode = "save_distance" if cfg.get("saveDistance") else "closest"
+        total, primary = cam.render_images(scene, bvh, samples, ambient, mode=mode,
+                                           save_image=cfg.get("saveImage", False), path=self.out_dir, name=name)
         prefix = f"_mb{bvh.node_memory}_ml{bvh.leaf_memory}"
         # cameraFast.h:134-147: <path>/WorkGroupSize_<wg>_<Normal|Wide|WideOld>/
         kind = ("Wide" if cfg.get("wideAlternative", True) else "WideOld") if cfg.get("wideRender") else "Normal"

@@ -63,6 +63,12 @@ std::vector<std::pair<int, int>> sweep(const RenderConfig& cfg);
 // sahFactorConfigSse.txt values.
 float load_sah_factor(int leafSize, int branchingFactor, int gangSize, const std::string& table = "");
 
+// Visit counts per ray in tile order (rt_trace_steps' counts, summed over the
+// ambient samples for the secondary rays).
+struct TileVisits {
+    std::vector<uint32_t> nodes, leaves;
+};
+
 // The flattened BVH on the GPU: FastNodeManager<gang, nodeMemory, wg>'s
 // traversals (fastNodeManager.h:835-1493) over ray batches.
 class FastNodeManager {
@@ -86,6 +92,14 @@ class FastNodeManager {
     // getSurfaceNormalPosition (:1699-1726): 3 floats per ray each, NaN for a miss.
     void getSurfaceNormalPosition(const std::vector<rt_ray>& rays, const std::vector<rt_hit>& hits,
                                   std::vector<float>& normals, std::vector<float>& positions) const;
+    // The instrumented traversal (rt_trace_histograms): adds this batch's
+    // counters to *hist (closest hit, or any hit).
+    void histograms(const std::vector<rt_ray>& rays, bool anyHit, rt_trace_hist& hist,
+                    std::vector<rt_hit>* hits = nullptr);
+    // Per-step workload of tiles of `tile` rays (rt_trace_steps), one record
+    // vector per tile; visits receives each ray's node / leaf visits.
+    std::vector<std::vector<rt_step_stat>> workgroupSteps(const std::vector<rt_ray>& rays, unsigned tile, bool anyHit,
+                                                          TileVisits* visits = nullptr);
 
     unsigned branchingFactor, leafSize, gangSize, nodeMemory, leafMemory;
     float averageBvhDepth;
@@ -98,6 +112,29 @@ class FastNodeManager {
     rt_scene_t scene_ = nullptr;
     rt_bvh_desc desc_{};
 };
+
+// _Info.txt of the instrumented renderer (cameraData.h:241-329) from the
+// rt_trace_histograms counters of the primary and the secondary (ambient)
+// rays: per-ray averages, success ratios, waste factor, child fullness
+// 0..N and leaf fullness 0..L of the primary rays, visits per depth
+// 0..bvhDepth-1.  Counts must be nonzero (the reference divides by them).
+void write_info(const std::string& path, const std::string& name, unsigned branchingFactor, unsigned leafSize,
+                unsigned bvhDepth, const rt_trace_hist& primary, const rt_trace_hist& secondary);
+
+// The workgroup analysis files of CameraData::workGroupAnalysisPerImage
+// (cameraData.h:660-1109) under <path>/WorkGroupSize_<wg>_Version_<0|1>/:
+// _PrimaryWorkGroupWiskerPlot.txt, _SecondaryWorkGroupWiskerPlot.txt,
+// _WorkGroupData.txt, _WorkGroupUniqueWork.txt.  primary / secondary: the
+// rt_trace_steps records of every full tile (secondary: every ambient
+// sample's records back to back).  Returns the four paths.  Whisker rows
+// that tie on (median, max) keep the tile order (std::sort leaves it open).
+std::vector<std::string> write_workgroup_analysis(const std::string& path, const std::string& name,
+                                                  const std::string& problem, int cameraId, unsigned width,
+                                                  unsigned height, unsigned wg,
+                                                  const std::vector<std::vector<rt_step_stat>>& primary,
+                                                  const std::vector<std::vector<rt_step_stat>>& secondary,
+                                                  const TileVisits& primaryVisits, const TileVisits& secondaryVisits,
+                                                  bool wideAlternative = true);
 
 // Gray value of an ambient-occlusion pixel (cameraFast.h:404-439): a hit is
 // (uint8)((1 - occluded / samples) * 255) in float32, a miss 0; with no
@@ -150,6 +187,31 @@ class CameraFast {
     float focal_;
 };
 
+// CameraData's analysis renderer (cameraData.h:84-560) on the GPU: per camera
+// the primary rays (closest hit) and ambientSampleCount ambient rays per hit
+// go through the instrumented traversal, and _Info.txt gets their counters;
+// with a wide renderer and doWorkGroupAnalysis the per-step tile workload of
+// the primary and ambient rays (a pixel without a hit gets a NaN ray that is
+// not started) goes into the workgroup analysis files.
+class CameraData {
+   public:
+    CameraData(std::string path, std::string name, std::string problem, int workGroupSize, bool wideRender,
+               std::vector<Vec3> positions, std::vector<Vec3> lookCenters, unsigned width = 1920,
+               unsigned height = 1088, Vec3 upward = {0, 1, 0}, float focalLength = 0.866f);
+    // Returns the _Info.txt path.
+    std::string renderImages(FastNodeManager& m, unsigned ambientSampleCount, float ambientDistance,
+                             bool doWorkGroupAnalysis, bool wideAlternative = true);
+
+   private:
+    std::string path_, name_, problem_;
+    int wg_;
+    bool wideRender_;
+    std::vector<Vec3> positions_, targets_;
+    unsigned width_, height_;
+    Vec3 up_;
+    float focal_;
+};
+
 // RayTracer (rayTracer.h:15-85) with the performance test: for every
 // scenario and every (L, N) of the sweep, build, upload and render.
 // Scenarios are procedural (the reference's glTF models are not part of its
@@ -161,7 +223,8 @@ class RayTracer {
     explicit RayTracer(RenderConfig cfg, std::string path = "perf", unsigned gangSize = 4);
     // Returns the median total render time per (scenario, N, L).
     std::vector<std::tuple<int, int, int, float>> run();
-    // RayTracer::renderImage (rayTracer.cpp:588-745, performance branch).
+    // RayTracer::renderImage (rayTracer.cpp:588-745): the performance test
+    // (doPerformanceTest) and the analysis render (renderAnalysisImage).
     float renderImage(unsigned branchingFactor, unsigned leafSize, unsigned gangSize, const std::vector<float>& tris,
                       const std::vector<Vec3>& cameraPositions, const std::vector<Vec3>& cameraTargets,
                       const std::string& name, float sahFactor);

@@ -65,3 +65,27 @@ def test_bvhrt_performance_run(rt, tmp_path):
     assert 0 < (want > 0).sum() < want.size
     assert np.array_equal(got[..., 0], want) and np.all(got[..., 3] == 255)
     scene.close()
+
+
+def test_bvhrt_analysis_matches_python(rt, tmp_path):
+    """renderAnalysisImage with the wide renderer and the workgroup analysis:
+    bvhrt (C++) and the Python RayTracer write byte-identical _Info.txt and
+    WorkGroup*.txt files for the same config."""
+    from bvhraytracer_amd import raytracer
+
+    cfg_text = ("xRes = 64\nyRes = 48\nworkGroupSize = 16\nminLeafSize = 4\nmaxLeafSize = 4\nleafStep = 4\n"
+                "minBranch = 8\nmaxBranch = 8\nbranchStep = 4\nambientSampleCount = 2\ndoPerformanceTest = false\n"
+                "renderAnalysisImage = true\nwideRender = true\ndoWorkGroupAnalysis = true\nscenario = 0\n")
+    cfg = tmp_path / "config.txt"
+    cfg.write_text(cfg_text)
+    print("bvhrt analysis spheres 64x48 N8L4", flush=True)
+    r = subprocess.run([BVHRT, str(cfg), str(tmp_path / "cpp")], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    raytracer.RayTracer(raytracer.read_config(str(cfg)), out_dir=str(tmp_path / "py")).run(("spheres",))
+    names = ["spheres_b8_l4_Info.txt"] + [f"WorkGroupSize_16_Version_1/spheres_b8_l4_c0_{k}.txt" for k in
+                                          ("PrimaryWorkGroupWiskerPlot", "SecondaryWorkGroupWiskerPlot",
+                                           "WorkGroupData", "WorkGroupUniqueWork")]
+    for n in names:
+        py = (tmp_path / "py" / n).read_text()
+        assert len(py.splitlines()) > 5
+        assert (tmp_path / "cpp" / n).read_text() == py, n
