@@ -153,6 +153,8 @@ SIGNATURES = [
     ("rt_host_bvh_free", ctypes.c_int, [_P]),
     ("rt_camera_rays", ctypes.c_int, [ctypes.POINTER(RtCamera), _U32, _U32, _U32, _P]),
     ("rt_hit_surface", ctypes.c_int, [_P, _P, _P, _U32, _P, _P]),
+    ("rt_render_ambient", ctypes.c_int, [_P, ctypes.POINTER(RtCamera), _U32, _U32, _U32, _U32, _U32, ctypes.c_float,
+                                         _P, _P, _P]),
     ("rt_spawn_ambient", ctypes.c_int, [_P, _P, _P, _U32, _U32, ctypes.c_float, _P, _P, _P]),
     ("rt_spawn_shadow", ctypes.c_int, [_P, _P, _P, _U32, _P, ctypes.c_float, _P, _P,
                                        ctypes.POINTER(_U32)]),

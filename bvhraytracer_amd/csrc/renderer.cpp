@@ -569,7 +569,6 @@ std::string CameraFast::imageFile(int cameraId) const {
 std::tuple<float, float, float> CameraFast::renderImage(bool saveImage, FastNodeManager& m,
                                                         unsigned ambientSampleCount, float ambientDistance,
                                                         int cameraId, std::vector<uint8_t>* image) {
-    const auto t0 = std::chrono::steady_clock::now();
     rt_camera cam{};
     for (int a = 0; a < 3; ++a) {
         cam.pos[a] = positions_[cameraId][a];
@@ -578,46 +577,24 @@ std::tuple<float, float, float> CameraFast::renderImage(bool saveImage, FastNode
     }
     cam.focal = focal_;
     const size_t n = (size_t)width_ * height_;
-    std::vector<rt_ray> rays(n);
-    check(rt_camera_rays(&cam, width_, height_, (uint32_t)wg_, rays.data()), "rt_camera_rays");
-    std::vector<rt_hit> hits;
+    std::vector<uint8_t> gray(n);
+    float ms[2] = {0, 0};
     // the ray-stream traversal has no save-distance form
-    m.intersect(rays, hits, nullptr, saveDistance_ && !wideRender_, wideRender_);
-    const float tPrimary = (float)seconds_since(t0);
-    std::vector<uint16_t> occl(n, 0);
-    if (ambientSampleCount) {
-        std::vector<float> nrm, pos;
-        m.getSurfaceNormalPosition(rays, hits, nrm, pos);
-        std::vector<rt_ray> amb(n);
-        std::vector<uint32_t> src(n);
-        std::vector<uint8_t> occ;
-        for (unsigned a = 0; a < ambientSampleCount; ++a) {
-            uint32_t k = 0;
-            check(rt_spawn_ambient(nrm.data(), pos.data(), hits.data(), (uint32_t)n, a, ambientDistance, amb.data(),
-                                   src.data(), &k),
-                  "rt_spawn_ambient");
-            if (k == 0) break;
-            std::vector<rt_ray> batch(amb.begin(), amb.begin() + k);
-            m.intersectSecondary(batch, occ, nullptr, wideRender_);
-            for (uint32_t i = 0; i < k; ++i) occl[src[i]] += occ[i];
+    const uint32_t flags = wideRender_ ? RT_STREAM : (saveDistance_ ? RT_SAVE_DIST : RT_CLOSEST);
+    check(rt_render_ambient(m.scene(), &cam, width_, height_, (uint32_t)wg_, flags, ambientSampleCount,
+                            ambientDistance, gray.data(), ms, nullptr),
+          "rt_render_ambient");
+    if (saveImage) {
+        const std::vector<uint8_t> rows = tile_order_to_rows(gray, width_, height_, (unsigned)wg_);
+        std::vector<uint8_t> rgba(n * 4);
+        for (size_t i = 0; i < n; ++i) {
+            rgba[4 * i] = rgba[4 * i + 1] = rgba[4 * i + 2] = rows[i];
+            rgba[4 * i + 3] = 255;
         }
+        write_png(imageFile(cameraId), rgba, width_, height_);
     }
-    const float total = (float)seconds_since(t0);
-    if (image || saveImage) {
-        std::vector<uint8_t> gray(n);
-        for (size_t i = 0; i < n; ++i) gray[i] = ambient_gray(hits[i].slot >= 0, occl[i], ambientSampleCount);
-        if (saveImage) {
-            const std::vector<uint8_t> rows = tile_order_to_rows(gray, width_, height_, (unsigned)wg_);
-            std::vector<uint8_t> rgba(n * 4);
-            for (size_t i = 0; i < n; ++i) {
-                rgba[4 * i] = rgba[4 * i + 1] = rgba[4 * i + 2] = rows[i];
-                rgba[4 * i + 3] = 255;
-            }
-            write_png(imageFile(cameraId), rgba, width_, height_);
-        }
-        if (image) *image = std::move(gray);
-    }
-    return std::make_tuple(total, tPrimary, 0.0f);
+    if (image) *image = std::move(gray);
+    return std::make_tuple(ms[1] / 1000.0f, ms[0] / 1000.0f, 0.0f);
 }
 
 std::pair<float, float> CameraFast::renderImages(bool saveImage, FastNodeManager& m, unsigned ambientSampleCount,

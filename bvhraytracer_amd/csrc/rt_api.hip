@@ -41,6 +41,24 @@ int hip_fail(hipError_t e, const char* what) {
         if (_e != hipSuccess) return hip_fail(_e, what); \
     } while (0)
 
+// A NULL stream argument keeps the null stream's order: the library's own
+// non-blocking stream first waits for the work already queued on the null
+// stream (inputs made there are complete), and a call that returns before its
+// work ends makes the null stream wait for it.  The library keeps its own
+// stream because graph capture and per-stream scratch need a real stream.
+int null_stream_before(hipStream_t own, hipEvent_t& ev) {
+    if (!ev) HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+    HIP_TRY(hipEventRecord(ev, nullptr), "hipEventRecord");
+    HIP_TRY(hipStreamWaitEvent(own, ev, 0), "hipStreamWaitEvent");
+    return RT_OK;
+}
+
+int null_stream_after(hipStream_t own, hipEvent_t ev) {
+    HIP_TRY(hipEventRecord(ev, own), "hipEventRecord");
+    HIP_TRY(hipStreamWaitEvent(nullptr, ev, 0), "hipStreamWaitEvent");
+    return RT_OK;
+}
+
 // Per-stream scratch: work counters, stack spill and host staging buffers.
 struct Workspace {
     uint32_t* work = nullptr;  // kWorkWords words
@@ -100,6 +118,7 @@ struct rt_scene {
     uint32_t cus = 0;
     std::unordered_map<hipStream_t, Workspace> ws;
     hipStream_t ownStream = nullptr;
+    hipEvent_t joinEv = nullptr;  // orders ownStream with the null stream
     float lastMs = 0;
     uint64_t raysTraced = 0;
     uint32_t errorRays = 0;
@@ -282,6 +301,7 @@ void free_scene(rt_scene* s) {
     if (s->leafLc) (void)hipFree(s->leafLc);
     if (s->leafPrims) (void)hipFree(s->leafPrims);
     if (s->leafTest) (void)hipFree(s->leafTest);
+    if (s->joinEv) (void)hipEventDestroy(s->joinEv);
     if (s->ownStream) (void)hipStreamDestroy(s->ownStream);
     (void)hipSetDevice(cur);
     delete s;
@@ -414,6 +434,16 @@ extern "C" int rt_hit_surface_device(rt_scene_t s, const rt_ray* rays, const rt_
     return RT_OK;
 }
 
+// The stream rt_trace uses for a scene when the caller passes none.
+int rt::scene_stream(rt_scene_t s, hipStream_t* out) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->ownStream) HIP_TRY(hipStreamCreateWithFlags(&s->ownStream, hipStreamNonBlocking), "hipStreamCreate");
+    const int rc = null_stream_before(s->ownStream, s->joinEv);
+    if (rc != RT_OK) return rc;
+    *out = s->ownStream;
+    return RT_OK;
+}
+
 extern "C" int rt_set_schedule(rt_scene_t s, uint32_t leaf_ratio16) {
     if (!s) return set_error(RT_ERR_ARG, "rt_set_schedule: null scene");
     if (leaf_ratio16 > 1024) return set_error(RT_ERR_ARG, "rt_set_schedule: ratio must be at most 1024 (64.0)");
@@ -468,9 +498,13 @@ extern "C" int rt_trace_steps(rt_scene_t s, const rt_ray* rays, uint32_t n, uint
         }
     } restore{s->device, prevDev};
     hipStream_t st = reinterpret_cast<hipStream_t>(hip_stream);
+    const bool joinNull = !st;  // the call waits for its work: nothing to join after
+    (void)joinNull;
     if (!st) {
         if (!s->ownStream) HIP_TRY(hipStreamCreateWithFlags(&s->ownStream, hipStreamNonBlocking), "hipStreamCreate");
         st = s->ownStream;
+        const int jr = null_stream_before(st, s->joinEv);
+        if (jr != RT_OK) return jr;
     }
     Workspace* w = nullptr;
     int rc = get_workspace(s, st, w);
@@ -564,9 +598,12 @@ int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, voi
         }
     } restore{s->device, prevDev};
     hipStream_t st = reinterpret_cast<hipStream_t>(hip_stream);
+    const bool joinNull = !st;
     if (!st) {
         if (!s->ownStream) HIP_TRY(hipStreamCreateWithFlags(&s->ownStream, hipStreamNonBlocking), "hipStreamCreate");
         st = s->ownStream;
+        const int jr = null_stream_before(st, s->joinEv);
+        if (jr != RT_OK) return jr;
     }
     Workspace* w = nullptr;
     int rc = get_workspace(s, st, w);
@@ -657,6 +694,10 @@ int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, voi
         HIP_TRY(hipMemcpyAsync(out, dOut, outBytes, hipMemcpyDeviceToHost, st), "hipMemcpy(out)");
         if (cntBytes) HIP_TRY(hipMemcpyAsync(counts, dCounts, cntBytes, hipMemcpyDeviceToHost, st), "hipMemcpy(counts)");
     }
+    if (dev && (flags & RT_NO_SYNC) && joinNull) {
+        const int jr = null_stream_after(st, s->joinEv);
+        if (jr != RT_OK) return jr;
+    }
     if (!dev || !(flags & RT_NO_SYNC)) {
         HIP_TRY(hipStreamSynchronize(st), "trace kernel");
         float ms = 0;
@@ -729,6 +770,7 @@ struct rt_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool pending = false;
     hipStream_t ownStream = nullptr;
+    hipEvent_t joinEv = nullptr;  // orders ownStream with the null stream
     // captured graph: [reset kernel] -> [batched trace kernel]
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -767,6 +809,7 @@ void free_batch(rt_batch* b) {
     if (b->spill) (void)hipFree(b->spill);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->joinEv) (void)hipEventDestroy(b->joinEv);
     if (b->ownStream) (void)hipStreamDestroy(b->ownStream);
     (void)hipSetDevice(cur);
     delete b;
@@ -865,9 +908,12 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
         }
     } restore{b->device, prevDev};
     hipStream_t st = reinterpret_cast<hipStream_t>(hip_stream);
+    const bool joinNull = !st;
     if (!st) {
         if (!b->ownStream) HIP_TRY(hipStreamCreateWithFlags(&b->ownStream, hipStreamNonBlocking), "hipStreamCreate");
         st = b->ownStream;
+        const int jr = null_stream_before(st, b->joinEv);
+        if (jr != RT_OK) return jr;
     }
     // the staging tables are reused: the previous batch must have consumed them
     if (b->pending) {
@@ -970,6 +1016,10 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     }
     b->pending = true;
     b->raysTraced += total;
+    if ((flags & RT_NO_SYNC) && joinNull) {
+        const int jr = null_stream_after(st, b->joinEv);
+        if (jr != RT_OK) return jr;
+    }
     if (!(flags & RT_NO_SYNC)) {
         HIP_TRY(hipEventSynchronize(b->ev1), "batched trace kernel");
         b->pending = false;

@@ -13,4 +13,9 @@ hipError_t launch_surface(const float* tris, const uint8_t* leafLc, uint32_t Lm,
                           const rt_ray* rays, const rt_hit* hits, uint32_t n, float* nrm, float* pos,
                           hipStream_t stream);
 
+// The stream rt_trace uses for scene s when the caller passes none (created
+// on first use); calls that enqueue several steps use it too, so every step
+// runs on one stream.
+int scene_stream(rt_scene_t s, hipStream_t* out);
+
 }  // namespace rt

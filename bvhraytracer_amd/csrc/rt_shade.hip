@@ -127,6 +127,37 @@ __global__ void __launch_bounds__(kShadeBlock) k_diffuse(const rt_ray* __restric
     out[r] = ray;
 }
 
+// Dense ambient rays of one sample: ray i = rt_spawn_ambient's ray for a hit
+// pixel, dead_ray() for a miss (never occluded).
+__global__ void __launch_bounds__(kShadeBlock) k_ambient(const float* __restrict__ nrm, const float* __restrict__ pos,
+                                                         const rt_hit* __restrict__ hits, uint32_t n, uint32_t sample,
+                                                         float dist, rt_ray* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
+    if (i >= n) return;
+    out[i] = hits[i].slot >= 0 ? ambient_ray(i, sample, nrm + (size_t)i * 3, pos + (size_t)i * 3, dist) : dead_ray();
+}
+
+__global__ void __launch_bounds__(kShadeBlock) k_occl_add(const uint8_t* __restrict__ occ, uint32_t* __restrict__ acc,
+                                                          uint32_t n) {
+    const uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
+    if (i < n) acc[i] += occ[i];
+}
+
+// Gray value per pixel (cameraFast.h:404-439): (uint8)((1 - occluded /
+// samples) * 255) for a hit, 0 for a miss or with no samples.
+__global__ void __launch_bounds__(kShadeBlock) k_gray(const rt_hit* __restrict__ hits,
+                                                      const uint32_t* __restrict__ acc, uint32_t samples, uint32_t n,
+                                                      uint8_t* __restrict__ gray) {
+    const uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
+    if (i >= n) return;
+    uint8_t g = 0;
+    if (hits[i].slot >= 0 && samples != 0) {
+        const float factor = 1.0f - ((float)acc[i] / (float)samples);
+        g = (uint8_t)(factor * 255.0f);
+    }
+    gray[i] = g;
+}
+
 }  // namespace
 
 hipError_t launch_surface(const float* tris, const uint8_t* leafLc, uint32_t Lm, uint32_t nLeaves,
@@ -207,4 +238,83 @@ extern "C" int rt_spawn_diffuse_device(const rt_ray* primary, const float* norma
     if (e != hipSuccess) return set_error(RT_ERR_HIP, std::string("diffuse spawn: ") + hipGetErrorString(e));
     if (f != hipSuccess) return set_error(RT_ERR_HIP, std::string("hipFreeAsync: ") + hipGetErrorString(f));
     return RT_OK;
+}
+
+extern "C" int rt_render_ambient(rt_scene_t s, const rt_camera* cam, uint32_t width, uint32_t height, uint32_t wg,
+                                 uint32_t flags, uint32_t samples, float dist, uint8_t* gray_out, float* ms_out,
+                                 void* hip_stream) {
+    if (!s || !cam || !gray_out || width == 0 || height == 0 || wg == 0)
+        return set_error(RT_ERR_ARG, "rt_render_ambient: bad argument");
+    if (flags & ~(RT_SAVE_DIST | RT_STREAM))
+        return set_error(RT_ERR_ARG, "rt_render_ambient: flags are RT_CLOSEST or RT_SAVE_DIST, plus RT_STREAM");
+    if (width % wg != 0) return set_error(RT_ERR_ARG, "rt_render_ambient: width must be a multiple of wg");
+    const uint64_t n64 = (uint64_t)width * height;
+    if (n64 >= 0x80000000ull) return set_error(RT_ERR_ARG, "rt_render_ambient: at most 2^31-1 pixels");
+    const uint32_t n = (uint32_t)n64;
+    // every step on one stream: the caller's, or the one rt_trace picks for
+    // the scene when none is given (not the null stream, which the scene's
+    // non-blocking stream does not wait for)
+    hipStream_t st = static_cast<hipStream_t>(hip_stream);
+    if (!st) {
+        const int r = scene_stream(s, &st);
+        if (r != RT_OK) return r;
+    }
+    // rays | hits | normals | positions | ambient rays | occluded | counts | gray
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t oRays = 0, oHits = oRays + up(n * sizeof(rt_ray)), oNrm = oHits + up(n * sizeof(rt_hit));
+    const size_t oPos = oNrm + up((size_t)n * 12), oAmb = oPos + up((size_t)n * 12);
+    const size_t oOcc = oAmb + up(n * sizeof(rt_ray)), oAcc = oOcc + up(n), oGray = oAcc + up((size_t)n * 4);
+    const size_t total = oGray + up(n);
+    char* buf = nullptr;
+    hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), total, st);
+    if (e != hipSuccess) return set_error(RT_ERR_NOMEM, std::string("hipMallocAsync: ") + hipGetErrorString(e));
+    rt_ray* rays = reinterpret_cast<rt_ray*>(buf + oRays);
+    rt_hit* hits = reinterpret_cast<rt_hit*>(buf + oHits);
+    float* nrm = reinterpret_cast<float*>(buf + oNrm);
+    float* pos = reinterpret_cast<float*>(buf + oPos);
+    rt_ray* amb = reinterpret_cast<rt_ray*>(buf + oAmb);
+    uint8_t* occ = reinterpret_cast<uint8_t*>(buf + oOcc);
+    uint32_t* acc = reinterpret_cast<uint32_t*>(buf + oAcc);
+    uint8_t* gray = reinterpret_cast<uint8_t*>(buf + oGray);
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    int rc = RT_OK;
+    for (auto& x : ev)
+        if (hipEventCreate(&x) != hipSuccess) rc = set_error(RT_ERR_HIP, "rt_render_ambient: hipEventCreate");
+    auto step = [&](hipError_t err, const char* what) {
+        if (rc == RT_OK && err != hipSuccess) rc = set_error(RT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(err));
+    };
+    if (rc == RT_OK) step(hipEventRecord(ev[0], st), "hipEventRecord");
+    if (rc == RT_OK) rc = rt_camera_rays_device(cam, width, height, wg, rays, st);
+    const uint32_t dev = RT_DEVICE_PTRS | RT_NO_SYNC;
+    if (rc == RT_OK) rc = rt_trace(s, rays, n, flags | dev, hits, nullptr, st);
+    if (rc == RT_OK) step(hipEventRecord(ev[1], st), "hipEventRecord");
+    if (rc == RT_OK && samples) {
+        rc = rt_hit_surface_device(s, rays, hits, n, nrm, pos, st);
+        if (rc == RT_OK) step(launch_zero(acc, n, st), "zero kernel");
+        for (uint32_t a = 0; rc == RT_OK && a < samples; ++a) {
+            hipLaunchKernelGGL(k_ambient, dim3(blocks_for(n)), dim3(kShadeBlock), 0, st, nrm, pos, hits, n, a, dist, amb);
+            step(hipGetLastError(), "ambient kernel");
+            if (rc == RT_OK) rc = rt_trace(s, amb, n, RT_ANY | (flags & RT_STREAM) | dev, occ, nullptr, st);
+            if (rc == RT_OK) {
+                hipLaunchKernelGGL(k_occl_add, dim3(blocks_for(n)), dim3(kShadeBlock), 0, st, occ, acc, n);
+                step(hipGetLastError(), "occlusion kernel");
+            }
+        }
+    }
+    if (rc == RT_OK) {
+        hipLaunchKernelGGL(k_gray, dim3(blocks_for(n)), dim3(kShadeBlock), 0, st, hits, acc, samples, n, gray);
+        step(hipGetLastError(), "gray kernel");
+    }
+    if (rc == RT_OK) step(hipEventRecord(ev[2], st), "hipEventRecord");
+    if (rc == RT_OK) step(hipMemcpyAsync(gray_out, gray, n, hipMemcpyDeviceToHost, st), "hipMemcpy(gray)");
+    const hipError_t f = hipFreeAsync(buf, st);
+    step(hipStreamSynchronize(st), "rt_render_ambient");
+    step(f, "hipFreeAsync");
+    if (rc == RT_OK && ms_out) {
+        step(hipEventElapsedTime(&ms_out[0], ev[0], ev[1]), "hipEventElapsedTime");
+        step(hipEventElapsedTime(&ms_out[1], ev[0], ev[2]), "hipEventElapsedTime");
+    }
+    for (auto& x : ev)
+        if (x) (void)hipEventDestroy(x);
+    return rc;
 }

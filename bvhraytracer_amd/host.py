@@ -252,6 +252,21 @@ class Scene:
         out = out[:nt * step_cap].reshape(nt, step_cap) if nt else out[:0]
         return [out[i, :lens[i]].copy() for i in range(nt)], cnt
 
+    def render_ambient(self, pos, target, width, height, wg=16, samples=1, dist=1.0, mode="closest", wide=False,
+                       up=(0.0, 1.0, 0.0), focal=0.866):
+        """rt_render_ambient: one performance-renderer frame on the GPU.
+        Returns (gray values per pixel in tile order, (primary ms, frame ms))."""
+        cam = RtCamera((ctypes.c_float * 3)(*pos), (ctypes.c_float * 3)(*target), (ctypes.c_float * 3)(*up),
+                       float(focal))
+        if mode not in ("closest", "save_distance"):
+            raise ValueError("render_ambient: mode must be 'closest' or 'save_distance'")
+        flags = MODES[mode] | (_lib.RT_STREAM if wide else 0)
+        gray = np.empty(width * height, np.uint8)
+        ms = (ctypes.c_float * 2)()
+        check(lib().rt_render_ambient(self._h, ctypes.byref(cam), width, height, wg, flags, int(samples), float(dist),
+                                      ptr(gray), ms, None))
+        return gray, (ms[0], ms[1])
+
     def surface_device(self, rays, hits, normal=None, position=None, stream=None):
         """rt_hit_surface_device: hit reconstruction from the resident
         triangles.  rays (n,10) float32, hits (n,6) int32 CUDA tensors;

@@ -195,6 +195,15 @@ class CameraFast:
         total = time.perf_counter() - t0
         return total, t_primary, hits, ambient_image(occl, hit, ambient_samples)
 
+    def render_image_device(self, scene, cam, ambient_samples, ambient_distance, mode="closest", wide=False):
+        """The same frame as render_image with every step on the GPU
+        (rt_render_ambient; identical gray values).  Times are the GPU's
+        (HIP events): (total s, primary s, None, image)."""
+        gray, (p_ms, t_ms) = scene.render_ambient(self.positions[cam], self.targets[cam], self.width, self.height,
+                                                  self.wg, ambient_samples, ambient_distance, mode=mode, wide=wide,
+                                                  focal=self.focal)
+        return t_ms / 1000.0, p_ms / 1000.0, None, gray
+
     def save_image(self, path, name, cam, image):
         """<path>/<name>_c<cam>_Perf.png (cameraFast.h:829-852), gray RGBA."""
         g = tile_order_to_rows(image, self.width, self.height, self.wg)
@@ -206,19 +215,24 @@ class CameraFast:
         return out
 
     def render_images(self, scene, bvh, ambient_samples=1, ambient_distance=1.0, samples=5,
-                      mode="closest", save_image=False, path=".", name="image"):
+                      mode="closest", save_image=False, path=".", name="image", device=True):
         """With save_image, one untimed render per camera first writes its
-        _Perf.png (cameraFast.h:81-89)."""
+        _Perf.png (cameraFast.h:81-89).  device: every frame on the GPU
+        (render_image_device, GPU times); else the host-driven render_image
+        (wall times)."""
+        def one(cam):
+            if device:
+                return self.render_image_device(scene, cam, ambient_samples, ambient_distance, mode)
+            return self.render_image(scene, bvh, cam, ambient_samples, ambient_distance, mode)
+
         totals, primaries = [], []
         if save_image:
             os.makedirs(path, exist_ok=True)
             for cam in range(len(self.positions)):
-                img = self.render_image(scene, bvh, cam, ambient_samples, ambient_distance, mode)[3]
-                self.save_image(path, name, cam, img)
+                self.save_image(path, name, cam, one(cam)[3])
         for cam in range(len(self.positions)):
-            self.render_image(scene, bvh, cam, ambient_samples, ambient_distance, mode)  # warm-up
-            res = sorted(self.render_image(scene, bvh, cam, ambient_samples, ambient_distance, mode)[:2]
-                         for _ in range(samples))
+            one(cam)  # warm-up
+            res = sorted(one(cam)[:2] for _ in range(samples))
             med = res[samples // 2]
             totals.append(med[0])
             primaries.append(med[1])

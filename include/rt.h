@@ -130,7 +130,10 @@ int rt_upload_bvh(const rt_bvh_desc* desc, rt_scene_t* out);
 
 /* Traces n rays.  out is rt_hit[n] for RT_CLOSEST / RT_SAVE_DIST and
  * uint8_t[n] (1 = occluded) for RT_ANY.  counts may be NULL unless RT_COUNTS.
- * hip_stream is a hipStream_t (NULL = the library's stream).  With
+ * hip_stream is a hipStream_t.  NULL keeps the null stream's order: the
+ * work starts after what is already queued on the null stream, and with
+ * RT_NO_SYNC later null-stream work waits for it (it runs on a stream of
+ * the library's).  With
  * RT_DEVICE_PTRS the buffers stay on the device; otherwise they are staged. */
 int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags,
              void* out, rt_counts* counts, void* hip_stream);
@@ -243,7 +246,8 @@ typedef struct rt_batch_item {
                          with the new item table) instead of a direct launch  */
 
 int rt_batch_create(uint32_t max_items, rt_batch_t* out);
-/* Items must all live on the batch's device; total rays < 2^31. */
+/* Items must all live on the batch's device; total rays < 2^31.  A NULL
+ * hip_stream keeps the null stream's order, as for rt_trace. */
 int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t n_items, uint32_t flags,
                    void* hip_stream);
 int rt_batch_set_launch(rt_batch_t b, uint32_t block_size, uint32_t stack_window);
@@ -397,6 +401,21 @@ int rt_spawn_shadow_device(const float* normal, const float* position, const rt_
 int rt_spawn_diffuse_device(const rt_ray* primary, const float* normal, const float* position,
                             const rt_hit* hits, uint32_t n, uint64_t seed, uint32_t n_out, float tmax,
                             rt_ray* out, void* hip_stream);
+
+/* One frame of the performance renderer on the GPU (CameraFast::renderImage,
+ * cameraFast.h:318-461): the primary rays of cam in tile order traced
+ * (flags: RT_CLOSEST or RT_SAVE_DIST, plus RT_STREAM), their hit surfaces,
+ * `samples` ambient rays per hit pixel (rt_spawn_ambient's rays, any hit,
+ * with RT_STREAM if given), and per pixel the gray value (uint8)((1 -
+ * occluded / samples) * 255), 0 for a miss, written to gray_out[width *
+ * height] (host memory, tile order).  ms_out (optional, host, 2 floats) =
+ * {primary rays, whole frame} in ms by HIP events.  The result equals the
+ * host pipeline's (rt_camera_rays, rt_trace, rt_hit_surface,
+ * rt_spawn_ambient).  Device scratch is allocated stream-ordered
+ * (hipMallocAsync); the call waits for the frame. */
+int rt_render_ambient(rt_scene_t s, const rt_camera* cam, uint32_t width, uint32_t height, uint32_t wg,
+                      uint32_t flags, uint32_t samples, float dist, uint8_t* gray_out, float* ms_out,
+                      void* hip_stream);
 
 /* sahFactorConfigSse.txt lookup (RayTracer::loadSahFactor, rayTracer.cpp:880-925). */
 float rt_sah_factor(uint32_t leaf_size, uint32_t branching, uint32_t gang, const char* table_path);

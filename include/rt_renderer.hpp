@@ -167,9 +167,10 @@ class CameraFast {
                                          float ambientDistance, bool mute = true, bool wideAlternative = true);
 
     // renderImage (:170-582): primary rays (closest hit, or save distance),
-    // hit surfaces, ambientSampleCount any-hit rays per hit.  Returns (total
-    // seconds, primary-ray seconds, 0); image (optional) receives the gray
-    // value per pixel in tile order; saveImage writes the PNG.
+    // hit surfaces, ambientSampleCount any-hit rays per hit, all on the GPU
+    // (rt_render_ambient).  Returns (frame seconds, primary-ray seconds, 0) by
+    // HIP events; image (optional) receives the gray value per pixel in tile
+    // order; saveImage writes the PNG.
     std::tuple<float, float, float> renderImage(bool saveImage, FastNodeManager& m, unsigned ambientSampleCount,
                                                 float ambientDistance, int cameraId,
                                                 std::vector<uint8_t>* image = nullptr);

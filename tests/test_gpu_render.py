@@ -94,3 +94,28 @@ def test_workgroup_analysis_end_to_end(rt, tmp_path):
     assert len(lines[0]) == len(lines[1]) == len(lines[3]) == tiles + 1
     assert len(lines[2]) == max(len(t) for t in prim) + 1
     scene.close()
+
+
+@pytest.mark.parametrize("mode,wide,samples", [("closest", False, 3), ("save_distance", False, 1), ("closest", True, 2),
+                                               ("closest", False, 0)])
+def test_device_frame_equals_host_pipeline(rt, mode, wide, samples):
+    """rt_render_ambient (camera rays, trace, surfaces, ambient rays, trace,
+    gray values, all on the GPU) gives the host-driven frame's gray values
+    exactly, for a partial last tile row too."""
+    from bvhraytracer_amd.raytracer import CameraFast
+
+    print(f"device frame {mode} wide={wide} samples={samples}", flush=True)
+    tris = rt.interior_scene(3, 60_000, 40)
+    bvh = rt.Bvh(tris, 8, 4)
+    scene = rt.Scene(bvh)
+    cam = CameraFast([(0.0, 5.0, -14.0)], [(0.0, 3.0, 0.0)], 96, 40, 16)
+    host = cam.render_image(scene, bvh, 0, samples, 2.0, mode=mode)[3]
+    total, primary, _, dev = cam.render_image_device(scene, 0, samples, 2.0, mode=mode, wide=wide)
+    assert np.array_equal(dev, host)
+    assert 0 < primary <= total
+    if samples:
+        # some pixels lit, some occluded or missed; with several samples,
+        # partial occlusion too
+        assert 0 < (host > 0).sum() < len(host)
+        assert len(np.unique(host)) > (2 if samples > 1 else 1)
+    scene.close()

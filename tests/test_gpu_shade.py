@@ -136,3 +136,25 @@ def test_device_bounce_pipeline(rt, frame):
     assert_closest_equal(_host(d_bh, rt.HIT_DTYPE), ref["hits"], bvh, _host(d_bc, rt.COUNTS_DTYPE),
                          ref["counts"], "device bounce")
     assert (ref["diag"] & oracle.DIAG_BOGUS).sum() == 0
+
+
+def test_null_stream_chain(rt, frame):
+    """Every device entry point called with no stream, without waiting in
+    between: camera rays -> trace -> surfaces -> diffuse rays -> trace on the
+    null stream's order.  The library traces on a stream of its own, which
+    must wait for the null stream's work and make it wait in turn."""
+    import torch
+
+    tris, bvh, scene, prim, hits, nrm, pos, d_prim, d_hits = frame
+    for rep in range(3):
+        d_rays = rt.camera_rays_device(CAM_POS, CAM_TGT, 320, 240, 16)
+        d_h = torch.empty((d_rays.shape[0], 6), dtype=torch.int32, device="cuda")
+        scene.intersect_device(d_rays, d_h, sync=False)
+        d_n, d_p = scene.surface_device(d_rays, d_h)
+        d_b = rt.spawn_diffuse_device(d_rays, d_n, d_p, d_h, 40_000, seed=11)
+        d_bh = torch.empty((40_000, 6), dtype=torch.int32, device="cuda")
+        scene.intersect_device(d_b, d_bh, sync=False)
+        got = _host(d_bh, rt.HIT_DTYPE)  # .cpu() runs on the null stream
+        bounce = rt.spawn_diffuse(prim, nrm, pos, hits, 40_000, seed=11)
+        assert np.array_equal(_bits(_host(d_b, rt.RAY_DTYPE)), _bits(bounce)), rep
+        assert np.array_equal(got, scene.intersect(bounce)), rep
