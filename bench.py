@@ -221,6 +221,34 @@ def bench_frame(rt, torch, scene, bvh, n_bounce, log, reps=3):
     return out
 
 
+def bench_render_frame(rt, scene, bvh, tris, log, reps=3):
+    """One frame of the reference's performance renderer (CameraFast::
+    renderImage: 1920x1088 primaries, closest hit, one ambient-occlusion ray
+    per hit): device = rt_render_ambient (every stage on the GPU, wall clock
+    of the call), host-assisted = the host-driven render_image (host rays and
+    surfaces, host-pointer traces).  Median of reps frames after a warm-up;
+    the two gray images must be identical."""
+    from bvhraytracer_amd.raytracer import CameraFast, ambient_distance
+
+    cam = CameraFast([(0.0, 5.0, -14.0)], [(0.0, 3.0, 0.0)], 1920, 1088, 16)
+    dist = ambient_distance(tris)
+    out, imgs = {}, {}
+    fns = (("device", lambda: scene.render_ambient(cam.positions[0], cam.targets[0], 1920, 1088, 16, 1, dist)[0]),
+           ("host_assisted", lambda: cam.render_image(scene, bvh, 0, 1, dist)[3]))
+    for name, fn in fns:
+        imgs[name] = fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        out[name + "_ms"] = round(float(np.median(ts)) * 1000.0, 3)
+    out["identical"] = bool(np.array_equal(imgs["device"], imgs["host_assisted"]))
+    log(f"render frame: device {out['device_ms']} ms, host-assisted {out['host_assisted_ms']} ms, "
+        f"identical {out['identical']}")
+    return out
+
+
 def bench_serving(rt, args, log):
     """16 resident scenes (100k..2M tris log-spaced, seeds 200..215), Poisson
     arrivals at 200 req/s for args.serve_seconds (seed 5), rays per request
@@ -406,6 +434,7 @@ def main():
     primary_mrays = len(prim) / (float(np.median(pms)) / 1000.0) / 1e6
 
     frame = None if args.no_frame else bench_frame(rt, torch, scene, bvh, n, log)
+    render_frame = None if args.no_frame else bench_render_frame(rt, scene, bvh, tris, log)
     batched = None if args.no_batch else bench_batched(rt, torch, args, log)
     serving = None if args.no_serve else bench_serving(rt, args, log)
 
@@ -454,6 +483,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "bounce_frame": frame,
+        "render_frame": render_frame,
         "batched": batched,
         "serving": serving,
     }
