@@ -370,7 +370,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards rays "
-              "across GPUs by replicating the BVH, which this build does not launch.", file=sys.stderr)
+              "across GPUs by replicating the BVH, a layout not launched here.", file=sys.stderr)
         sys.exit(2)
 
     import torch

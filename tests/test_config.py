@@ -43,7 +43,7 @@ def test_reference_key_semantics(tmp_path):
 
 
 def test_shipped_config_values(tmp_path):
-    # the values of the reference's config.txt that this build keeps
+    # the sweep values of the reference's shipped config.txt
     p = tmp_path / "config.txt"
     p.write_text("minLeafSize = 4\nmaxLeafSize = 16\nleafStep = 4\nminBranch = 4\nmaxBranch = 16\n"
                  "branchStep = 4\nworkGroupSize = 16\n")
