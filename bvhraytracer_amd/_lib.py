@@ -57,6 +57,7 @@ class RtBuildParams(ctypes.Structure):
         ("branching", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32), ("gang", ctypes.c_uint32),
         ("sort_each_split", ctypes.c_uint32), ("leaf_split_option", ctypes.c_int32),
         ("sah_factor", ctypes.c_float), ("threads", ctypes.c_uint32),
+        ("node_memory", ctypes.c_uint32), ("leaf_memory", ctypes.c_uint32),
     ]
 
 

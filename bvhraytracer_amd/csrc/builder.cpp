@@ -459,6 +459,18 @@ int build_bvh(const float* tris, uint32_t n, const rt_build_params& p, HostBvh& 
     out.G = G;
     out.M = (p.branching + G - 1) / G * G;
     out.Lm = (p.leaf_size + G - 1) / G * G;
+    // the node / leaf memory tests (rayTracer.cpp:41-52,630-640) flatten into
+    // larger records: a multiple of G at least the minimum
+    if (p.node_memory) {
+        if (p.node_memory < out.M || p.node_memory % G)
+            return set_error(RT_ERR_ARG, "rt_build: node_memory must be a multiple of the gang size >= roundup(N, G)");
+        out.M = p.node_memory;
+    }
+    if (p.leaf_memory) {
+        if (p.leaf_memory < out.Lm || p.leaf_memory % G)
+            return set_error(RT_ERR_ARG, "rt_build: leaf_memory must be a multiple of the gang size >= roundup(L, G)");
+        out.Lm = p.leaf_memory;
+    }
     if (out.M > 16 || out.Lm > 16)
         return set_error(RT_ERR_UNSUPPORTED, "rt_build: node and leaf memory are limited to 16");
     std::vector<Prim> prims(n);

@@ -14,6 +14,7 @@
 #include <filesystem>
 #include <fstream>
 #include <iostream>
+#include <memory>
 
 #include "../../include/rt_renderer.hpp"
 
@@ -134,9 +135,11 @@ float load_sah_factor(int leafSize, int branchingFactor, int gangSize, const std
 // ---------------------------------------------------------------------------
 
 FastNodeManager::FastNodeManager(const std::vector<float>& tris, unsigned branching, unsigned leaf, unsigned gang,
-                                 float sahFactor, bool sortEachSplit, int leafSplitOption, unsigned threads) {
+                                 float sahFactor, bool sortEachSplit, int leafSplitOption, unsigned threads,
+                                 unsigned nodeMemory_, unsigned leafMemory_) {
     if (tris.size() % 9 != 0) throw Error("FastNodeManager: triangle array is not a multiple of 9 floats");
-    rt_build_params p{branching, leaf, gang, sortEachSplit ? 1u : 0u, leafSplitOption, sahFactor, threads};
+    rt_build_params p{branching, leaf, gang, sortEachSplit ? 1u : 0u, leafSplitOption, sahFactor, threads,
+                      nodeMemory_, leafMemory_};
     check(rt_build(tris.data(), (uint32_t)(tris.size() / 9), &p, &host_), "rt_build");
     if (rt_host_bvh_desc(host_, &desc_) != RT_OK) {
         rt_host_bvh_free(host_);
@@ -764,20 +767,36 @@ float RayTracer::renderImage(unsigned branchingFactor, unsigned leafSize, unsign
     const std::string problem = "_b" + std::to_string(branchingFactor) + "_l" + std::to_string(leafSize);
     FastNodeManager m(tris, branchingFactor, leafSize, gangSize, sahFactor, cfg_.sortEachSplit, cfg_.leafSplitOption);
     const float ambientDistance = ambient_distance(tris);
-    const std::string prefix = "_mb" + std::to_string(m.nodeMemory) + "_ml" + std::to_string(m.leafMemory);
+    const unsigned samples = (unsigned)std::max(0, cfg_.ambientSampleCount);
     float total = 0;
     if (cfg_.doPerformanceTest) {
-        CameraFast cam(path_, name, problem, prefix, cfg_.workGroupSize, cfg_.saveDistance, cfg_.wideRender,
-                       cameraPositions, cameraTargets, (unsigned)cfg_.xRes, (unsigned)cfg_.yRes);
-        total = cam.renderImages(cfg_.saveImage, m, (unsigned)std::max(0, cfg_.ambientSampleCount), ambientDistance,
-                                 true, cfg_.wideAlternative)
-                    .first;
+        // leaf memory (outer) and node memory (inner) from the minimum up to
+        // 3 gangs more with the memory tests (rayTracer.cpp:41-52,630-640);
+        // this engine's records hold at most 16 slots
+        const unsigned minLm = m.leafMemory, minM = m.nodeMemory;
+        const unsigned maxLm = cfg_.doLeafMemoryTest ? std::min(16u, minLm + 3 * gangSize) : minLm;
+        const unsigned maxM = cfg_.doNodeMemoryTest ? std::min(16u, minM + 3 * gangSize) : minM;
+        for (unsigned lm = minLm; lm <= maxLm; lm += gangSize)
+            for (unsigned nm = minM; nm <= maxM; nm += gangSize) {
+                const bool base = lm == minLm && nm == minM;
+                std::unique_ptr<FastNodeManager> other;
+                if (!base)
+                    other.reset(new FastNodeManager(tris, branchingFactor, leafSize, gangSize, sahFactor,
+                                                    cfg_.sortEachSplit, cfg_.leafSplitOption, 0, nm, lm));
+                FastNodeManager& mm = base ? m : *other;
+                const std::string prefix = "_mb" + std::to_string(mm.nodeMemory) + "_ml" + std::to_string(mm.leafMemory);
+                CameraFast cam(path_, name, problem, prefix, cfg_.workGroupSize, cfg_.saveDistance, cfg_.wideRender,
+                               cameraPositions, cameraTargets, (unsigned)cfg_.xRes, (unsigned)cfg_.yRes);
+                const float t = cam.renderImages(cfg_.saveImage && base, mm, samples, ambientDistance, true,
+                                                 cfg_.wideAlternative)
+                                    .first;
+                if (base) total = t;
+            }
     }
     if (cfg_.renderAnalysisImage) {
         CameraData cam(path_, name, problem, cfg_.workGroupSize, cfg_.wideRender, cameraPositions, cameraTargets,
                        (unsigned)cfg_.xRes, (unsigned)cfg_.yRes);
-        cam.renderImages(m, (unsigned)std::max(0, cfg_.ambientSampleCount), ambientDistance, cfg_.doWorkGroupAnalysis,
-                         cfg_.wideAlternative);
+        cam.renderImages(m, samples, ambientDistance, cfg_.doWorkGroupAnalysis, cfg_.wideAlternative);
     }
     return total;
 }

@@ -72,7 +72,7 @@ class Bvh:
     """
 
     def __init__(self, tris, branching, leaf_size, gang=4, sort_each_split=True,
-                 leaf_split_option=1, sah_factor=None, threads=0):
+                 leaf_split_option=1, sah_factor=None, threads=0, node_memory=0, leaf_memory=0):
         tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
         if sah_factor is None:
             sah_factor = load_sah_factor(leaf_size, branching, gang)
@@ -80,7 +80,8 @@ class Bvh:
         self.sah_factor = float(sah_factor)
         self.tris = tris
         p = RtBuildParams(self.branching, self.leaf_size, self.gang, int(bool(sort_each_split)),
-                          int(leaf_split_option), ctypes.c_float(self.sah_factor).value, int(threads))
+                          int(leaf_split_option), ctypes.c_float(self.sah_factor).value, int(threads),
+                          int(node_memory), int(leaf_memory))
         h = ctypes.c_void_p()
         check(lib().rt_build(ptr(tris), tris.shape[0], ctypes.byref(p), ctypes.byref(h)))
         self._h = h

@@ -572,18 +572,40 @@ class RayTracer:
             scene.close()
             return 0.0
         mode = "save_distance" if cfg.get("saveDistance") else "closest"
-        total, primary = cam.render_images(scene, bvh, samples, ambient, mode=mode,
-                                           save_image=cfg.get("saveImage", False), path=self.out_dir, name=name)
-        prefix = f"_mb{bvh.node_memory}_ml{bvh.leaf_memory}"
         # cameraFast.h:134-147: <path>/WorkGroupSize_<wg>_<Normal|Wide|WideOld>/
         kind = ("Wide" if cfg.get("wideAlternative", True) else "WideOld") if cfg.get("wideRender") else "Normal"
         perf_dir = os.path.join(self.out_dir, f"WorkGroupSize_{cam.wg}_{kind}")
         os.makedirs(perf_dir, exist_ok=True)
-        write_perf(os.path.join(perf_dir, f"{name}{problem}{prefix}_Perf.txt"), name, problem, self.gang,
-                   bvh.node_memory, bvh.leaf_memory, branching, leaf_size, total, primary, 0.0,
-                   bvh.average_depth, bvh.triangle_count)
+        # leaf memory (outer) and node memory (inner) up to 3 gangs above the
+        # minimum with the memory tests (rayTracer.cpp:41-52,630-640), at most 16
+        g = self.gang
+        lms = range(bvh.leaf_memory, (min(16, bvh.leaf_memory + 3 * g) if cfg.get("doLeafMemoryTest") else
+                                      bvh.leaf_memory) + 1, g)
+        nms = range(bvh.node_memory, (min(16, bvh.node_memory + 3 * g) if cfg.get("doNodeMemoryTest") else
+                                      bvh.node_memory) + 1, g)
+        base_total = None
+        for lm in lms:
+            for nm in nms:
+                base = lm == bvh.leaf_memory and nm == bvh.node_memory
+                b2, s2 = bvh, None
+                if not base:
+                    b2 = Bvh(tris, branching, leaf_size, gang=g, sort_each_split=cfg.get("sortEachSplit", True),
+                             leaf_split_option=cfg.get("leafSplitOption", 1), sah_factor=sah, node_memory=nm,
+                             leaf_memory=lm)
+                    s2 = Scene(b2)
+                total, primary = cam.render_images(s2 or scene, b2, samples, ambient, mode=mode,
+                                                   save_image=cfg.get("saveImage", False) and base,
+                                                   path=self.out_dir, name=name)
+                prefix = f"_mb{b2.node_memory}_ml{b2.leaf_memory}"
+                write_perf(os.path.join(perf_dir, f"{name}{problem}{prefix}_Perf.txt"), name, problem, g,
+                           b2.node_memory, b2.leaf_memory, branching, leaf_size, total, primary, 0.0,
+                           b2.average_depth, b2.triangle_count)
+                if s2 is not None:
+                    s2.close()
+                if base:
+                    base_total = total
         scene.close()
-        return total
+        return base_total
 
     def run(self, scenes=("spheres",)):
         results = {}

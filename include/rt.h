@@ -325,6 +325,10 @@ typedef struct rt_build_params {
     int32_t leaf_split_option;  /* config.txt leafSplitOption (1) */
     float sah_factor;           /* loadSahFactor(L, N, G)         */
     uint32_t threads;           /* 0 = all cores                  */
+    uint32_t node_memory;       /* 0 = roundup(N, G); else a larger multiple of G,
+                                   the node memory test (rayTracer.cpp:41-52)   */
+    uint32_t leaf_memory;       /* 0 = roundup(L, G); else a larger multiple of G,
+                                   the leaf memory test (rayTracer.cpp:630-640) */
 } rt_build_params;
 
 typedef struct rt_host_bvh* rt_host_bvh_t;

@@ -75,8 +75,11 @@ class FastNodeManager {
    public:
     // Bvh(prims, N, L, sortEachSplit, leafSplitOption) + recursiveBvh(sahFactor)
     // + the FastNodeManager flattener (rt_build), uploaded to the current device.
+    // nodeMemory / leafMemory: 0 = roundup(N, G) / roundup(L, G), or a larger
+    // multiple of G (the node and leaf memory tests).
     FastNodeManager(const std::vector<float>& tris, unsigned branchingFactor, unsigned leafSize, unsigned gangSize,
-                    float sahFactor, bool sortEachSplit = true, int leafSplitOption = 1, unsigned threads = 0);
+                    float sahFactor, bool sortEachSplit = true, int leafSplitOption = 1, unsigned threads = 0,
+                    unsigned nodeMemory = 0, unsigned leafMemory = 0);
     ~FastNodeManager();
     FastNodeManager(const FastNodeManager&) = delete;
     FastNodeManager& operator=(const FastNodeManager&) = delete;

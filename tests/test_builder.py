@@ -171,3 +171,33 @@ def test_root_leaf_and_tiny_inputs(rt):
         rt.Bvh(np.zeros((0, 9), np.float32), 4, 4)
     with pytest.raises(rt.RtError):
         rt.Bvh(one, 4, 4, gang=3)
+
+
+def test_memory_overrides(rt, orc):
+    """Node / leaf memory above the minimum (the node and leaf memory tests,
+    rayTracer.cpp:41-52,630-640): the same tree in records of M / Lm slots
+    with the loop-count markers.  The closest t is the same; the visit order
+    may change, because a node whose child count is below M iterates the
+    zero padding of its order row first (the padding quirk), so node counts
+    and ties can differ (the GPU parity tests cover these trees)."""
+    tris = rt.interior_scene(9, 20_000, 12)
+    rays = rt.camera_rays((0.0, 5.0, -14.0), (0.0, 3.0, 0.0), 64, 48, 16)
+    base = rt.Bvh(tris, 4, 4)
+    ref = orc.Oracle.from_bvh(base).trace(rays)
+    for nm, lm in ((8, 4), (4, 12), (16, 16)):
+        b = rt.Bvh(tris, 4, 4, node_memory=nm, leaf_memory=lm)
+        assert (b.node_memory, b.leaf_memory) == (nm, lm)
+        assert np.array_equal(b.base[~b.is_leaf()], base.base[~base.is_leaf()])
+        assert len(b.tri_points) == len(base.tri_points) // base.leaf_memory * lm
+        r = orc.Oracle.from_bvh(b).trace(rays)
+        assert np.array_equal(r["hits"]["slot"] >= 0, ref["hits"]["slot"] >= 0)
+        assert np.array_equal(r["hits"]["t"], ref["hits"]["t"]), (nm, lm)
+        if nm == base.node_memory:
+            # same visit order: the same leaves in the same order, so the same
+            # hits and counts, at the larger leaf block stride
+            assert np.array_equal(r["hits"]["leaf"] // (9 * lm), ref["hits"]["leaf"] // (9 * base.leaf_memory))
+            for f in ("nodes", "leaves", "node_slots", "leaf_slots"):
+                assert np.array_equal(r["counts"][f], ref["counts"][f]), (nm, lm, f)
+    for bad in (dict(node_memory=6), dict(leaf_memory=2), dict(leaf_memory=20)):
+        with pytest.raises(rt.RtError):
+            rt.Bvh(tris, 4, 4, **bad)

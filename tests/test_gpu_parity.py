@@ -300,3 +300,32 @@ def test_wave_schedules(rt, sched, N, L):
     g, gc = scene.intersect(rays, mode="save_distance", counts=True)
     ref = oracle_for(bvh).trace(rays, mode=oracle.ORC_SAVE_DIST)
     assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], f"sched {sched} save-distance")
+
+
+@pytest.mark.parametrize("N,L,nm,lm", [(4, 4, 8, 4), (4, 4, 16, 16), (8, 4, 12, 8), (5, 3, 12, 12)])
+def test_memory_test_layouts(rt, N, L, nm, lm):
+    """Trees flattened with node / leaf memory above the minimum (the memory
+    tests of rayTracer.cpp:41-52,630-640): the padded order rows change the
+    visit order (padding quirk), and the GPU follows the oracle bit for bit
+    in all three traversals."""
+    tris = rt.interior_scene(13, 40_000, 25)
+    bvh = rt.Bvh(tris, N, L, node_memory=nm, leaf_memory=lm)
+    assert (bvh.node_memory, bvh.leaf_memory) == (nm, lm)
+    prim = rt.camera_rays((0.0, 5.0, -14.0), (0.0, 3.0, 0.0), 96, 64, 16)
+    scene = rt.Scene(bvh)
+    hits = scene.intersect(prim)
+    nrm, pos = bvh.surface(prim, hits)
+    rays = np.concatenate([prim, rt.spawn_diffuse(prim, nrm, pos, hits, 8000, seed=4)])
+    print(f"memory test N{N}L{L} M{nm} Lm{lm}: {len(rays)} rays", flush=True)
+    o = oracle_for(bvh)
+    for mode, omode in (("closest", oracle.ORC_CLOSEST), ("save_distance", oracle.ORC_SAVE_DIST)):
+        g, gc = scene.intersect(rays, mode=mode, counts=True)
+        ref = o.trace(rays, mode=omode)
+        assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], f"{mode} M{nm} Lm{lm}")
+    srays, _ = rt.spawn_shadow(nrm, pos, hits, (0.0, 9.5, 0.0), 1.0)
+    occ, oc = scene.intersect(srays, mode="any", counts=True)
+    ro = o.trace(srays, mode=oracle.ORC_ANY)
+    assert np.array_equal(occ, ro["occluded"])
+    for f in ("nodes", "leaves", "node_slots", "leaf_slots"):
+        assert np.array_equal(oc[f], ro["counts"][f]), f
+    scene.close()

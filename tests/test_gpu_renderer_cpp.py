@@ -89,3 +89,18 @@ def test_bvhrt_analysis_matches_python(rt, tmp_path):
         py = (tmp_path / "py" / n).read_text()
         assert len(py.splitlines()) > 5
         assert (tmp_path / "cpp" / n).read_text() == py, n
+
+
+def test_bvhrt_memory_tests(rt, tmp_path):
+    """doLeafMemoryTest / doNodeMemoryTest: one _Perf.txt per (node memory,
+    leaf memory) from the minimum up to 3 gangs more, at most 16."""
+    cfg = tmp_path / "config.txt"
+    cfg.write_text("xRes = 32\nyRes = 32\nworkGroupSize = 16\nminLeafSize = 8\nmaxLeafSize = 8\nleafStep = 4\n"
+                   "minBranch = 12\nmaxBranch = 12\nbranchStep = 4\nambientSampleCount = 1\n"
+                   "doPerformanceTest = true\ndoLeafMemoryTest = true\ndoNodeMemoryTest = true\nscenario = 0\n")
+    print("bvhrt memory tests N12 L8", flush=True)
+    r = subprocess.run([BVHRT, str(cfg), str(tmp_path / "out")], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    files = sorted(os.listdir(tmp_path / "out" / "WorkGroupSize_16_Normal"))
+    want = sorted(f"spheres_b12_l8_mb{m}_ml{l}_Perf.txt" for l in (8, 12, 16) for m in (12, 16))
+    assert files == want
