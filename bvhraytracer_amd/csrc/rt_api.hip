@@ -437,6 +437,7 @@ extern "C" int rt_hit_surface_device(rt_scene_t s, const rt_ray* rays, const rt_
 // The stream rt_trace uses for a scene when the caller passes none.
 int rt::scene_stream(rt_scene_t s, hipStream_t* out) {
     std::lock_guard<std::mutex> lk(s->mu);
+    HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
     if (!s->ownStream) HIP_TRY(hipStreamCreateWithFlags(&s->ownStream, hipStreamNonBlocking), "hipStreamCreate");
     const int rc = null_stream_before(s->ownStream, s->joinEv);
     if (rc != RT_OK) return rc;

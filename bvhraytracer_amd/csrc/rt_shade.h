@@ -14,8 +14,9 @@ hipError_t launch_surface(const float* tris, const uint8_t* leafLc, uint32_t Lm,
                           hipStream_t stream);
 
 // The stream rt_trace uses for scene s when the caller passes none (created
-// on first use); calls that enqueue several steps use it too, so every step
-// runs on one stream.
+// on first use, ordered after the null stream's queued work); calls that
+// enqueue several steps use it too, so every step runs on one stream.  Makes
+// the scene's device current.
 int scene_stream(rt_scene_t s, hipStream_t* out);
 
 }  // namespace rt

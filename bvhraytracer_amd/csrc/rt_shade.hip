@@ -255,6 +255,12 @@ extern "C" int rt_render_ambient(rt_scene_t s, const rt_camera* cam, uint32_t wi
     // the scene when none is given (not the null stream, which the scene's
     // non-blocking stream does not wait for)
     hipStream_t st = static_cast<hipStream_t>(hip_stream);
+    int prevDev = 0;
+    (void)hipGetDevice(&prevDev);
+    struct Restore {
+        int prev;
+        ~Restore() { (void)hipSetDevice(prev); }
+    } restore{prevDev};
     if (!st) {
         const int r = scene_stream(s, &st);
         if (r != RT_OK) return r;
