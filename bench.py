@@ -221,6 +221,27 @@ def bench_frame(rt, torch, scene, bvh, n_bounce, log, reps=3):
     return out
 
 
+def bench_copy(torch, log, nbytes=1 << 30, reps=5):
+    """STREAM-like device copy of nbytes (read nbytes + write nbytes), the
+    HBM rate a plain streaming kernel reaches on this GPU: median of reps
+    after a warm-up, HIP events."""
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    ms = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        e1.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    gbps = 2 * nbytes / (float(np.median(ms)) / 1000.0) / 1e9
+    log(f"device copy: {gbps:.0f} GB/s (read + write of {nbytes} bytes)")
+    del a, b
+    return {"bytes_read_plus_written": 2 * nbytes, "gbps": round(gbps, 1)}
+
+
 def bench_render_frame(rt, scene, bvh, tris, log, reps=3):
     """One frame of the reference's performance renderer (CameraFast::
     renderImage: 1920x1088 primaries, closest hit, one ambient-occlusion ray
@@ -435,6 +456,7 @@ def main():
 
     frame = None if args.no_frame else bench_frame(rt, torch, scene, bvh, n, log)
     render_frame = None if args.no_frame else bench_render_frame(rt, scene, bvh, tris, log)
+    copy = bench_copy(torch, log)
     batched = None if args.no_batch else bench_batched(rt, torch, args, log)
     serving = None if args.no_serve else bench_serving(rt, args, log)
 
@@ -454,11 +476,20 @@ def main():
             r = o.trace(rays[:sample], counts=False)
             times.append(time.perf_counter() - t)
         cpu_rate = sample / float(np.median(times)) / 1e6
+        # the stream form (intersectWideAlternative per 256-ray tile) beside it
+        st = []
+        for _ in range(3):
+            t = time.perf_counter()
+            o.trace(rays[:sample], stream=True, counts=False)
+            st.append(time.perf_counter() - t)
+        stream_rate = sample / float(np.median(st)) / 1e6
         cpu = {"value": round(cpu_rate, 3), "unit": "Mrays/s", "cores": int(r["threads"]),
-               "kind": "port",
+               "kind": "port", "stream_value": round(stream_rate, 3),
                "sample": f"first {sample} of the {n} diffuse rays, closest-hit per-ray traversal, "
-                         f"1 warm-up + 5 timed, median; {cpu_model()}"}
-        log(f"cpu baseline {cpu_rate:.3f} Mrays/s on {r['threads']} threads ({sample} rays)")
+                         f"1 warm-up + 5 timed, median (stream_value: the stream traversal per "
+                         f"256-ray tile, 3 timed, median); {cpu_model()}"}
+        log(f"cpu baseline {cpu_rate:.3f} Mrays/s per-ray, {stream_rate:.3f} stream, on {r['threads']} threads "
+            f"({sample} rays)")
 
     out = {
         "metric": METRIC,
@@ -484,6 +515,7 @@ def main():
         "cpu_baseline": cpu,
         "bounce_frame": frame,
         "render_frame": render_frame,
+        "hbm_copy": copy,
         "batched": batched,
         "serving": serving,
     }
