@@ -42,8 +42,10 @@ def spheres(rt):
 
 
 @pytest.mark.parametrize("NL", [(4, 4), (8, 8), (16, 16)])
-@pytest.mark.parametrize("tile", [64, 256, 1024])
+@pytest.mark.parametrize("tile", [16, 64, 100, 256, 1024])
 def test_primary_tiles(rt, spheres, NL, tile):
+    """Tiles of wg x wg rays, including blocks smaller than a wave (4x4) and
+    not a multiple of it (10x10)."""
     scene, bvh = spheres[NL]
     wg = int(round(tile ** 0.5))
     rays = rt.camera_rays((0.0, 4.0, 32.0), (0.0, 0.0, 0.0), 4 * wg, 2 * wg, wg)
