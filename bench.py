@@ -454,6 +454,17 @@ def main():
         pms.append(scene.stats()["last_kernel_ms"])
     primary_mrays = len(prim) / (float(np.median(pms)) / 1000.0) / 1e6
 
+    # the same 8M rays from host memory (rt_trace without RT_DEVICE_PTRS: rays
+    # copied in and hits copied out over PCIe in the call); never `value`
+    scene.intersect(rays)
+    hts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        scene.intersect(rays)
+        hts.append(time.perf_counter() - t0)
+    host_ptr_mrays = n / float(np.median(hts)) / 1e6
+    log(f"host-pointer rt_trace (PCIe in and out): {host_ptr_mrays:.1f} Mrays/s")
+
     frame = None if args.no_frame else bench_frame(rt, torch, scene, bvh, n, log)
     render_frame = None if args.no_frame else bench_render_frame(rt, scene, bvh, tris, log)
     copy = bench_copy(torch, log)
@@ -509,7 +520,7 @@ def main():
                    "rays": n, "triangles": int(len(tris)), "branching": N, "leaf_size": L,
                    "hit_fraction": round(hits_frac, 4),
                    "primary_mrays_1920x1080": round(primary_mrays, 2),
-                   "kernel_ms": round(k_ms, 4)},
+                   "host_pointer_mrays": round(host_ptr_mrays, 2), "kernel_ms": round(k_ms, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
