@@ -131,6 +131,21 @@ def bench_batched(rt, torch, args, log):
            "n_l": f"N{args.branching}L{args.leaf}", "graph": True}
     if args.batch_autotune:
         out["autotuned"] = batched_autotuned(rt, torch, host_rays, scenes, outs, stream, total, args, log)
+    if not args.no_cpu:
+        # the CPU baseline of this workload: the oracle per scene, summed
+        # (one warm-up pass, then one timed pass over all 64 scenes)
+        import oracle
+
+        orcs = [(oracle.Oracle.from_bvh(bvh), rays) for (bvh, _, _, _), (_, rays) in zip(scenes, host_rays)]
+        for o, rays in orcs:
+            o.trace(rays, counts=False)
+        t = time.perf_counter()
+        for o, rays in orcs:
+            used = o.trace(rays, counts=False)["threads"]
+        cpu_s = time.perf_counter() - t
+        out["cpu_port_mrays"] = round(total / cpu_s / 1e6, 3)
+        out["cpu_threads"] = int(used)
+        log(f"batched workload on the CPU oracle: {out['cpu_port_mrays']} Mrays/s on {used} threads")
     for bvh, sc, d_r, d_h in scenes:
         sc.close()
     return out
