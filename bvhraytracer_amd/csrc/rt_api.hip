@@ -577,6 +577,8 @@ namespace {
 int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, void* out, rt_counts* counts,
                void* hip_stream, rt_trace_hist* hist) {
     if (!s) return set_error(RT_ERR_ARG, "rt_trace: null scene");
+    if (flags & ~(RT_ANY | RT_SAVE_DIST | RT_STREAM | RT_COUNTS | RT_DEVICE_PTRS | RT_NO_SYNC | RT_COOP))
+        return set_error(RT_ERR_ARG, "rt_trace: unknown flag bits");
     if (n == 0) return RT_OK;
     if (!rays || !out) return set_error(RT_ERR_ARG, "rt_trace: null rays or output");
     if ((flags & RT_COUNTS) && !counts) return set_error(RT_ERR_ARG, "rt_trace: RT_COUNTS without a counts buffer");
@@ -895,6 +897,7 @@ extern "C" int rt_batch_set_launch(rt_batch_t b, uint32_t block_size, uint32_t s
 extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t n_items, uint32_t flags,
                               void* hip_stream) {
     if (!b) return set_error(RT_ERR_ARG, "rt_batch_trace: null batch");
+    if (flags & ~(RT_GRAPH | RT_NO_SYNC)) return set_error(RT_ERR_ARG, "rt_batch_trace: unknown flag bits");
     if (n_items == 0) return RT_OK;
     if (!items) return set_error(RT_ERR_ARG, "rt_batch_trace: null items");
     if (n_items > b->maxItems) return set_error(RT_ERR_ARG, "rt_batch_trace: more items than the batch was created for");

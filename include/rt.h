@@ -134,7 +134,11 @@ int rt_upload_bvh(const rt_bvh_desc* desc, rt_scene_t* out);
  * work starts after what is already queued on the null stream, and with
  * RT_NO_SYNC later null-stream work waits for it (it runs on a stream of
  * the library's).  With
- * RT_DEVICE_PTRS the buffers stay on the device; otherwise they are staged. */
+ * RT_DEVICE_PTRS the buffers stay on the device; otherwise they are staged.
+ * n == 0 returns RT_OK without reading any buffer.  Flag bits other than the
+ * RT_* trace flags above, RT_ANY with RT_SAVE_DIST, and RT_COUNTS without a
+ * counts buffer return RT_ERR_ARG; RT_SAVE_DIST with RT_STREAM or RT_COOP,
+ * and RT_COOP with RT_STREAM, return RT_ERR_UNSUPPORTED. */
 int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags,
              void* out, rt_counts* counts, void* hip_stream);
 
@@ -247,7 +251,9 @@ typedef struct rt_batch_item {
 
 int rt_batch_create(uint32_t max_items, rt_batch_t* out);
 /* Items must all live on the batch's device; total rays < 2^31.  A NULL
- * hip_stream keeps the null stream's order, as for rt_trace. */
+ * hip_stream keeps the null stream's order, as for rt_trace.  flags takes
+ * RT_GRAPH and RT_NO_SYNC (other bits: RT_ERR_ARG); items with n == 0 are
+ * skipped. */
 int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t n_items, uint32_t flags,
                    void* hip_stream);
 int rt_batch_set_launch(rt_batch_t b, uint32_t block_size, uint32_t stack_window);
