@@ -448,7 +448,9 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
         t.live &= ~(1u << s);
     };
     // entries pushed after the first land on the stack
-    auto fits = [&](uint32_t npush) { return npush == 0 || t.sp + npush - 1u <= sk.cap; };
+    // the highest stack position a node's pushes write is sp + npush - 1
+    // (the previous pending entry first); positions run up to cap - 1
+    auto fits = [&](uint32_t npush) { return npush == 0 || t.sp + npush <= sk.cap; };
     // The push sequence is the order row filtered by the live mask.  Each
     // slot occurs once in the row, so the positions to push are known before
     // the first push: the loops below run once per push, not once per slot.
