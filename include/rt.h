@@ -138,7 +138,11 @@ int rt_upload_bvh(const rt_bvh_desc* desc, rt_scene_t* out);
  * n == 0 returns RT_OK without reading any buffer.  Flag bits other than the
  * RT_* trace flags above, RT_ANY with RT_SAVE_DIST, and RT_COUNTS without a
  * counts buffer return RT_ERR_ARG; RT_SAVE_DIST with RT_STREAM or RT_COOP,
- * and RT_COOP with RT_STREAM, return RT_ERR_UNSUPPORTED. */
+ * and RT_COOP with RT_STREAM, return RT_ERR_UNSUPPORTED.
+ * Any number of host threads may call into one scene at once (the ISPC
+ * kernels this replaces are called from OpenMP workers, cameraFast.h:320,465):
+ * the scene serialises its host-side work, each hip_stream gets its own
+ * scratch, and rt_last_error() reports the calling thread's own error. */
 int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags,
              void* out, rt_counts* counts, void* hip_stream);
 
