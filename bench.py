@@ -469,6 +469,29 @@ def main():
         pms.append(scene.stats()["last_kernel_ms"])
     primary_mrays = len(prim) / (float(np.median(pms)) / 1000.0) / 1e6
 
+    # every traversal of the boundary on the same scene (kernel time by HIP
+    # events, median of 3 after a warm-up): save distance, ray stream and
+    # cooperative on the 8M diffuse rays, any hit on the primaries' shadow rays
+    def kernel_mrays(d_in, d_out, count, **kw):
+        scene.intersect_device(d_in, d_out, stream=stream, **kw)
+        ks = []
+        for _ in range(3):
+            scene.intersect_device(d_in, d_out, stream=stream, **kw)
+            ks.append(scene.stats()["last_kernel_ms"])
+        return round(count / (float(np.median(ks)) / 1000.0) / 1e6, 1)
+
+    variants = {"closest_per_ray": round(n / (k_ms / 1000.0) / 1e6, 1),
+                "save_distance": kernel_mrays(d_rays, d_hits, n, mode="save_distance"),
+                "closest_stream": kernel_mrays(d_rays, d_hits, n, stream_sched=True),
+                "closest_cooperative": kernel_mrays(d_rays, d_hits, n, coop=True)}
+    shadow, _ = rt.spawn_shadow(nrm, pos, phits, (0.0, 9.5, 0.0), 1.0)
+    d_sh = torch.from_numpy(shadow.view(np.float32).reshape(-1, 10)).to(dev)
+    d_occ = torch.empty(len(shadow), dtype=torch.uint8, device=dev)
+    variants["shadow_any_hit"] = kernel_mrays(d_sh, d_occ, len(shadow), mode="any")
+    variants["shadow_any_hit_stream"] = kernel_mrays(d_sh, d_occ, len(shadow), mode="any", stream_sched=True)
+    log(f"traversal variants (Mrays/s, kernel time): {variants}")
+    del d_sh, d_occ
+
     # the same 8M rays from host memory (rt_trace without RT_DEVICE_PTRS: rays
     # copied in and hits copied out over PCIe in the call); never `value`
     scene.intersect(rays)
@@ -539,6 +562,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
+        "traversals_mrays": variants,
         "bounce_frame": frame,
         "render_frame": render_frame,
         "hbm_copy": copy,
