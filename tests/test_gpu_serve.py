@@ -88,3 +88,30 @@ def test_admission_cancel_and_errors(rt, world):
     ref = sc.intersect(rays[:9000])
     assert np.array_equal(subs[0][1].view(np.uint8), ref.view(np.uint8))
     srv.close()
+
+
+def test_wait_timeout_and_close_in_flight(rt, world):
+    """A zero-timeout wait right after submitting a 512k-ray request (20 MB of
+    rays to copy in) times out with RT_ERR_BUSY and leaves the request alive;
+    a later wait collects it (results equal rt_trace alone) and a third wait
+    finds the id gone.  Closing the server with requests queued and in flight
+    returns: launched work is waited for, queued requests are dropped."""
+    bvh, sc, rays, _ = world[0]
+    big = np.resize(rays, 1 << 19)
+    srv = rt.Server(hbm_cap_bytes=1 << 30, slots=2)
+    sid = srv.add_scene(sc)
+    rid, out = srv.submit(sid, big)
+    with pytest.raises(rt.RtError) as e:
+        srv.wait(rid, timeout=0.0)
+    assert e.value.code == -6
+    srv.wait(rid, timeout=60)
+    alone = sc.intersect(big)
+    assert np.array_equal(out.view(np.uint8), alone.view(np.uint8))
+    with pytest.raises(rt.RtError) as e:
+        srv.wait(rid, timeout=0.0)
+    assert e.value.code == -1
+    for _ in range(6):
+        srv.submit(sid, big)
+    srv.close()
+    # the scene outlives the server and still traces as before
+    assert np.array_equal(sc.intersect(big[:5000]).view(np.uint8), alone[:5000].view(np.uint8))
