@@ -310,6 +310,9 @@ int rt_server_wait(rt_server_t sv, uint64_t request_id, double timeout_s);
 int rt_server_cancel(rt_server_t sv, uint64_t request_id);
 int rt_server_drain(rt_server_t sv);
 int rt_server_get_stats(rt_server_t sv, rt_server_stats* out, int reset);
+/* Stops the dispatcher, waits for launched work and drops queued requests
+ * (their output buffers are not written).  No other thread may be inside an
+ * rt_server_* call on this server when it is called. */
 int rt_server_destroy(rt_server_t sv);
 
 /* ---- host side: scenes, builder, camera, config ------------------------------- */
