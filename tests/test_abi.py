@@ -95,3 +95,12 @@ int main(int argc, char** argv) {
                     "-Wl,-rpath," + os.path.dirname(lib), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe), str(tmp_path / "config.txt")], capture_output=True, text=True, check=True)
     assert out.stdout.split("\n")[:2] == ["4,4 4,8 8,4 8,8 169 20", "error"]
+
+
+def test_integration_table_names_every_entry_point():
+    """INTEGRATION.md's entry-point table names every function rt.h declares
+    (each with the reference interface it replaces, or "no counterpart")."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    table = text[text.index("## 4. Entry points"):]
+    missing = [n for n in declared_functions() if f"`{n}`" not in table]
+    assert not missing, missing
