@@ -69,7 +69,22 @@ struct Workspace {
     uint64_t* hist = nullptr;  // rt_trace_hist counters of the instrumented launch
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
+    uint64_t lastUse = 0;  // scene's call counter at the last use
 };
+
+// A scene keeps the scratch of at most this many streams; the least recently
+// used one is freed (after its last launch) when another stream arrives.
+constexpr size_t kMaxWorkspaces = 16;
+
+void free_workspace(Workspace& w) {
+    if (w.work) (void)hipFree(w.work);
+    if (w.spill) (void)hipFree(w.spill);
+    if (w.stage) (void)hipFree(w.stage);
+    if (w.hist) (void)hipFree(w.hist);
+    if (w.ev0) (void)hipEventDestroy(w.ev0);
+    if (w.ev1) (void)hipEventDestroy(w.ev1);
+    w = Workspace();
+}
 
 struct LayoutInfo {
     uint32_t inner = 0, leaves = 0, maxDepth = 0;
@@ -117,6 +132,7 @@ struct rt_scene {
     uint32_t blockSize = 256, stackWindow = 16;
     uint32_t cus = 0;
     std::unordered_map<hipStream_t, Workspace> ws;
+    uint64_t useClock = 0;
     hipStream_t ownStream = nullptr;
     hipEvent_t joinEv = nullptr;  // orders ownStream with the null stream
     float lastMs = 0;
@@ -285,15 +301,7 @@ void free_scene(rt_scene* s) {
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(s->device);
-    for (auto& kv : s->ws) {
-        Workspace& w = kv.second;
-        if (w.work) (void)hipFree(w.work);
-        if (w.spill) (void)hipFree(w.spill);
-        if (w.stage) (void)hipFree(w.stage);
-        if (w.hist) (void)hipFree(w.hist);
-        if (w.ev0) (void)hipEventDestroy(w.ev0);
-        if (w.ev1) (void)hipEventDestroy(w.ev1);
-    }
+    for (auto& kv : s->ws) free_workspace(kv.second);
     if (s->nodes) (void)hipFree(s->nodes);
     if (s->tris) (void)hipFree(s->tris);
     if (s->triIds) (void)hipFree(s->triIds);
@@ -326,7 +334,23 @@ uint32_t ray_chunk(uint64_t n, uint64_t grid, uint32_t bd) {
 uint32_t stack_capacity(const rt_scene* s) { return ((1 + (s->maxDepth + 1) * s->M + 8) + 7) / 8 * 8; }
 
 int get_workspace(rt_scene* s, hipStream_t st, Workspace*& out) {
-    Workspace& w = s->ws[st];
+    auto it = s->ws.find(st);
+    if (it == s->ws.end()) {
+        if (s->ws.size() >= kMaxWorkspaces) {
+            // the calls of a scene are serialised by its lock, so no other call
+            // uses the evicted scratch; its last launch ends at ev1 (host-pointer
+            // calls wait for their copies before returning)
+            auto lru = s->ws.begin();
+            for (auto j = s->ws.begin(); j != s->ws.end(); ++j)
+                if (j->second.lastUse < lru->second.lastUse) lru = j;
+            if (lru->second.timed) HIP_TRY(hipEventSynchronize(lru->second.ev1), "hipEventSynchronize(scratch)");
+            free_workspace(lru->second);
+            s->ws.erase(lru);
+        }
+        it = s->ws.emplace(st, Workspace()).first;
+    }
+    Workspace& w = it->second;
+    w.lastUse = ++s->useClock;
     if (!w.work) {
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.work), kWorkWords * 4), "hipMalloc(work)");
         HIP_TRY(hipEventCreate(&w.ev0), "hipEventCreate");

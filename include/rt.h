@@ -142,7 +142,9 @@ int rt_upload_bvh(const rt_bvh_desc* desc, rt_scene_t* out);
  * Any number of host threads may call into one scene at once (the ISPC
  * kernels this replaces are called from OpenMP workers, cameraFast.h:320,465):
  * the scene serialises its host-side work, each hip_stream gets its own
- * scratch, and rt_last_error() reports the calling thread's own error. */
+ * scratch, and rt_last_error() reports the calling thread's own error.  A
+ * scene keeps the scratch of at most 16 streams; when another stream
+ * arrives, the least recently used one is freed after its last launch. */
 int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags,
              void* out, rt_counts* counts, void* hip_stream);
 

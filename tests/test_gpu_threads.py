@@ -111,3 +111,41 @@ def test_last_error_is_per_thread(rt, world):
         assert np.array_equal(hits, alone["closest"][0]), i
 
     _run(work)
+
+
+def test_many_streams_keep_bounded_scratch(rt, world):
+    """48 different streams on one scene, each call returning before its
+    kernel ends (RT_NO_SYNC): the scene keeps the scratch of 16 streams and
+    frees the least recently used one after its last launch, so every result
+    is intact and the library's device memory stays bounded."""
+    import torch
+
+    scene, rays, alone = world
+    n = len(rays)
+    d_rays = torch.from_numpy(rays.view(np.float32).reshape(-1, 10)).cuda()
+    ref = torch.from_numpy(alone["closest"][0].view(np.int32).reshape(n, 6)).cuda()
+
+    def library_bytes():
+        torch.cuda.synchronize()
+        free, _ = torch.cuda.mem_get_info()
+        return -free - torch.cuda.memory_reserved()
+
+    base = library_bytes()
+    first = torch.cuda.Stream()
+    out0 = torch.empty((n, 6), dtype=torch.int32, device="cuda")
+    scene.intersect_device(d_rays, out0, stream=first)
+    one = library_bytes() - base  # the scratch of one new stream
+    outs = []
+    print(f"48 streams, {one / 1e6:.1f} MB of scratch per stream", flush=True)
+    for i in range(48):
+        st = torch.cuda.Stream()
+        st.wait_stream(torch.cuda.current_stream())
+        out = torch.empty((n, 6), dtype=torch.int32, device="cuda")
+        scene.intersect_device(d_rays, out, stream=st, sync=False)
+        outs.append(out)
+    torch.cuda.synchronize()
+    grown = library_bytes() - base
+    assert torch.equal(out0, ref)
+    for i, out in enumerate(outs):
+        assert torch.equal(out, ref), i
+    assert grown <= 17 * max(one, 1 << 20), (grown, one)
