@@ -336,18 +336,22 @@ uint32_t stack_capacity(const rt_scene* s) { return ((1 + (s->maxDepth + 1) * s-
 int get_workspace(rt_scene* s, hipStream_t st, Workspace*& out) {
     auto it = s->ws.find(st);
     if (it == s->ws.end()) {
+        Workspace reuse;
         if (s->ws.size() >= kMaxWorkspaces) {
             // the calls of a scene are serialised by its lock, so no other call
             // uses the evicted scratch; its last launch ends at ev1 (host-pointer
-            // calls wait for their copies before returning)
+            // calls wait for their copies before returning).  Its buffers and
+            // events pass to the new stream instead of being freed, so streams
+            // used in rotation do not reallocate on every call.
             auto lru = s->ws.begin();
             for (auto j = s->ws.begin(); j != s->ws.end(); ++j)
                 if (j->second.lastUse < lru->second.lastUse) lru = j;
             if (lru->second.timed) HIP_TRY(hipEventSynchronize(lru->second.ev1), "hipEventSynchronize(scratch)");
-            free_workspace(lru->second);
+            reuse = lru->second;
+            reuse.timed = false;
             s->ws.erase(lru);
         }
-        it = s->ws.emplace(st, Workspace()).first;
+        it = s->ws.emplace(st, reuse).first;
     }
     Workspace& w = it->second;
     w.lastUse = ++s->useClock;
@@ -360,12 +364,14 @@ int get_workspace(rt_scene* s, hipStream_t st, Workspace*& out) {
     return RT_OK;
 }
 
+// Grows a device buffer to at least `need` bytes, by at least half its old
+// size, so a sequence of slowly growing calls reallocates O(log) times.
 int ensure(void** p, size_t& have, size_t need, const char* what) {
     if (have >= need) return RT_OK;
+    const size_t grow = std::max(need, have + have / 2);
     if (*p) (void)hipFree(*p);
     *p = nullptr;
     have = 0;
-    const size_t grow = std::max(need, have + have / 2);
     HIP_TRY(hipMalloc(p, grow), what);
     have = grow;
     return RT_OK;

@@ -782,7 +782,8 @@ __global__ void __launch_bounds__(1024) k_trace_steps(KParams p) {
     const SceneCtx sc = plain_scene(p, MT);
     const uint32_t tiles = p.n / bd;
     const bool lane0 = __lane_id() == 0;
-    if (tid < 5) ctr[tid] = 0;
+    // blocks may have fewer than 5 threads (tiles of 1..4 rays)
+    for (uint32_t w = tid; w < 5; w += bd) ctr[w] = 0;
     __syncthreads();
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const uint32_t r = tile * bd + tid;

@@ -111,3 +111,42 @@ def stale_tree():
         {"tris": far2, "ids": [2]},
     ] + [{"tris": l, "ids": [30 + i]} for i, l in enumerate(a_leaves)]
     return build(4, 4, 4, spec)
+
+
+def marker_tree():
+    """M = 8, G = 4, Lm = 8: loop counts come from the NaN markers.
+    Node 0 (root, 3 children -> nodeRealSize 4 != M, marker, k = 4):
+      slot 0 = node 1, inner, box x [3, 8], y/z [0, 4]
+      slot 1 = node 2, leaf A, 3 triangles (leafRealSize 4, marker, k = 4)
+               in the plane y = 5.5 region: box x [1, 2], y/z [0, 6]; the
+               triangles lie at y, z in [5, 6], off every test ray
+      slot 2 = node 3, leaf B, 5 copies of one triangle at x = 9.5 covering
+               y + z <= 4 (leafRealSize 8 == Lm, no marker, k = 8)
+      order x [1, 0, 2] (keys 3.5, 11.5, 19.5); y, z [0, 2, 1] (4.5, 4.5 tie
+      broken by index, then 6.5)
+    Node 1 (6 children -> nodeRealSize 8 == M, no marker, k = 8):
+      slots 0..4 = nodes 4..8, leaves of one triangle at x = 3, 4, 5, 6, 7
+               covering y + z <= 1 (leafRealSize 4, marker, k = 4)
+      slot 5 = node 9, leaf of 6 triangles at x = 7.5 (y + z <= 1,
+               leafRealSize 8, no marker, k = 8)
+      order rows [0, 1, 2, 3, 4, 5] on every axis (y/z keys all tie)
+    Leaf blocks in FastNode order: A = 0, B = 1, nodes 4..9 = 2..7, so a
+    leaf's primIdBegin is 72 * block."""
+    def wall(x, size):
+        return [x, 0, 0, x, size, 0, x, 0, size]
+
+    def box(x0, x1, hi):
+        return (np.float32([x0, 0, 0]), np.float32([x1, hi, hi]))
+
+    a_tris = [[1.0, 5, 5, 1.5, 6, 5, 2.0, 5, 6], [1.5, 5, 5, 1.5, 6, 5, 1.5, 5, 6], [2.0, 5, 5, 1.0, 6, 5, 1.0, 5, 6]]
+    kids = [[wall(float(x), 1)] for x in (3, 4, 5, 6, 7)] + [[wall(7.5, 1)] * 6]
+    spec = [
+        {"children": [1, 2, 3], "boxes": [box(3, 8, 4), box(1, 2, 6), box(9.5, 9.5, 4)],
+         "order": [[1, 0, 2], [0, 2, 1], [0, 2, 1]]},
+        {"children": [4, 5, 6, 7, 8, 9], "boxes": [box(3, 3, 1), box(4, 4, 1), box(5, 5, 1), box(6, 6, 1),
+                                                   box(7, 7, 1), box(7.5, 7.5, 1)],
+         "order": [[0, 1, 2, 3, 4, 5]] * 3},
+        {"tris": a_tris, "ids": [100, 101, 102]},
+        {"tris": [wall(9.5, 4)] * 5, "ids": [110, 111, 112, 113, 114]},
+    ] + [{"tris": t, "ids": [120 + 10 * i + j for j in range(len(t))]} for i, t in enumerate(kids)]
+    return build(8, 8, 4, spec)

@@ -45,6 +45,33 @@ def test_hist_stale_tree(rt):
     assert st["leaf_depth"][1] == 1 and st["tri_hits"] == 1
 
 
+@pytest.mark.parametrize("mode", ["closest", "save_distance", "any"])
+def test_marker_tree_on_gpu(rt, mode):
+    """marker_tree (M = 8, G = 4, Lm = 8, loop counts from NaN markers)
+    through rt_upload_bvh and rt_trace: the hand-derived hits and counts of
+    the oracle tests."""
+    from tests.test_oracle_traversal import MARKER_MINUS_X, MARKER_PLUS_X, MARKER_PLUS_X_HIGH
+
+    scene = scene_of(rt, handbvh.marker_tree())
+    rays = np.concatenate([fray([0, 0.25, 0.25], [8, 0, 0]), fray([0, 1.5, 1.5], [8, 0, 0]),
+                           fray([12, 0.25, 0.25], [-8, 0, 0])])
+    out, cnt = scene.intersect(rays, mode=mode, counts=True)
+    c = [(int(x["nodes"]), int(x["node_slots"]), int(x["leaves"]), int(x["leaf_slots"])) for x in cnt]
+    if mode == "any":
+        assert list(out) == [1, 1, 1]
+        assert c[0] == c[2] == (1, 4, 1, 8)
+        return
+    if mode == "save_distance":
+        assert (out["leaf"][0], out["slot"][0], out["t"][0]) == (144, 0, np.float32(0.375))
+        assert c[0] == (2, 12, 4, 20)
+        return
+    for i, want in enumerate((MARKER_PLUS_X, MARKER_PLUS_X_HIGH, MARKER_MINUS_X)):
+        h = out[i]
+        got = (int(h["leaf"]), int(h["slot"]), float(h["t"]), float(h["b1"]), float(h["b2"])) + c[i]
+        assert got == want, i
+        assert h["tri"] == (120 if i == 0 else 113)
+
+
 def test_hist_quirk_tree(rt):
     """quirk_tree(full=False) along +x: root with 3 children (slots 0 and 1
     hold the same triangle, hit; slot 2 missed), both leaves visited."""

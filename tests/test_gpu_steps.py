@@ -52,6 +52,16 @@ def test_primary_tiles(rt, spheres, NL, tile):
     _check(scene, bvh, rays, tile, "closest")
 
 
+@pytest.mark.parametrize("tile", [1, 4])
+def test_tiny_tiles(rt, spheres, tile):
+    """Blocks of 1 and 4 threads (workGroupSize 1 and 2): fewer threads than
+    the block's five step counters, which must still start at zero."""
+    scene, bvh = spheres[(8, 8)]
+    wg = int(round(tile ** 0.5))
+    rays = rt.camera_rays((0.0, 4.0, 32.0), (0.0, 0.0, 0.0), 16 * wg, 8 * wg, wg)
+    _check(scene, bvh, rays, tile, "closest")
+
+
 def test_ambient_tiles_skip_nan_origins(rt, spheres):
     """Ambient any-hit rays of a primary frame; pixels without a hit carry a
     NaN origin and are not started (terminations of record 0)."""

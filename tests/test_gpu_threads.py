@@ -114,10 +114,12 @@ def test_last_error_is_per_thread(rt, world):
 
 
 def test_many_streams_keep_bounded_scratch(rt, world):
-    """48 different streams on one scene, each call returning before its
+    """48 calls on fresh torch streams (torch hands them out from a pool of
+    32, so at most 32 distinct handles), each call returning before its
     kernel ends (RT_NO_SYNC): the scene keeps the scratch of 16 streams and
-    frees the least recently used one after its last launch, so every result
-    is intact and the library's device memory stays bounded."""
+    hands the least recently used one to the next new stream after its last
+    launch, so every result is intact and the library's device memory stays
+    bounded."""
     import torch
 
     scene, rays, alone = world
