@@ -127,6 +127,8 @@ SIGNATURES = [
     ("rt_stats", ctypes.c_int, [_P, ctypes.POINTER(RtStats)]),
     ("rt_trace_histograms", ctypes.c_int, [_P, _P, _U32, _U32, _P, ctypes.POINTER(RtTraceHist), _P]),
     ("rt_trace_steps", ctypes.c_int, [_P, _P, _U32, _U32, _U32, _U32, _P, _P, _P, _P]),
+    ("rt_check_recip", ctypes.c_int, [_U32, ctypes.c_uint64, _P, ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_uint64), _P]),
     ("rt_set_launch", ctypes.c_int, [_P, _U32, _U32]),
     ("rt_set_schedule", ctypes.c_int, [_P, _U32]),
     ("rt_release", ctypes.c_int, [_P]),

@@ -752,6 +752,29 @@ int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, voi
 }
 }  // namespace
 
+extern "C" int rt_check_recip(uint32_t first_bits, uint64_t count, float* values, uint64_t* mismatches,
+                              uint64_t* first_mismatch, void* hip_stream) {
+    if (!mismatches) return set_error(RT_ERR_ARG, "rt_check_recip: null mismatch counter");
+    if (count > (1ull << 32)) return set_error(RT_ERR_ARG, "rt_check_recip: at most 2^32 patterns");
+    *mismatches = 0;
+    if (first_mismatch) *first_mismatch = ~0ull;
+    if (count == 0) return RT_OK;
+    hipStream_t st = static_cast<hipStream_t>(hip_stream);
+    unsigned long long* d = nullptr;
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&d), 2 * sizeof(unsigned long long)), "hipMalloc(counter)");
+    const unsigned long long init[2] = {0ull, ~0ull};
+    unsigned long long res[2] = {0, 0};
+    hipError_t e = hipMemcpyAsync(d, init, sizeof(init), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_check_recip(first_bits, count, values, d, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(res, d, sizeof(res), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(e, "rt_check_recip");
+    *mismatches = res[0];
+    if (first_mismatch) *first_mismatch = res[1];
+    return RT_OK;
+}
+
 extern "C" int rt_stats(rt_scene_t s, rt_stats_t* o) {
     if (!s || !o) return set_error(RT_ERR_ARG, "rt_stats: null argument");
     std::lock_guard<std::mutex> lk(s->mu);

@@ -114,6 +114,13 @@ hipError_t launch_coop(uint32_t W, int mode, uint32_t grid, const KParams& p, hi
 int coop_occupancy(uint32_t W, int mode, uint32_t depth);
 
 hipError_t launch_reset(uint32_t* work, hipStream_t stream);
+
+// Self-test of the leaf test's reciprocal against the correctly rounded
+// division over `count` float bit patterns from `first` (wrapping):
+// counter[0] += mismatches, counter[1] = min(counter[1], lowest mismatching
+// pattern); values (optional) receives the reciprocals.
+hipError_t launch_check_recip(uint32_t first, uint64_t count, float* values, unsigned long long* counter,
+                              hipStream_t stream);
 // Zeroes `words` 32-bit words with a kernel (no memset node if captured).
 hipError_t launch_zero(uint32_t* p, uint32_t words, hipStream_t stream);
 

@@ -1238,6 +1238,40 @@ __global__ void k_reset(uint32_t* work) {
     for (uint32_t i = threadIdx.x; i < kWorkWords; i += blockDim.x) work[i] = 0;
 }
 
+// Self-test of recip(): for the float bit patterns first, first + 1, ...
+// (count of them, wrapping at 2^32) compares recip(x) with the correctly
+// rounded division 1.0f / x; a mismatch is a bit difference other than two
+// NaNs.  Mismatches are counted wave-aggregated into counter[0] (64-bit) and
+// the lowest mismatching pattern is kept in counter[1] (atomicMin).  values,
+// if not null, receives recip(x) for each pattern.  Every thread's loop is
+// bounded by count.
+__global__ void __launch_bounds__(256) k_check_recip(uint32_t first, uint64_t count, float* values,
+                                                     unsigned long long* counter) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long bad = 0, lowest = ~0ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const uint32_t bits = first + (uint32_t)i;
+        const float x = __uint_as_float(bits);
+        const float r = recip(x);
+        const float q = 1.0f / x;
+        if (values) values[i] = r;
+        const bool same = __float_as_uint(r) == __float_as_uint(q) || (r != r && q != q);
+        if (!same) {
+            ++bad;
+            lowest = min(lowest, (unsigned long long)bits);
+        }
+    }
+    // wave reduction, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        bad += __shfl_xor(bad, off);
+        lowest = min(lowest, (unsigned long long)__shfl_xor(lowest, off));
+    }
+    if (__lane_id() == 0 && bad) {
+        atomicAdd(&counter[0], bad);
+        atomicMin(&counter[1], lowest);
+    }
+}
+
 __global__ void k_zero(uint32_t* p, uint32_t words) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -1514,6 +1548,13 @@ hipError_t launch_zero(uint32_t* p, uint32_t words, hipStream_t stream) {
 
 hipError_t launch_reset(uint32_t* work, hipStream_t stream) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, work);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_recip(uint32_t first, uint64_t count, float* values, unsigned long long* counter,
+                              hipStream_t stream) {
+    const uint64_t blocks = std::min<uint64_t>(2048, std::max<uint64_t>(1, (count + 255) / 256));
+    hipLaunchKernelGGL(k_check_recip, dim3((uint32_t)blocks), dim3(256), 0, stream, first, count, values, counter);
     return hipGetLastError();
 }
 

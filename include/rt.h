@@ -214,6 +214,20 @@ typedef struct rt_step_stat {
 int rt_trace_steps(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, uint32_t tile,
                    uint32_t step_cap, rt_step_stat* out, uint32_t* steps, rt_counts* counts, void* hip_stream);
 
+/* Self-test of the one arithmetic primitive the kernels do not take from the
+ * compiler: the reciprocal of the leaf test (inv = 1.f / div,
+ * rayTracerTemplate.ispc:59, correctly rounded like the ISPC code's divps).
+ * The kernels form it from the hardware reciprocal and one Newton step; this
+ * compares it with the correctly rounded division 1.0f / x for the `count`
+ * float bit patterns first_bits, first_bits + 1, ... (wrapping at 2^32,
+ * count <= 2^32) on the current device.  *mismatches = patterns whose
+ * results differ in any bit (two NaNs count as equal); *first_mismatch
+ * (optional) = the lowest such pattern, or UINT64_MAX.  values (optional,
+ * device, count floats) receives the reciprocals.  No reference
+ * counterpart; the call waits for the result. */
+int rt_check_recip(uint32_t first_bits, uint64_t count, float* values, uint64_t* mismatches,
+                   uint64_t* first_mismatch, void* hip_stream);
+
 /* Overrides the launch configuration (block size 64, 128, 192 or 256,
  * LDS stack window 8/16/32 entries).  0 keeps the current value. */
 int rt_set_launch(rt_scene_t s, uint32_t block_size, uint32_t stack_window);
