@@ -1,12 +1,13 @@
-"""The headline workloads at full size (1M-triangle interior at N8L8,
-1920x1080 primaries, their shadow rays, 8M diffuse bounce rays), checked by
-properties that do not depend on size: the per-ray, ray-stream and
-cooperative traversals agree on every ray (hits and counts), a random
-sample of 20,000 rays matches the oracle bit for bit, and the hit t and
-barycentrics are compared with an FP64 re-evaluation of Moeller-Trumbore
-(the differences are printed: t relative, barycentrics absolute).  Every
-GPU result is bit-identical to the FP32 reference arithmetic; the FP64
-comparison states how far that arithmetic is from exact."""
+"""The headline workloads at full size (1M-triangle interior, 1920x1080
+primaries, their shadow rays, 8M diffuse bounce rays): every ray of the
+N x L sweep (primaries and shadow rays at all 16 points) and of the diffuse
+workload (N8L8, N16L16, N4L4) equals the oracle, counts included; the
+per-ray, ray-stream and cooperative traversals agree on every ray; and the
+hit t and barycentrics of a 20,000-ray sample are compared with an FP64
+re-evaluation of Moeller-Trumbore (the differences are printed: t relative,
+barycentrics absolute).  Every GPU result is bit-identical to the FP32
+reference arithmetic; the FP64 comparison states how far that arithmetic is
+from exact."""
 import numpy as np
 import pytest
 
@@ -46,16 +47,6 @@ def _device(rt, scene, rays, mode="closest", **kw):
     o = o.view(rt.HIT_DTYPE).reshape(-1) if mode != "any" else o
     return o, cnt.cpu().numpy().view(rt.COUNTS_DTYPE).reshape(-1)
 
-
-def test_diffuse_8m_schedules_agree(rt, world):
-    tris, bvh, scene, prim, diffuse, shadow = world
-    print(f"8M diffuse N8L8: per-ray vs stream vs cooperative", flush=True)
-    h0, c0 = _device(rt, scene, diffuse)
-    assert (h0["slot"] >= 0).mean() > 0.99
-    for kw in ({"stream_sched": True}, {"coop": True}):
-        h, c = _device(rt, scene, diffuse, **kw)
-        assert np.array_equal(h, h0), kw
-        assert np.array_equal(c, c0), kw
 
 
 def test_primary_and_shadow_schedules_agree(rt, world):
@@ -97,26 +88,57 @@ def test_sample_matches_oracle_and_fp64(rt, world):
     assert mx["b1"] < 1e-3 and mx["b2"] < 1e-3
 
 
-@pytest.mark.parametrize("N,L", [(4, 4), (16, 16)])
-def test_other_node_sizes_at_full_scene(rt, world, N, L):
-    """The same 1M triangles at N4L4 and N16L16: 2M diffuse rays agree across
-    the per-ray, stream and cooperative kernels, and a 5,000-ray sample equals
-    the oracle (16-wide nodes take the two-word order rows and 48-byte headers)."""
+@pytest.mark.parametrize("N,L", [(8, 8), (16, 16), (4, 4)])
+def test_diffuse_8m_every_ray_matches_oracle(rt, world, N, L):
+    """Config 3 in full: all 8,000,000 diffuse rays at N8L8, N16L16 and N4L4
+    equal the oracle ray for ray (hit identity, t, barycentrics and the four
+    counts), and the ray-stream and cooperative kernels agree with the
+    per-ray kernel on every ray."""
     import torch
 
-    tris, _, _, prim, diffuse, _ = world
-    print(f"1M tris N{N}L{L}: 2M diffuse rays, three schedules", flush=True)
+    tris, bvh8, scene8, prim, diffuse, _ = world
+    print(f"1M tris N{N}L{L}: 8M diffuse rays, three kernels and the oracle", flush=True)
+    bvh = bvh8 if (N, L) == (8, 8) else rt.Bvh(tris, N, L)
+    scene = scene8 if (N, L) == (8, 8) else rt.Scene(bvh)
+    try:
+        h0, c0 = _device(rt, scene, diffuse)
+        assert (h0["slot"] >= 0).mean() > 0.99
+        for kw in ({"stream_sched": True}, {"coop": True}):
+            h, c = _device(rt, scene, diffuse, **kw)
+            assert np.array_equal(h, h0) and np.array_equal(c, c0), kw
+        ref = oracle_for(bvh).trace(diffuse)
+        assert_closest_equal(h0, ref["hits"], bvh, c0, ref["counts"], f"N{N}L{L} 8M diffuse")
+    finally:
+        if scene is not scene8:
+            scene.close()
+        torch.cuda.synchronize()
+
+
+NL_SWEEP = [(n, l) for n in (4, 8, 12, 16) for l in (4, 8, 12, 16)]
+
+
+@pytest.mark.parametrize("N,L", NL_SWEEP)
+def test_config2_sweep_every_ray_matches_oracle(rt, world, N, L):
+    """Config 2 in full at every point of the N x L sweep: all 2,073,600
+    primary rays (closest hit) and all their shadow rays (any hit toward the
+    light) equal the oracle ray for ray, counts included."""
+    import torch
+
+    tris, _, _, prim, _, shadow = world
+    print(f"1M tris N{N}L{L}: {len(prim)} primaries + {len(shadow)} shadow rays vs the oracle", flush=True)
     bvh = rt.Bvh(tris, N, L)
     scene = rt.Scene(bvh)
     try:
-        rays = diffuse[:2_000_000]
-        h0, c0 = _device(rt, scene, rays)
-        for kw in ({"stream_sched": True}, {"coop": True}):
-            h, c = _device(rt, scene, rays, **kw)
-            assert np.array_equal(h, h0) and np.array_equal(c, c0), kw
-        idx = np.sort(np.random.default_rng(N).choice(len(rays), 5_000, replace=False))
-        ref = oracle_for(bvh).trace(rays[idx])
-        assert_closest_equal(h0[idx], ref["hits"], bvh, c0[idx], ref["counts"], f"N{N}L{L} sample")
+        h, c = _device(rt, scene, prim)
+        o = oracle_for(bvh)
+        ref = o.trace(prim)
+        assert_closest_equal(h, ref["hits"], bvh, c, ref["counts"], f"N{N}L{L} primaries")
+        occ, sc = _device(rt, scene, shadow, mode="any")
+        ref = o.trace(shadow, mode=oracle.ORC_ANY)
+        assert np.array_equal(occ, ref["occluded"]), f"N{N}L{L} shadow occlusion"
+        for f in ("nodes", "leaves", "node_slots", "leaf_slots"):
+            assert np.array_equal(sc[f], ref["counts"][f]), f"N{N}L{L} shadow counts.{f}"
+        assert 0 < int(occ.sum()) < len(occ)
     finally:
         scene.close()
         torch.cuda.synchronize()
