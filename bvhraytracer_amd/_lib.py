@@ -124,6 +124,7 @@ _U32 = ctypes.c_uint32
 SIGNATURES = [
     ("rt_upload_bvh", ctypes.c_int, [ctypes.POINTER(RtBvhDesc), ctypes.POINTER(_P)]),
     ("rt_trace", ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P]),
+    ("rt_trace_noalloc", ctypes.c_int, [_P, _P, _U32, _U32, _P, _P, _P]),
     ("rt_stats", ctypes.c_int, [_P, ctypes.POINTER(RtStats)]),
     ("rt_trace_histograms", ctypes.c_int, [_P, _P, _U32, _U32, _P, ctypes.POINTER(RtTraceHist), _P]),
     ("rt_trace_steps", ctypes.c_int, [_P, _P, _U32, _U32, _U32, _U32, _P, _P, _P, _P]),

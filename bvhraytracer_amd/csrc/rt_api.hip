@@ -752,6 +752,81 @@ int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, voi
 }
 }  // namespace
 
+// The per-ray trace into a caller-given scratch block (work counters, then
+// the stack spill): no allocation, no event, no wait.  The grid is rt_trace's,
+// trimmed so that every lane's spill fits in the block.
+int rt::trace_in_scratch(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, void* out,
+                         rt_counts* counts, hipStream_t st, uint32_t* scratchBase, size_t scratchBytes,
+                         const char* who) {
+    const std::string w(who);
+    if (!s) return set_error(RT_ERR_ARG, w + ": null scene");
+    if (flags & ~(RT_ANY | RT_SAVE_DIST | RT_COUNTS | RT_DEVICE_PTRS | RT_NO_SYNC | RT_STREAM | RT_COOP))
+        return set_error(RT_ERR_ARG, w + ": unknown flag bits");
+    if (flags & (RT_STREAM | RT_COOP))
+        return set_error(RT_ERR_UNSUPPORTED, w + ": per-ray traversal only (no RT_STREAM or RT_COOP)");
+    if ((flags & RT_ANY) && (flags & RT_SAVE_DIST)) return set_error(RT_ERR_ARG, w + ": RT_ANY and RT_SAVE_DIST are exclusive");
+    if (n == 0) return RT_OK;
+    if (!rays || !out) return set_error(RT_ERR_ARG, w + ": null rays or output");
+    if ((flags & RT_COUNTS) && !counts) return set_error(RT_ERR_ARG, w + ": RT_COUNTS without a counts buffer");
+    if (n >= 0x80000000u) return set_error(RT_ERR_ARG, w + ": at most 2^31-1 rays per call");
+    if (!scratchBase || scratchBytes < kWorkWords * 4) return set_error(RT_ERR_ARG, w + ": no scratch");
+    const int mode = (flags & RT_ANY) ? kAny : ((flags & RT_SAVE_DIST) ? kSaveDist : kClosest);
+    std::lock_guard<std::mutex> lk(s->mu);
+    int prevDev = 0;
+    (void)hipGetDevice(&prevDev);
+    if (prevDev != s->device) HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
+    struct Restore {
+        int dev, prev;
+        ~Restore() {
+            if (dev != prev) (void)hipSetDevice(prev);
+        }
+    } restore{s->device, prevDev};
+    const uint32_t S = s->stackWindow;
+    uint32_t bd = s->blockSize;
+    while (bd > 64 && trace_lds_bytes(s->M, mode, S, bd) > 65536) bd -= 64;
+    const int occ = std::max(1, trace_occupancy(s->M, mode, S, bd));
+    const uint32_t cap = stack_capacity(s);
+    const uint32_t spillCap = cap > S ? cap - S : 0;
+    const size_t perBlock = (size_t)bd * spillCap * 4 * (mode == kSaveDist ? 2 : 1);
+    const size_t spillBytes = scratchBytes - kWorkWords * 4;
+    if (perBlock > spillBytes)
+        return set_error(RT_ERR_UNSUPPORTED, w + ": the scene's stack spill for one block of " + std::to_string(bd) +
+                                                 " lanes (" + std::to_string(perBlock) +
+                                                 " bytes) exceeds the scratch (" + std::to_string(spillBytes) + " bytes)");
+    const uint64_t fit = perBlock ? spillBytes / perBlock : ~0ull;
+    const uint64_t needBlocks = ((uint64_t)n + bd - 1) / bd;
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(
+        1, std::min(std::min((uint64_t)s->cus * (uint64_t)occ, needBlocks), fit));
+    KParams p{};
+    p.nodes = s->nodes;
+    p.tris = s->tris;
+    p.triIds = s->triIds;
+    p.nodeEntry = s->nodeEntry;
+    p.rays = rays;
+    p.hits = mode == kAny ? nullptr : static_cast<rt_hit*>(out);
+    p.occluded = mode == kAny ? static_cast<uint8_t*>(out) : nullptr;
+    p.counts = (flags & RT_COUNTS) ? counts : nullptr;
+    p.work = scratchBase;
+    p.spill = scratchBase + kWorkWords;
+    p.n = n;
+    p.rootEntry = s->rootEntry;
+    p.spillCap = spillCap;
+    p.gang = s->G;
+    p.Lm = s->Lm;
+    p.nodeCount = s->nodeCount;
+    p.M = s->M;
+    p.chunk = ray_chunk(n, grid, bd);
+    p.boxesBounded = s->boxesBounded ? 1u : 0u;
+    p.sched = s->sched;
+    p.leafTest = s->leafTest;
+    HIP_TRY(launch_reset(p.work, st), "reset kernel");
+    HIP_TRY(launch_trace(s->M, mode, S, bd, grid, p, st), "trace kernel launch");
+    s->raysTraced += n;
+    return RT_OK;
+}
+
+int rt::scene_device(rt_scene_t s) { return s ? s->device : -1; }
+
 extern "C" int rt_check_recip(uint32_t first_bits, uint64_t count, float* values, uint64_t* mismatches,
                               uint64_t* first_mismatch, void* hip_stream) {
     if (!mismatches) return set_error(RT_ERR_ARG, "rt_check_recip: null mismatch counter");

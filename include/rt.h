@@ -148,6 +148,29 @@ int rt_upload_bvh(const rt_bvh_desc* desc, rt_scene_t* out);
 int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags,
              void* out, rt_counts* counts, void* hip_stream);
 
+/* rt_trace for callers that cannot allocate: code under HIP graph capture,
+ * serving loops and harnesses that forbid hipMalloc.  No counterpart in the
+ * reference (its ISPC kernels keep their state on the caller's stack).
+ * The per-ray kernel's scratch (work counters and stack spill) lives in one
+ * fixed array of RT_NOALLOC_SCRATCH_BYTES per GPU, held from the library's
+ * load until the process ends; the call allocates nothing, records no event
+ * and does not wait: it enqueues a reset kernel and the trace kernel on
+ * hip_stream (NULL = the null stream) and returns.  rays, out and counts are
+ * device pointers (RT_DEVICE_PTRS and RT_NO_SYNC are implied); flags:
+ * RT_CLOSEST / RT_ANY / RT_SAVE_DIST and RT_COUNTS; RT_STREAM and RT_COOP
+ * return RT_ERR_UNSUPPORTED.  Results are those of rt_trace.  The launch
+ * uses as many persistent blocks as rt_trace up to what the spill fits; a
+ * scene whose stack spill for one block exceeds the array returns
+ * RT_ERR_UNSUPPORTED.  Every call on one GPU shares that GPU's array:
+ * calls whose kernels overlap on one GPU (different streams, or host
+ * threads) race and must be ordered by the caller.  The first call on each
+ * GPU resolves the array's device address (hipGetSymbolAddress): make it
+ * outside graph capture.  Rays stopped by a guard are not added to
+ * rt_stats.error_rays (nothing is read back). */
+#define RT_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int rt_trace_noalloc(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, void* out,
+                     rt_counts* counts, void* hip_stream);
+
 typedef struct rt_stats_t {
     uint64_t bytes_resident;  /* device bytes owned by the scene               */
     uint32_t inner_nodes, leaves;
