@@ -31,7 +31,21 @@ HBM_PEAK_GBPS = 8000.0
 # in KiB.  Keyed by (branching, leaf size, rays); the commands and the
 # per-dispatch values are in profiles/k_trace_N8L8_diffuse8M_hbm_traffic.md.
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
-MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 8383360, "write": 397396}}
+MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 8383360, "write": 397396},
+                        # batched leg (profiles/k_trace_batched_64scenes_hbm_traffic.md)
+                        ("batch", 8, 8, 9_872_996): {"fetch": 3175441, "write": 409272}}
+
+# What binds the timed kernel, from rocprofv3 counter runs of this benchmark
+# (profiles/k_trace_N8L8_diffuse8M_l2_valu_occupancy.md): the VALU pipes are
+# busy 74% of the time with 38.8% of the lanes active, the rest is waits on
+# dependent loads; HBM carries a quarter of its peak.
+MEASURED_COUNTERS = {(8, 8, 8_000_000): {"valu_busy": 0.74, "active_lanes": 0.388, "l2_hit": 0.858,
+                                         "source": "profiles/k_trace_N8L8_diffuse8M_l2_valu_occupancy.md"}}
+
+
+def measured_traffic(key):
+    mt = MEASURED_TRAFFIC_KIB.get(key)
+    return None if mt is None else int((mt["fetch"] * FETCH_SCALE + mt["write"] * WRITE_SCALE) * 1024)
 METRIC = "Mrays/s per MI355X (closest-hit, N8L8) batched vs plain; p99 request latency"
 
 
@@ -53,90 +67,109 @@ def algorithmic_bytes(counts, M):
             + 36 * c["leaf_slots"].astype(np.float64).sum())
 
 
-def clamp_lognormal(rng, mu, sigma, lo, hi, size):
-    return np.clip(np.exp(rng.normal(mu, sigma, size)), lo, hi).astype(np.int64)
+def useful_bytes(n, hist, M):
+    """Bytes of real work only: ray in and hit out (64 B per ray), the real
+    child boxes tested (24 B each) plus the header per inner visit, and the
+    real triangles tested (36 B each); the padded slots that the loop
+    counts k include are left out.  hist: rt_trace_histograms totals."""
+    return (64.0 * n + 24.0 * hist["aabb_tests"] + header_bytes(M) * hist["node_visits"]
+            + 36.0 * hist["tri_tests"])
 
 
-def random_camera_rays(rt, rng, n):
-    """Primary rays of a random camera inside the 30x10x30 room: n rays of a
-    256-wide image (tile order), the last tile row cut to n."""
-    pos = rng.uniform([-12.0, 1.0, -12.0], [12.0, 9.0, 12.0]).astype(np.float32)
-    d = rng.normal(size=3)
-    d[1] *= 0.3
-    tgt = (pos + 5.0 * d / np.linalg.norm(d)).astype(np.float32)
-    h = max(16, int(np.ceil(n / 256 / 16)) * 16)
-    return rt.camera_rays(tuple(pos), tuple(tgt), 256, h, 16)[:n]
+def hbm_frac(traffic, kernel_ms):
+    return None if traffic is None else round(traffic / (kernel_ms / 1000.0) / 1e9 / HBM_PEAK_GBPS, 4)
+
+
+def median_wall(fn, torch, reps, warmup=1):
+    """Median wall time (s) of fn() followed by a device synchronize."""
+    for _ in range(max(1, warmup)):
+        fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t)
+    return float(np.median(ts))
 
 
 def bench_batched(rt, torch, args, log):
     """64 scenes (seeds 100..163), triangles clamp(LogNormal(ln 2e5, 1), 1e4, 2e6),
     rays per scene clamp(LogNormal(ln 6.4e4, 1.2), 4096, 2^20), half primary
     (random camera in the room), half diffuse bounces from those hits.
-    plain = one rt_trace per scene; batched = one rt_batch_trace (graph)."""
-    rng = np.random.default_rng(4)
-    ntris = clamp_lognormal(rng, np.log(2e5), 1.0, 1e4, 2e6, 64)
-    nrays = clamp_lognormal(rng, np.log(6.4e4), 1.2, 4096, 1 << 20, 64)
+    batched = one rt_batch_trace (captured graph, re-armed); plain_sync = one
+    rt_trace per scene that waits for its kernel (the call's default);
+    plain_async = the same 64 rt_trace calls queued back to back on one
+    stream without waiting (RT_NO_SYNC), one synchronize at the end."""
+    from bvhraytracer_amd import workloads as wl
+
+    N, L = args.branching, args.leaf
+    ntris, nrays = wl.batch_sizes()
     t0 = time.time()
-    scenes, host_rays = [], []
-    for i in range(64):
-        tris = rt.interior_scene(100 + i, int(ntris[i]))
-        bvh = rt.Bvh(tris, args.branching, args.leaf)
-        sc = rt.Scene(bvh)
-        npri = int(nrays[i]) // 2
-        prim = random_camera_rays(rt, np.random.default_rng(1000 + i), npri)
-        ph = sc.intersect(prim)
-        nrm, pos = bvh.surface(prim, ph)
-        if (ph["slot"] >= 0).any():
-            diff = rt.spawn_diffuse(prim, nrm, pos, ph, int(nrays[i]) - npri, seed=100 + i)
-            rays = np.concatenate([prim, diff])
-        else:
-            rays = prim
+    scenes = []
+    for i in range(wl.BATCH_SCENES):
+        tris, bvh, sc, rays = wl.batch_scene(i, N, L, ntris[i], nrays[i])
         d_r = torch.from_numpy(rays.view(np.float32).reshape(-1, 10)).cuda()
         d_h = torch.empty((len(rays), 6), dtype=torch.int32, device="cuda")
-        scenes.append((bvh, sc, d_r, d_h))
-        host_rays.append((tris, rays))
-    total = sum(x[2].shape[0] for x in scenes)
+        scenes.append((tris, bvh, sc, rays, d_r, d_h))
+    total = sum(len(x[3]) for x in scenes)
     log(f"batched workload: 64 scenes, {int(ntris.sum())} tris, {total} rays, setup {time.time() - t0:.1f}s")
     stream = torch.cuda.current_stream()
 
-    def plain():
-        for bvh, sc, d_r, d_h in scenes:
-            sc.intersect_device(d_r, d_h, stream=stream)
+    def plain_sync():
+        for x in scenes:
+            x[2].intersect_device(x[4], x[5], stream=stream)
+
+    def plain_async():
+        for x in scenes:
+            x[2].intersect_device(x[4], x[5], stream=stream, sync=False)
 
     batch = rt.Batch(64)
-    outs = [torch.empty_like(x[3]) for x in scenes]
-    items = [(sc, d_r, o, "closest") for (bvh, sc, d_r, d_h), o in zip(scenes, outs)]
+    outs = [torch.empty_like(x[5]) for x in scenes]
+    items = [(x[2], x[4], o, "closest") for x, o in zip(scenes, outs)]
 
     def batched():
-        batch.trace(items, graph=True, stream=stream)
+        batch.trace(items, graph=True, stream=stream, sync=False)
 
-    res = {}
-    for name, fn in (("plain", plain), ("batched", batched)):
-        for _ in range(max(1, args.warmup)):
-            fn()
-        torch.cuda.synchronize()
-        ts = []
-        for _ in range(max(3, args.steps // 2)):
-            t = time.perf_counter()
-            fn()
-            torch.cuda.synchronize()
-            ts.append(time.perf_counter() - t)
-        res[name] = total / float(np.median(ts)) / 1e6
-    same = all(torch.equal(o, x[3]) for o, x in zip(outs, scenes))
-    log(f"batched {res['batched']:.1f} Mrays/s vs plain {res['plain']:.1f} Mrays/s, identical={same}")
+    reps = max(3, args.steps // 2)
+    res = {name: total / median_wall(fn, torch, reps, args.warmup) / 1e6
+           for name, fn in (("plain_sync", plain_sync), ("plain_async", plain_async), ("batched", batched))}
+    same = all(torch.equal(o, x[5]) for o, x in zip(outs, scenes))
+    # kernel time of the batched launch (HIP events around the graph launch)
+    kms = []
+    for _ in range(reps):
+        batch.trace(items, graph=True, stream=stream)
+        kms.append(batch.stats()["last_kernel_ms"])
+    k_ms = float(np.median(kms))
+    # algorithmic bytes of the same launch from a counts pass
+    cnts = [torch.empty((len(x[3]), 4), dtype=torch.int32, device="cuda") for x in scenes]
+    batch.trace([(x[2], x[4], o, "closest", c) for x, o, c in zip(scenes, outs, cnts)], graph=True, stream=stream)
+    alg = sum(algorithmic_bytes(c.cpu().numpy().view(rt.COUNTS_DTYPE).reshape(-1), x[1].node_memory)
+              for x, c in zip(scenes, cnts))
+    achieved = alg / (k_ms / 1000.0) / 1e9
+    traffic = measured_traffic(("batch", N, L, total))
+    log(f"batched {res['batched']:.1f} Mrays/s (kernel {k_ms:.3f} ms, {achieved:.0f} GB/s algorithmic) vs "
+        f"plain {res['plain_sync']:.1f} waiting / {res['plain_async']:.1f} queued, identical={same}")
     batch.close()
     out = {"scenes": 64, "triangles": int(ntris.sum()), "rays": int(total),
-           "plain_mrays": round(res["plain"], 2), "batched_mrays": round(res["batched"], 2),
-           "speedup": round(res["batched"] / res["plain"], 3), "identical_to_plain": bool(same),
-           "n_l": f"N{args.branching}L{args.leaf}", "graph": True}
-    if args.batch_autotune:
-        out["autotuned"] = batched_autotuned(rt, torch, host_rays, scenes, outs, stream, total, args, log)
+           "plain_sync_mrays": round(res["plain_sync"], 2), "plain_async_mrays": round(res["plain_async"], 2),
+           "batched_mrays": round(res["batched"], 2),
+           "speedup_vs_sync": round(res["batched"] / res["plain_sync"], 3),
+           "speedup_vs_async": round(res["batched"] / res["plain_async"], 3),
+           "identical_to_plain": bool(same), "n_l": f"N{N}L{L}", "graph": True,
+           "kernel_ms": round(k_ms, 4), "kernel_mrays": round(total / k_ms / 1e3, 2),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                        "hbm_frac": hbm_frac(traffic, k_ms), "algorithmic_bytes": int(alg)}}
+    if not args.no_batch_autotune:
+        out["autotuned"] = batched_autotuned(rt, torch, scenes, outs, stream, total, args, log)
     if not args.no_cpu:
         # the CPU baseline of this workload: the oracle per scene, summed
         # (one warm-up pass, then one timed pass over all 64 scenes)
         import oracle
 
-        orcs = [(oracle.Oracle.from_bvh(bvh), rays) for (bvh, _, _, _), (_, rays) in zip(scenes, host_rays)]
+        orcs = [(oracle.Oracle.from_bvh(x[1]), x[3]) for x in scenes]
         for o, rays in orcs:
             o.trace(rays, counts=False)
         t = time.perf_counter()
@@ -146,48 +179,45 @@ def bench_batched(rt, torch, args, log):
         out["cpu_port_mrays"] = round(total / cpu_s / 1e6, 3)
         out["cpu_threads"] = int(used)
         log(f"batched workload on the CPU oracle: {out['cpu_port_mrays']} Mrays/s on {used} threads")
-    for bvh, sc, d_r, d_h in scenes:
-        sc.close()
+    for x in scenes:
+        x[2].close()
     return out
 
 
-def batched_autotuned(rt, torch, host_rays, scenes, ref_outs, stream, total, args, log):
+def batched_autotuned(rt, torch, scenes, ref_outs, stream, total, args, log):
     """The same 64 scenes with N and L chosen per scene by the autotuner
-    (candidates N4L4, N4L8, N8L4, N8L8 timed on 16384 of the scene's own rays),
-    traced in one batched graph launch that mixes the chosen node/leaf
-    memories.  Tuning and the rebuilds are ingest cost, outside the timing."""
+    (candidates N4L4, N4L8, N8L4, N8L8 timed on 8192 of the scene's own
+    rays), traced in one batched graph launch that mixes the chosen node and
+    leaf memories.  Tuning and the rebuilds are ingest cost, outside the
+    timing."""
     from bvhraytracer_amd.autotune import Autotuner
 
     t0 = time.time()
     tuner = Autotuner(nl=((4, 4), (4, 8), (8, 4), (8, 8)), blocks=(256,), windows=(16,), schedules=(16,),
-                      sample=16384, reps=2)
+                      sample=8192, reps=2)
     tuned, choice = [], {}
-    for (tris, rays), (bvh0, sc0, d_r, d_h) in zip(host_rays, scenes):
+    for tris, bvh0, sc0, rays, d_r, d_h in scenes:
         best = tuner.tune_scene(tris, rays)
         key = f"N{best['N']}L{best['L']}"
         choice[key] = choice.get(key, 0) + 1
-        bvh = rt.Bvh(tris, best["N"], best["L"])
-        sc = rt.Scene(bvh)
-        tuned.append((bvh, sc, d_r, torch.empty_like(d_h)))
+        if (best["N"], best["L"]) == (bvh0.branching, bvh0.leaf_size):
+            tuned.append((None, sc0, d_r, torch.empty_like(d_h)))
+        else:
+            bvh = rt.Bvh(tris, best["N"], best["L"])
+            tuned.append((bvh, rt.Scene(bvh), d_r, torch.empty_like(d_h)))
     tune_s = time.time() - t0
     batch = rt.Batch(64)
     items = [(sc, d_r, o, "closest") for (bvh, sc, d_r, o) in tuned]
-    for _ in range(max(1, args.warmup)):
-        batch.trace(items, graph=True, stream=stream)
-    torch.cuda.synchronize()
-    ts = []
-    for _ in range(max(3, args.steps // 2)):
-        t = time.perf_counter()
-        batch.trace(items, graph=True, stream=stream)
-        torch.cuda.synchronize()
-        ts.append(time.perf_counter() - t)
-    mrays = total / float(np.median(ts)) / 1e6
+    sec = median_wall(lambda: batch.trace(items, graph=True, stream=stream, sync=False), torch,
+                      max(3, args.steps // 2), args.warmup)
+    mrays = total / sec / 1e6
     # the hit triangles are the same whatever the tree, up to exact ties
     same_tri = sum(int((o[:, 0] == r[:, 0]).sum().item()) for (b, s, d, o), r in zip(tuned, ref_outs)) / total
     log(f"batched autotuned {mrays:.1f} Mrays/s, choices {choice}, tuning {tune_s:.1f}s")
     batch.close()
     for bvh, sc, d_r, o in tuned:
-        sc.close()
+        if bvh is not None:
+            sc.close()
     return {"batched_mrays": round(mrays, 2), "choices": choice, "tune_seconds": round(tune_s, 1),
             "same_hit_triangle_fraction": round(same_tri, 6)}
 
@@ -285,41 +315,17 @@ def bench_render_frame(rt, scene, bvh, tris, log, reps=3):
     return out
 
 
-def bench_serving(rt, args, log):
-    """16 resident scenes (100k..2M tris log-spaced, seeds 200..215), Poisson
-    arrivals at 200 req/s for args.serve_seconds (seed 5), rays per request
-    clamp(LogNormal(ln 2.3e4, 1.3), 1024, 2^19), half shadow (any-hit), half
-    closest-hit; latency from submit to results in host memory."""
+def serve_run(rt, srv, sids, world, rate, seconds, seed):
+    """Poisson arrivals at `rate` req/s for `seconds` (workloads.serving_requests)
+    submitted at their arrival times from this thread while a second thread
+    waits for them in order.  Returns the server's statistics of the window
+    plus the arrival rate the submitting thread actually reached."""
     import threading
-    t0 = time.time()
-    counts = np.round(np.exp(np.linspace(np.log(1e5), np.log(2e6), 16))).astype(np.int64)
-    world = []
-    for i in range(16):
-        tris = rt.interior_scene(200 + i, int(counts[i]))
-        bvh = rt.Bvh(tris, args.branching, args.leaf)
-        sc = rt.Scene(bvh)
-        prim = random_camera_rays(rt, np.random.default_rng(2000 + i), 1 << 19)
-        ph = sc.intersect(prim)
-        nrm, pos = bvh.surface(prim, ph)
-        diff = rt.spawn_diffuse(prim, nrm, pos, ph, 1 << 19, seed=200 + i)
-        shadow, _ = rt.spawn_shadow(nrm, pos, ph, (0.0, 9.5, 0.0), 1.0)
-        shadow = np.resize(shadow, 1 << 19)
-        world.append((bvh, sc, np.concatenate([prim, diff]), shadow))
-    log(f"serving workload: 16 scenes, {int(counts.sum())} tris, setup {time.time() - t0:.1f}s")
-    rng = np.random.default_rng(5)
-    n_req = int(200 * args.serve_seconds)
-    gaps = rng.exponential(1 / 200.0, n_req)
-    sizes = clamp_lognormal(rng, np.log(2.3e4), 1.3, 1024, 1 << 19, n_req)
-    scene_of = rng.integers(0, 16, n_req)
-    is_shadow = rng.random(n_req) < 0.5
-    offs = [int(rng.integers(0, (1 << 20) - sizes[i] + 1)) if not is_shadow[i]
-            else int(rng.integers(0, (1 << 19) - sizes[i] + 1)) for i in range(n_req)]
-    srv = rt.Server(hbm_cap_bytes=8 << 30, slots=3)
-    sids = [srv.add_scene(w[1]) for w in world]
-    # warm the launch path (graph capture) outside the measured window
-    for i in range(8):
-        rid, _ = srv.submit(sids[i], world[i][2][:4096])
-        srv.wait(rid, 30)
+
+    from bvhraytracer_amd import workloads as wl
+
+    gaps, sizes, scene_of, shadow, offs = wl.serving_requests(rate, seconds, seed)
+    n_req = len(gaps)
     srv.stats(reset=True)
     done_q = []
     lock = threading.Lock()
@@ -328,12 +334,16 @@ def bench_serving(rt, args, log):
         k = 0
         while k < n_req:
             with lock:
-                ready = k < len(done_q)
-                rid = done_q[k] if ready else None
-            if not ready:
-                time.sleep(0.0005)
+                rid = done_q[k] if k < len(done_q) else None
+            if rid is None:
+                time.sleep(0.0002)
                 continue
-            srv.wait(rid, 120)
+            try:
+                srv.wait(rid, 120)
+            except rt.RtError:
+                pass  # counted as failed by the server
+            with lock:
+                done_q[k] = None  # drop the reference (the output buffer is freed)
             k += 1
 
     th = threading.Thread(target=waiter)
@@ -346,23 +356,67 @@ def bench_serving(rt, args, log):
         if dt > 0:
             time.sleep(dt)
         w = world[scene_of[i]]
-        src = w[3] if is_shadow[i] else w[2]
-        rid, _ = srv.submit(sids[scene_of[i]], src[offs[i]:offs[i] + sizes[i]],
-                            "any" if is_shadow[i] else "closest")
+        src = w[4] if shadow[i] else w[3]
+        rid, _ = srv.submit(sids[scene_of[i]], src[offs[i]:offs[i] + sizes[i]], "any" if shadow[i] else "closest")
         with lock:
             done_q.append(rid)
+    submit_s = time.perf_counter() - start
     th.join()
     st = srv.stats()
-    srv.close()
-    for w in world:
-        w[1].close()
-    log(f"serving: {st['completed']} requests, {st['mrays_per_s']:.1f} Mrays/s, p50 {st['p50_ms']:.2f} ms, "
-        f"p99 {st['p99_ms']:.2f} ms, {st['mean_requests_per_launch']:.1f} requests/launch")
-    return {"scenes": 16, "requests": int(st["completed"]), "arrival_rate": 200,
-            "seconds": args.serve_seconds, "offered_mrays_per_s": round(float(sizes.sum()) / args.serve_seconds / 1e6, 2),
+    return {"arrival_rate": rate, "requests": n_req, "submitted_rate": round(n_req / submit_s, 1),
+            "offered_mrays_per_s": round(float(sizes.sum()) / max(submit_s, 1e-9) / 1e6, 2),
             "mrays_per_s": round(st["mrays_per_s"], 2), "p50_ms": round(st["p50_ms"], 3),
             "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
-            "requests_per_launch": round(st["mean_requests_per_launch"], 2), "failed": int(st["failed"])}
+            "requests_per_launch": round(st["mean_requests_per_launch"], 2), "completed": int(st["completed"]),
+            "failed": int(st["failed"])}
+
+
+SWEEP_RATES = (2000, 10000, 20000, 40000)
+SWEEP_REQUESTS = 3000
+
+
+def bench_serving(rt, args, log):
+    """16 resident scenes (100k..2M tris log-spaced, seeds 200..215) at N/L.
+    First the configuration itself: Poisson arrivals at 200 req/s for
+    args.serve_seconds (seed 5), rays per request clamp(LogNormal(ln 2.3e4,
+    1.3), 1024, 2^19), half shadow (any-hit), half closest-hit; latency from
+    submit to results in host memory.  Then a sweep of higher arrival rates,
+    3000 requests each (seed 6), stopping at the first failed request: where
+    served Mrays/s stops following the offered load is the saturation knee."""
+    from bvhraytracer_amd import workloads as wl
+
+    t0 = time.time()
+    world = [wl.serving_scene(i, args.branching, args.leaf) for i in range(wl.SERVE_SCENES)]
+    log(f"serving workload: 16 scenes, {int(wl.serving_sizes().sum())} tris, setup {time.time() - t0:.1f}s")
+    srv = rt.Server(hbm_cap_bytes=8 << 30, slots=3)
+    sids = [srv.add_scene(w[2]) for w in world]
+    # warm the launch path (graph capture) outside the measured window
+    for i in range(8):
+        rid, _ = srv.submit(sids[i], world[i][3][:4096])
+        srv.wait(rid, 30)
+    base = serve_run(rt, srv, sids, world, 200, args.serve_seconds, 5)
+    log(f"serving at 200 req/s: {base['completed']} requests, {base['mrays_per_s']:.1f} Mrays/s, "
+        f"p50 {base['p50_ms']:.2f} ms, p99 {base['p99_ms']:.2f} ms, {base['requests_per_launch']:.2f} requests/launch")
+    sweep = []
+    if not args.no_serve_sweep:
+        for rate in SWEEP_RATES:
+            r = serve_run(rt, srv, sids, world, rate, SWEEP_REQUESTS / rate, 6)
+            sweep.append(r)
+            log(f"serving at {rate} req/s (submitted {r['submitted_rate']}): offered {r['offered_mrays_per_s']} "
+                f"Mrays/s, served {r['mrays_per_s']} Mrays/s, p50 {r['p50_ms']} ms, p99 {r['p99_ms']} ms, "
+                f"{r['requests_per_launch']} requests/launch, failed {r['failed']}")
+            if r["failed"]:
+                break
+    srv.close()
+    for w in world:
+        w[2].close()
+    out = {"scenes": 16, "seconds": args.serve_seconds, **base, "sweep": sweep}
+    if sweep:
+        # knee: the first rate whose served throughput falls below 90% of what was offered
+        knee = next((r["arrival_rate"] for r in sweep if r["mrays_per_s"] < 0.9 * r["offered_mrays_per_s"]), None)
+        out["saturation_knee_req_per_s"] = knee
+        out["peak_served_mrays_per_s"] = max(r["mrays_per_s"] for r in sweep)
+    return out
 
 
 def cpu_model():
@@ -389,9 +443,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-frame", action="store_true", help="skip the bounce-frame leg")
-    ap.add_argument("--batch-autotune", action="store_true",
-                    help="also run the batched leg with N and L autotuned per scene (adds ~1 min of ingest)")
-    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--no-batch-autotune", action="store_true",
+                    help="skip the batched leg with N and L autotuned per scene (~40 s of ingest)")
+    ap.add_argument("--no-serve-sweep", action="store_true", help="skip the serving arrival-rate sweep")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes (counter calibration)")
     args = ap.parse_args()
@@ -453,11 +508,14 @@ def main():
     value = n / (ms_per_step / 1000.0) / 1e6
     k_ms = float(np.mean(kms))
     achieved = alg_bytes / (k_ms / 1000.0) / 1e9
-    mt = MEASURED_TRAFFIC_KIB.get((N, L, n))
-    traffic = None if mt is None else int((mt["fetch"] * FETCH_SCALE + mt["write"] * WRITE_SCALE) * 1024)
+    traffic = measured_traffic((N, L, n))
     st = scene.stats()
     log(f"kernel {k_ms:.3f} ms avg, step {ms_per_step:.3f} ms, {value:.1f} Mrays/s, "
         f"{achieved:.0f} GB/s algorithmic, errors {st['error_rays']}")
+    # useful bytes: the real child boxes and triangles the same rays test
+    # (rt_trace_histograms totals), without the padded slots of k
+    _, hist = scene.histograms(rays)
+    useful = useful_bytes(n, hist, bvh.node_memory)
 
     # primary rays of the same scene (coherent), for reference
     d_prim = torch.from_numpy(prim.view(np.float32).reshape(-1, 10)).to(dev)
@@ -560,7 +618,12 @@ def main():
                    "primary_mrays_1920x1080": round(primary_mrays, 2),
                    "host_pointer_mrays": round(host_ptr_mrays, 2), "kernel_ms": round(k_ms, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "algorithmic_bytes": int(alg_bytes),
+                     "useful_bytes": int(useful), "useful_frac": round(useful / (k_ms / 1000.0) / 1e9 / HBM_PEAK_GBPS, 4),
+                     "hbm_frac": hbm_frac(traffic, k_ms),
+                     "limiter": "VALU issue and dependent-load latency, not HBM bandwidth",
+                     "counters": MEASURED_COUNTERS.get((N, L, n))},
         "cpu_baseline": cpu,
         "traversals_mrays": variants,
         "bounce_frame": frame,
