@@ -320,31 +320,28 @@ def serve_run(rt, srv, sids, world, rate, seconds, seed):
     submitted at their arrival times from this thread while a second thread
     waits for them in order.  Returns the server's statistics of the window
     plus the arrival rate the submitting thread actually reached."""
+    import queue
     import threading
 
     from bvhraytracer_amd import workloads as wl
 
     gaps, sizes, scene_of, shadow, offs = wl.serving_requests(rate, seconds, seed)
     n_req = len(gaps)
+    # result buffers allocated and touched before the window, as a client
+    # that reuses its buffers has them (first-touch page faults of fresh
+    # allocations would otherwise be timed as part of the server's copies)
+    outs = [np.zeros(int(sizes[k]), np.uint8) if shadow[k] else np.zeros(int(sizes[k]), rt.HIT_DTYPE)
+            for k in range(n_req)]
     srv.stats(reset=True)
-    done_q = []
-    lock = threading.Lock()
+    done_q = queue.SimpleQueue()
 
     def waiter():
-        k = 0
-        while k < n_req:
-            with lock:
-                rid = done_q[k] if k < len(done_q) else None
-            if rid is None:
-                time.sleep(0.0002)
-                continue
+        for _ in range(n_req):
+            rid = done_q.get()
             try:
-                srv.wait(rid, 120)
+                srv.wait(rid, 120)  # the server frees its hold on the output buffer
             except rt.RtError:
                 pass  # counted as failed by the server
-            with lock:
-                done_q[k] = None  # drop the reference (the output buffer is freed)
-            k += 1
 
     th = threading.Thread(target=waiter)
     th.start()
@@ -357,9 +354,9 @@ def serve_run(rt, srv, sids, world, rate, seconds, seed):
             time.sleep(dt)
         w = world[scene_of[i]]
         src = w[4] if shadow[i] else w[3]
-        rid, _ = srv.submit(sids[scene_of[i]], src[offs[i]:offs[i] + sizes[i]], "any" if shadow[i] else "closest")
-        with lock:
-            done_q.append(rid)
+        rid, _ = srv.submit(sids[scene_of[i]], src[offs[i]:offs[i] + sizes[i]], "any" if shadow[i] else "closest",
+                            out=outs[i])
+        done_q.put(rid)
     submit_s = time.perf_counter() - start
     th.join()
     st = srv.stats()

@@ -99,7 +99,8 @@ class RtBatchItem(ctypes.Structure):
 class RtServerConfig(ctypes.Structure):
     _fields_ = [("hbm_cap_bytes", ctypes.c_uint64), ("slots", ctypes.c_uint32),
                 ("max_items", ctypes.c_uint32), ("block_size", ctypes.c_uint32),
-                ("stack_window", ctypes.c_uint32), ("use_graph", ctypes.c_uint32)]
+                ("stack_window", ctypes.c_uint32), ("use_graph", ctypes.c_uint32),
+                ("copy_threads", ctypes.c_uint32)]
 
 
 class RtServerStats(ctypes.Structure):

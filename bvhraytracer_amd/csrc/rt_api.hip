@@ -70,6 +70,7 @@ struct Workspace {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     uint64_t lastUse = 0;  // scene's call counter at the last use
+    uint32_t users = 0;    // calls waiting for their launch outside the scene lock
 };
 
 // A scene keeps the scratch of at most this many streams; the least recently
@@ -137,7 +138,6 @@ struct rt_scene {
     hipEvent_t joinEv = nullptr;  // orders ownStream with the null stream
     float lastMs = 0;
     uint64_t raysTraced = 0;
-    uint32_t errorRays = 0;
 };
 
 namespace {
@@ -343,13 +343,19 @@ int get_workspace(rt_scene* s, hipStream_t st, Workspace*& out) {
             // calls wait for their copies before returning).  Its buffers and
             // events pass to the new stream instead of being freed, so streams
             // used in rotation do not reallocate on every call.
-            auto lru = s->ws.begin();
+            // scratch that a call is still waiting on is never taken; when
+            // every kept scratch is busy the set grows past the cap until
+            // those calls return
+            auto lru = s->ws.end();
             for (auto j = s->ws.begin(); j != s->ws.end(); ++j)
-                if (j->second.lastUse < lru->second.lastUse) lru = j;
-            if (lru->second.timed) HIP_TRY(hipEventSynchronize(lru->second.ev1), "hipEventSynchronize(scratch)");
-            reuse = lru->second;
-            reuse.timed = false;
-            s->ws.erase(lru);
+                if (j->second.users == 0 && (lru == s->ws.end() || j->second.lastUse < lru->second.lastUse)) lru = j;
+            if (lru != s->ws.end()) {
+                if (lru->second.timed)
+                    HIP_TRY(hipEventSynchronize(lru->second.ev1), "hipEventSynchronize(scratch)");
+                reuse = lru->second;
+                reuse.timed = false;
+                s->ws.erase(lru);
+            }
         }
         it = s->ws.emplace(st, reuse).first;
     }
@@ -357,6 +363,7 @@ int get_workspace(rt_scene* s, hipStream_t st, Workspace*& out) {
     w.lastUse = ++s->useClock;
     if (!w.work) {
         HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w.work), kWorkWords * 4), "hipMalloc(work)");
+        HIP_TRY(launch_zero(w.work, kWorkWords, st), "zero kernel");  // the guard counter starts at 0
         HIP_TRY(hipEventCreate(&w.ev0), "hipEventCreate");
         HIP_TRY(hipEventCreate(&w.ev1), "hipEventCreate");
     }
@@ -366,6 +373,7 @@ int get_workspace(rt_scene* s, hipStream_t st, Workspace*& out) {
 
 // Grows a device buffer to at least `need` bytes, by at least half its old
 // size, so a sequence of slowly growing calls reallocates O(log) times.
+// Callers make sure no queued work still uses the old buffer.
 int ensure(void** p, size_t& have, size_t need, const char* what) {
     if (have >= need) return RT_OK;
     const size_t grow = std::max(need, have + have / 2);
@@ -596,9 +604,6 @@ extern "C" int rt_trace_steps(rt_scene_t s, const rt_ray* rays, uint32_t n, uint
         if (cntBytes) HIP_TRY(hipMemcpyAsync(counts, dCounts, cntBytes, hipMemcpyDeviceToHost, st), "hipMemcpy(counts)");
     }
     HIP_TRY(hipStreamSynchronize(st), "step statistics kernel");
-    uint32_t errs = 0;
-    HIP_TRY(hipMemcpy(&errs, w->work + kErrWord, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
-    s->errorRays += errs;
     s->raysTraced += n;
     return RT_OK;
 }
@@ -620,7 +625,7 @@ int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, voi
     if ((flags & RT_STREAM) && (flags & RT_SAVE_DIST))
         return set_error(RT_ERR_UNSUPPORTED, "rt_trace: the ray-stream traversal has no save-distance form");
     const int mode = (flags & RT_ANY) ? kAny : ((flags & RT_SAVE_DIST) ? kSaveDist : kClosest);
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk(s->mu);
     int prevDev = 0;
     (void)hipGetDevice(&prevDev);
     if (prevDev != s->device) HIP_TRY(hipSetDevice(s->device), "hipSetDevice");
@@ -664,9 +669,6 @@ int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, voi
     const size_t slots = (size_t)grid * bd;
     const size_t spillNeed =
         std::max<size_t>(64, slots * spillCap * 4 * (hist ? 3 : (mode == kSaveDist ? 2 : 1)));
-    if ((rc = ensure(reinterpret_cast<void**>(&w->spill), w->spillBytes, spillNeed, "hipMalloc(spill)")) != RT_OK)
-        return rc;
-
     const bool dev = (flags & RT_DEVICE_PTRS) != 0;
     const size_t outBytes = (mode == kAny) ? (size_t)n : (size_t)n * sizeof(rt_hit);
     const size_t rayBytes = (size_t)n * sizeof(rt_ray);
@@ -674,10 +676,17 @@ int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, voi
     const rt_ray* dRays = rays;
     void* dOut = out;
     rt_counts* dCounts = (flags & RT_COUNTS) ? counts : nullptr;
+    const size_t outOff = (rayBytes + 255) / 256 * 256;
+    const size_t cntOff = outOff + (outBytes + 255) / 256 * 256;
+    const size_t stageNeed = dev ? 0 : cntOff + cntBytes + 256;
+    // a call on the same stream may still wait on this scratch (outside the
+    // lock): its queued work ends before a buffer is replaced
+    if (w->users && (spillNeed > w->spillBytes || stageNeed > w->stageBytes))
+        HIP_TRY(hipStreamSynchronize(st), "hipStreamSynchronize(scratch)");
+    if ((rc = ensure(reinterpret_cast<void**>(&w->spill), w->spillBytes, spillNeed, "hipMalloc(spill)")) != RT_OK)
+        return rc;
     if (!dev) {
-        const size_t outOff = (rayBytes + 255) / 256 * 256;
-        const size_t cntOff = outOff + (outBytes + 255) / 256 * 256;
-        if ((rc = ensure(&w->stage, w->stageBytes, cntOff + cntBytes + 256, "hipMalloc(stage)")) != RT_OK) return rc;
+        if ((rc = ensure(&w->stage, w->stageBytes, stageNeed, "hipMalloc(stage)")) != RT_OK) return rc;
         char* base = static_cast<char*>(w->stage);
         dRays = reinterpret_cast<const rt_ray*>(base);
         dOut = base + outOff;
@@ -727,20 +736,27 @@ int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, voi
         HIP_TRY(hipMemcpyAsync(out, dOut, outBytes, hipMemcpyDeviceToHost, st), "hipMemcpy(out)");
         if (cntBytes) HIP_TRY(hipMemcpyAsync(counts, dCounts, cntBytes, hipMemcpyDeviceToHost, st), "hipMemcpy(counts)");
     }
+    if (hist)
+        HIP_TRY(hipMemcpyAsync(hist, w->hist, sizeof(rt_trace_hist), hipMemcpyDeviceToHost, st), "hipMemcpy(hist)");
     if (dev && (flags & RT_NO_SYNC) && joinNull) {
         const int jr = null_stream_after(st, s->joinEv);
         if (jr != RT_OK) return jr;
     }
-    if (!dev || !(flags & RT_NO_SYNC)) {
-        HIP_TRY(hipStreamSynchronize(st), "trace kernel");
-        float ms = 0;
-        if (hipEventElapsedTime(&ms, w->ev0, w->ev1) == hipSuccess) s->lastMs = ms;
-        uint32_t errs = 0;
-        HIP_TRY(hipMemcpy(&errs, w->work + kErrWord, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
-        s->errorRays += errs;
-    }
+    if (dev && (flags & RT_NO_SYNC) && !hist) return RT_OK;
+    // Wait without the scene lock, so that calls on other streams enqueue and
+    // run meanwhile; the busy count keeps this scratch from being handed to
+    // another stream until the wait ends.
+    ++w->users;
+    const hipEvent_t ev0 = w->ev0, ev1 = w->ev1;
+    lk.unlock();
+    const hipError_t e = hipStreamSynchronize(st);
+    float ms = 0;
+    const bool timed = e == hipSuccess && hipEventElapsedTime(&ms, ev0, ev1) == hipSuccess;
+    lk.lock();
+    --w->users;
+    if (timed) s->lastMs = ms;
+    if (e != hipSuccess) return hip_fail(e, "trace kernel");
     if (hist) {
-        HIP_TRY(hipMemcpy(hist, w->hist, sizeof(rt_trace_hist), hipMemcpyDeviceToHost), "hipMemcpy(hist)");
         hist->rays = n;
         hist->node_visits = hist->leaf_visits = 0;
         for (int i = 0; i < RT_HIST_DEPTHS; ++i) {
@@ -873,7 +889,18 @@ extern "C" int rt_stats(rt_scene_t s, rt_stats_t* o) {
     o->block_size = s->blockSize;
     o->stack_window = s->stackWindow;
     o->max_stack = stack_capacity(s);
-    o->error_rays = s->errorRays;
+    // guard counters accumulate on the device, one per stream's scratch;
+    // launches still in flight are waited for
+    uint64_t errs = 0;
+    for (auto& kv : s->ws) {
+        Workspace& w = kv.second;
+        if (!w.work) continue;
+        if (w.timed) HIP_TRY(hipEventSynchronize(w.ev1), "hipEventSynchronize(stats)");
+        uint32_t v = 0;
+        HIP_TRY(hipMemcpy(&v, w.work + kErrWord, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
+        errs += v;
+    }
+    o->error_rays = (uint32_t)std::min<uint64_t>(errs, 0xFFFFFFFFu);
     return RT_OK;
 }
 
@@ -912,7 +939,6 @@ struct rt_batch {
     uint32_t* graphSpill = nullptr;
     float lastMs = 0;
     uint64_t raysTraced = 0;
-    uint32_t errorRays = 0;
     uint64_t graphLaunches = 0;
 };
 
@@ -1002,7 +1028,9 @@ extern "C" int rt_batch_create(uint32_t max_items, rt_batch_t* out) {
         (e = hipHostMalloc(reinterpret_cast<void**>(&b->hItems), sizeof(BatchItem) * max_items, hipHostMallocDefault)) != hipSuccess ||
         (e = hipHostMalloc(reinterpret_cast<void**>(&b->hScenes), sizeof(BatchScene) * max_items, hipHostMallocDefault)) != hipSuccess ||
         (e = hipMalloc(reinterpret_cast<void**>(&b->work), kWorkWords * 4)) != hipSuccess ||
-        (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess) {
+        (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess ||
+        // the guard counter accumulates from here
+        (e = launch_zero(b->work, kWorkWords, nullptr)) != hipSuccess || (e = hipStreamSynchronize(nullptr)) != hipSuccess) {
         free_batch(b);
         return hip_fail(e, "rt_batch_create");
     }
@@ -1157,9 +1185,6 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
         b->pending = false;
         float ms = 0;
         if (hipEventElapsedTime(&ms, b->ev0, b->ev1) == hipSuccess) b->lastMs = ms;
-        uint32_t errs = 0;
-        HIP_TRY(hipMemcpy(&errs, b->work + kErrWord, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
-        b->errorRays += errs;
     }
     return RT_OK;
 }
@@ -1178,7 +1203,11 @@ extern "C" int rt_batch_stats(rt_batch_t b, rt_stats_t* o) {
     o->block_size = b->blockSize;
     o->stack_window = b->stackWindow;
     o->max_stack = b->graphSpillCap + b->stackWindow;
-    o->error_rays = b->errorRays;
+    // the guard counter accumulates on the device over the batch's launches
+    if (b->pending) HIP_TRY(hipEventSynchronize(b->ev1), "hipEventSynchronize(stats)");
+    uint32_t errs = 0;
+    HIP_TRY(hipMemcpy(&errs, b->work + kErrWord, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
+    o->error_rays = errs;
     return RT_OK;
 }
 

@@ -22,7 +22,9 @@ namespace rt {
 enum TraceMode { kClosest = 0, kAny = 1, kSaveDist = 2, kMixed = 3 };
 
 // Work counters: the ray counter and the guarded-condition counter, each on
-// its own 128-byte line.
+// its own 128-byte line.  launch_reset zeroes the ray counter's line before
+// every launch; the guard counter accumulates from the block's creation
+// (zeroed then with launch_zero) and is read by the statistics calls.
 constexpr uint32_t kRayCounter = 0;
 constexpr uint32_t kErrWord = 32;  // rays stopped by a guard
 constexpr uint32_t kWorkWords = 64;
@@ -113,6 +115,7 @@ uint32_t coop_width(uint32_t M, uint32_t Lm);
 hipError_t launch_coop(uint32_t W, int mode, uint32_t grid, const KParams& p, hipStream_t stream);
 int coop_occupancy(uint32_t W, int mode, uint32_t depth);
 
+// Zeroes the ray counter before a launch (not the guard counter).
 hipError_t launch_reset(uint32_t* work, hipStream_t stream);
 
 // Self-test of the leaf test's reciprocal against the correctly rounded

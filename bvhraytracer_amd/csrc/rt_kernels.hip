@@ -199,6 +199,7 @@ struct Trav {
     uint32_t live;    // slots whose aabbDistances entry is not NaN
     uint32_t hitEntry;
     int hitSlot;
+    float b1, b2;     // barycentrics of the hit (when the step tracks them)
     uint32_t cN, cL, cNS, cLS;
     bool hasCur, occl, err, anyMode, rev, fast;
     int axis;
@@ -229,6 +230,7 @@ __device__ __forceinline__ void trav_init(Trav& t, const rt_ray& in, uint32_t ro
     t.live = 0;
     t.hitEntry = 0;
     t.hitSlot = -1;
+    t.b1 = t.b2 = 0.0f;
     t.occl = false;
     t.err = false;
     t.anyMode = anyMode;
@@ -259,7 +261,9 @@ enum HistWord : uint32_t {
 
 // One traversal step: take the next entry and visit it (node or leaf).
 // HIST adds the visit statistics (depth per stack entry, LDS counters).
-template <int MT, int MODE, int S, bool HIST = false>
+// TRACK keeps the closest hit's barycentrics and slot as the leaf test finds
+// them (the finish then writes them without re-running the triangle test).
+template <int MT, int MODE, int S, bool HIST = false, bool TRACK = false>
 __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk, Trav& t) {
     const uint32_t M = MT ? (uint32_t)MT : sc.M;
     const uint32_t recw = MT ? record_words(MT) : sc.recw;
@@ -327,9 +331,16 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
             // (slot % G) keeps its first strict minimum (tStore starts at
             // 1e6), the result is the largest kept slot among instances at
             // the minimum.  Kept here as the minimum t and the mask of slots
-            // at it; gang_slot picks the result from the mask.
+            // at it; gang_slot picks the result from the mask.  With TRACK
+            // the result is followed slot by slot instead: slots come in
+            // increasing order, so a slot at the minimum whose instance has
+            // no earlier slot there is the largest kept slot so far.
             float bt = 1000000.0f;
             uint32_t hm = 0;
+            int win = -1;
+            float wb1 = 0.0f, wb2 = 0.0f;
+            // slots of one program instance: every G-th bit (G = 4, 8, 16)
+            const uint32_t inst = TRACK ? 0xFFFFu / ((1u << sc.G) - 1u) : 0u;
             for (uint32_t s0 = 0; s0 == 0 || s0 < kt; s0 += LEAF_CHUNK) {
                 LeafChunk a;
                 a.load(T, k, s0);
@@ -342,6 +353,12 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
                     if constexpr (HIST) triHits += (h && s < prims) ? 1u : 0u;
                     const bool less = h && tt < bt;
                     const bool same = h && tt == bt && hm != 0;
+                    if constexpr (TRACK) {
+                        const bool take = less || (same && (hm & (inst << (s & (sc.G - 1u)))) == 0);
+                        win = take ? (int)s : win;
+                        wb1 = take ? b1 : wb1;
+                        wb2 = take ? b2 : wb2;
+                    }
                     bt = less ? tt : bt;
                     hm = less ? (1u << s) : (same ? (hm | (1u << s)) : hm);
                 }
@@ -349,7 +366,13 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
             if (hm) {
                 t.ray.tmax = bt;
                 t.hitEntry = e;
-                t.hitSlot = gang_slot(hm, sc.G);
+                if constexpr (TRACK) {
+                    t.hitSlot = win;
+                    t.b1 = wb1;
+                    t.b2 = wb2;
+                } else {
+                    t.hitSlot = gang_slot(hm, sc.G);
+                }
             }
         }
         if constexpr (HIST) {
@@ -517,6 +540,9 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
 }
 
 // Writes the result of a finished ray.
+// TRACKED: the step kept the hit's barycentrics (trav_step<..., TRACK>);
+// otherwise they are recomputed from the hit triangle.
+template <bool TRACKED = false>
 __device__ __forceinline__ void trav_finish(const SceneCtx& sc, const Trav& t, rt_hit* hits, uint8_t* occluded,
                                             rt_counts* counts, uint32_t r, uint32_t* errCounter) {
     if (t.anyMode) {
@@ -525,11 +551,14 @@ __device__ __forceinline__ void trav_finish(const SceneCtx& sc, const Trav& t, r
         rt_hit h;
         if (t.hitSlot >= 0) {
             const uint32_t ord = t.hitEntry & kEntryIndexMask;
-            const uint32_t k = ((t.hitEntry >> kEntryKShift) & 3u) * 4u + 4u;
-            const float* T = sc.tris + (size_t)ord * leaf_stride(sc.Lm) + t.hitSlot;
-            float tt, b1, b2;
-            (void)mt(T[0], T[k], T[2 * k], T[3 * k], T[4 * k], T[5 * k], T[6 * k], T[7 * k], T[8 * k], t.ray, tt,
-                     b1, b2);
+            float b1 = t.b1, b2 = t.b2;
+            if constexpr (!TRACKED) {
+                const uint32_t k = ((t.hitEntry >> kEntryKShift) & 3u) * 4u + 4u;
+                const float* T = sc.tris + (size_t)ord * leaf_stride(sc.Lm) + t.hitSlot;
+                float tt;
+                (void)mt(T[0], T[k], T[2 * k], T[3 * k], T[4 * k], T[5 * k], T[6 * k], T[7 * k], T[8 * k], t.ray,
+                         tt, b1, b2);
+            }
             h.tri = sc.triIds[(size_t)ord * sc.Lm + t.hitSlot];
             h.leaf = ord * 9u * sc.Lm;
             h.slot = t.hitSlot;
@@ -715,9 +744,9 @@ __device__ __forceinline__ void trace_body(const KParams& p) {
             const uint32_t nn = (uint32_t)__popcll(__ballot(!leafNext));
             if (leafNext != (16u * nl >= p.sched * nn)) continue;
         }
-        trav_step<MT, MODE, S, HIST>(sc, sk, t);
+        trav_step<MT, MODE, S, HIST, !HIST>(sc, sk, t);
         if (!trav_done(t)) continue;
-        trav_finish(sc, t, hits, occluded, counts, r, p.work + kErrWord);
+        trav_finish<!HIST>(sc, t, hits, occluded, counts, r, p.work + kErrWord);
         has = false;
     }
     if constexpr (HIST) {
@@ -1234,8 +1263,11 @@ __global__ void __launch_bounds__(256) k_coop(KParams p) {
     }
 }
 
+// Zeroes the ray counter's line before a launch.  The guard counter (its
+// own line, kErrWord) is not reset: it accumulates over the launches that
+// share the work block and the host reads it when it reports statistics.
 __global__ void k_reset(uint32_t* work) {
-    for (uint32_t i = threadIdx.x; i < kWorkWords; i += blockDim.x) work[i] = 0;
+    for (uint32_t i = threadIdx.x; i < kErrWord; i += blockDim.x) work[i] = 0;
 }
 
 // Self-test of recip(): for the float bit patterns first, first + 1, ...

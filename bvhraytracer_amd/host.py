@@ -348,9 +348,9 @@ class Server:
     footprint, continuous batching into batched persistent launches."""
 
     def __init__(self, hbm_cap_bytes=0, slots=0, max_items=0, block_size=0, stack_window=0,
-                 use_graph=True):
+                 use_graph=True, copy_threads=0):
         cfg = RtServerConfig(int(hbm_cap_bytes), int(slots), int(max_items), int(block_size),
-                             int(stack_window), int(bool(use_graph)))
+                             int(stack_window), int(bool(use_graph)), int(copy_threads))
         h = ctypes.c_void_p()
         check(lib().rt_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
@@ -363,10 +363,15 @@ class Server:
         self._scenes.append(scene)
         return sid.value
 
-    def submit(self, scene_id, rays, mode="closest"):
-        """Returns (request id, output array filled when the request completes)."""
+    def submit(self, scene_id, rays, mode="closest", out=None):
+        """Returns (request id, output array filled when the request completes).
+        out: optional caller-owned result array (HIT_DTYPE or uint8, len(rays))."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
-        out = np.empty(len(rays), np.uint8) if mode == "any" else np.empty(len(rays), HIT_DTYPE)
+        want = np.dtype(np.uint8) if mode == "any" else HIT_DTYPE
+        if out is None:
+            out = np.empty(len(rays), want)
+        elif out.dtype != want or len(out) != len(rays) or not out.flags.c_contiguous:
+            raise ValueError("submit: out must be a contiguous array of len(rays) results of the mode's type")
         rid = ctypes.c_uint64()
         check(lib().rt_server_submit(self._h, int(scene_id), ptr(rays), len(rays), MODES[mode], ptr(out),
                                      ctypes.byref(rid)))

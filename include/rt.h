@@ -141,10 +141,14 @@ int rt_upload_bvh(const rt_bvh_desc* desc, rt_scene_t* out);
  * and RT_COOP with RT_STREAM, return RT_ERR_UNSUPPORTED.
  * Any number of host threads may call into one scene at once (the ISPC
  * kernels this replaces are called from OpenMP workers, cameraFast.h:320,465):
- * the scene serialises its host-side work, each hip_stream gets its own
- * scratch, and rt_last_error() reports the calling thread's own error.  A
- * scene keeps the scratch of at most 16 streams; when another stream
- * arrives, the least recently used one is freed after its last launch. */
+ * the scene serialises the enqueueing of the launches, a call that waits for
+ * its results waits without the scene's lock (so calls on different streams
+ * overlap on the device), each hip_stream gets its own scratch, and
+ * rt_last_error() reports the calling thread's own error.  A scene keeps the
+ * scratch of 16 streams; when another stream arrives, the least recently
+ * used scratch that no call is waiting on passes to it after its last launch
+ * (when every kept scratch is waited on, the set grows until those calls
+ * return). */
 int rt_trace(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags,
              void* out, rt_counts* counts, void* hip_stream);
 
@@ -181,7 +185,9 @@ typedef struct rt_stats_t {
     uint32_t stack_window;    /* LDS stack entries per lane                    */
     uint32_t max_stack;       /* stack capacity per ray (LDS + spill)          */
     uint32_t error_rays;      /* rays stopped by a guard: stack capacity exceeded,
-                                 or a padded child id past the node array       */
+                                 or a padded child id past the node array; the
+                                 counters accumulate on the device and rt_stats
+                                 waits for launches still in flight to read them */
 } rt_stats_t;
 
 int rt_stats(rt_scene_t s, rt_stats_t* out);
@@ -322,6 +328,8 @@ typedef struct rt_server_config {
     uint32_t block_size;    /* launch configuration (0 = defaults)                 */
     uint32_t stack_window;
     uint32_t use_graph;     /* replay captured hipGraphs (1) or launch directly (0)  */
+    uint32_t copy_threads;  /* host threads that copy request rays into the pinned
+                               staging pool and results out of it (0 = 4)       */
 } rt_server_config;
 
 typedef struct rt_server_stats {

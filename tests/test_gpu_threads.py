@@ -151,3 +151,57 @@ def test_many_streams_keep_bounded_scratch(rt, world):
     for i, out in enumerate(outs):
         assert torch.equal(out, ref), i
     assert grown <= 17 * max(one, 1 << 20), (grown, one)
+
+
+def test_waiting_call_does_not_block_other_streams(rt, world):
+    """rt_trace waits for its launch without holding the scene's lock.  While
+    thread A makes calls that each trace 1M rays and wait for them, thread B
+    enqueues small calls on its own stream without waiting (RT_NO_SYNC): B's
+    calls return in far less than one of A's waits (holding the lock across
+    the wait would make each of B's calls wait for A's current call).  Every
+    result equals the call made alone."""
+    import time
+
+    import torch
+
+    scene, rays, alone = world
+    big = np.resize(rays, 1 << 20)
+    d_big = torch.from_numpy(big.view(np.float32).reshape(-1, 10)).cuda()
+    out_big = torch.empty((len(big), 6), dtype=torch.int32, device="cuda")
+    n = 4096
+    d_small = torch.from_numpy(rays[:n].view(np.float32).reshape(-1, 10)).cuda()
+    out_small = torch.empty((n, 6), dtype=torch.int32, device="cuda")
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    scene.intersect_device(d_big, out_big, stream=sa)  # scratch of both streams exists
+    scene.intersect_device(d_small, out_small, stream=sb)
+    t = time.perf_counter()
+    scene.intersect_device(d_big, out_big, stream=sa)
+    one_wait = time.perf_counter() - t
+    started = threading.Event()
+    a_done = []
+
+    def a_work():
+        started.set()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            scene.intersect_device(d_big, out_big, stream=sa)
+        a_done.append(time.perf_counter() - t0)
+
+    th = threading.Thread(target=a_work)
+    th.start()
+    started.wait()
+    time.sleep(one_wait / 2)  # A is inside a call, waiting
+    lat = []
+    for _ in range(100):
+        t0 = time.perf_counter()
+        scene.intersect_device(d_small, out_small, stream=sb, sync=False)
+        lat.append(time.perf_counter() - t0)
+    b_total = sum(lat)
+    th.join()
+    sb.synchronize()
+    print(f"A: 20 waiting calls of 1M rays in {a_done[0] * 1e3:.1f} ms ({one_wait * 1e3:.2f} ms alone); "
+          f"B: 100 queued calls in {b_total * 1e3:.2f} ms, median {np.median(lat) * 1e6:.0f} us", flush=True)
+    ref = torch.from_numpy(np.resize(alone["closest"][0], 1 << 20).view(np.int32).reshape(-1, 6)).cuda()
+    assert torch.equal(out_big, ref)
+    assert torch.equal(out_small, ref[:n])
+    assert np.median(lat) < 0.25 * one_wait, (np.median(lat), one_wait)
