@@ -1,8 +1,11 @@
 """ORACLE — test infrastructure, not product code.
 
 ctypes binding of liboracle.so (cpu_ref.c), the CPU restatement of the
-reference traversal.  Only tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg use it, and only as the checker or the timed CPU baseline.
+reference traversal, and of liboracle_simd.so (cpu_simd.c), the same
+traversals with 4-wide SSE node and leaf tests (the shape of the reference's
+ISPC kernels; bit-identical results).  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg use it, and only as the checker or the timed
+CPU baseline.
 """
 import ctypes
 import os
@@ -11,6 +14,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+SIMD_LIB_PATH = os.path.join(HERE, "liboracle_simd.so")
 
 ORC_CLOSEST, ORC_ANY, ORC_SAVE_DIST = 0, 1, 2
 ORC_WIDE_AXIS_BUG, ORC_WIDE_SHARED_DIST, ORC_WIDE_ROOT_AS_NODE = 1, 2, 4
@@ -33,15 +37,16 @@ class OrcBvh(ctypes.Structure):
 
 
 _P = ctypes.c_void_p
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise ImportError(f"{LIB_PATH} missing: run make -C oracle")
-        L = ctypes.CDLL(LIB_PATH)
+def lib(simd=False):
+    """liboracle.so, or with simd=True liboracle_simd.so (same entry points)."""
+    if simd not in _libs:
+        path = SIMD_LIB_PATH if simd else LIB_PATH
+        if not os.path.exists(path):
+            raise ImportError(f"{path} missing: run make -C oracle")
+        L = ctypes.CDLL(path)
         sig = {
             "orc_aabb_intersect": (ctypes.c_int, [_P, ctypes.c_int, _P, _P]),
             "orc_tri_intersect": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int, _P, ctypes.c_int]),
@@ -65,8 +70,8 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _lib = L
-    return _lib
+        _libs[simd] = L
+    return _libs[simd]
 
 
 def _p(a):
@@ -74,9 +79,11 @@ def _p(a):
 
 
 class Oracle:
-    """Oracle view of a set of FastNode arrays (keeps the arrays alive)."""
+    """Oracle view of a set of FastNode arrays (keeps the arrays alive).
+    simd=True runs the traversals with the 4-wide SSE node and leaf tests."""
 
-    def __init__(self, M, Lm, G, bounds, base, order, child_type, tri_points):
+    def __init__(self, M, Lm, G, bounds, base, order, child_type, tri_points, simd=False):
+        self.simd = bool(simd)
         self.arrays = [np.ascontiguousarray(bounds, np.float32), np.ascontiguousarray(base, np.uint32),
                        np.ascontiguousarray(order, np.int8), np.ascontiguousarray(child_type, np.uint16),
                        np.ascontiguousarray(tri_points, np.float32)]
@@ -87,9 +94,9 @@ class Oracle:
                         t.size, _p(t))
 
     @classmethod
-    def from_bvh(cls, bvh):
+    def from_bvh(cls, bvh, simd=False):
         return cls(bvh.node_memory, bvh.leaf_memory, bvh.gang, bvh.bounds, bvh.base, bvh.order,
-                   bvh.child_type, bvh.tri_points)
+                   bvh.child_type, bvh.tri_points, simd=simd)
 
     def trace(self, rays, mode=ORC_CLOSEST, stream=False, tile=256, threads=0, counts=True):
         rays = np.ascontiguousarray(rays).view(RAY_DTYPE)
@@ -98,7 +105,7 @@ class Oracle:
         occ = np.zeros(n, np.uint8) if mode == ORC_ANY else None
         cnt = np.zeros(n, COUNTS_DTYPE) if counts else None
         diag = np.zeros(n, np.uint32)
-        used = lib().orc_trace(ctypes.byref(self.s), _p(rays), n, mode, int(stream), tile, threads,
+        used = lib(self.simd).orc_trace(ctypes.byref(self.s), _p(rays), n, mode, int(stream), tile, threads,
                                _p(hits), _p(occ), _p(cnt), _p(diag))
         return {"hits": hits, "occluded": occ, "counts": cnt, "diag": diag, "threads": used}
 
@@ -109,7 +116,7 @@ class Oracle:
         n = len(rays)
         ids = np.zeros((n, cap), np.uint32)
         lens = np.zeros(n, np.uint32)
-        lib().orc_visit_sequences(ctypes.byref(self.s), _p(rays), n, mode, _p(ids), cap, _p(lens))
+        lib(self.simd).orc_visit_sequences(ctypes.byref(self.s), _p(rays), n, mode, _p(ids), cap, _p(lens))
         if n and int(lens.max()) > cap:
             raise ValueError(f"a ray takes {int(lens.max())} visits, more than cap {cap}")
         return ids, lens
@@ -121,7 +128,7 @@ class Oracle:
         ti = np.zeros(n, np.int32)
         cnt = np.zeros(n, COUNTS_DTYPE)
         diag = np.zeros(n, np.uint32)
-        lib().orc_intersect_wide(ctypes.byref(self.s), _p(rays), n, flags, _p(li), _p(ti), _p(cnt), _p(diag))
+        lib(self.simd).orc_intersect_wide(ctypes.byref(self.s), _p(rays), n, flags, _p(li), _p(ti), _p(cnt), _p(diag))
         return li, ti, cnt, rays["tmax"].copy()
 
     def secondary_wide(self, rays, flags=0):
@@ -130,7 +137,7 @@ class Oracle:
         res = np.zeros(n, np.uint16)
         cnt = np.zeros(n, COUNTS_DTYPE)
         diag = np.zeros(n, np.uint32)
-        lib().orc_intersect_secondary_wide(ctypes.byref(self.s), _p(rays), n, flags, _p(res), _p(cnt), _p(diag))
+        lib(self.simd).orc_intersect_secondary_wide(ctypes.byref(self.s), _p(rays), n, flags, _p(res), _p(cnt), _p(diag))
         return res, cnt
 
 

@@ -15,6 +15,14 @@
 
 #define ORC_STACK_CAP 1024
 
+/* The node and leaf tests the traversals call.  cpu_simd.c includes this
+ * file with its 4-wide forms of the three tests defined in their place. */
+#ifndef ORC_NODE_TEST
+#define ORC_NODE_TEST orc_aabb_intersect
+#define ORC_LEAF_TEST orc_tri_intersect
+#define ORC_LEAF_ANY orc_tri_anyhit
+#endif
+
 /* ISPC max()/min() on sse4 lower to maxps/minps: (a > b) ? a : b and
  * (a < b) ? a : b — the second operand wins on NaN and on equal values. */
 static inline float smax(float a, float b) { return (a > b) ? a : b; }
@@ -221,7 +229,7 @@ static int intersect_impl(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_index, 
         if (is_leaf(b, id)) {
             const int k = orc_leaf_loop_count(b, b->base[id]);
             count_add(counts, q, id, 1, k);
-            const int r = orc_tri_intersect(b->tri, b->base[id], k, ri, (int)b->G);
+            const int r = ORC_LEAF_TEST(b->tri, b->base[id], k, ri, (int)b->G);
             if (r != -1) {
                 result = 1;
                 *leaf_index = b->base[id];
@@ -231,7 +239,7 @@ static int intersect_impl(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_index, 
             const int k = orc_node_loop_count(b, id);
             count_add(counts, q, id, 0, k);
             const float* bd = b->bounds + (size_t)id * 6 * M;
-            if (orc_aabb_intersect(bd, k, dist, ri)) {
+            if (ORC_NODE_TEST(bd, k, dist, ri)) {
                 const int8_t* ord = b->order + (size_t)id * 3 * M + (size_t)code * M;
                 for (uint32_t q = 0; q < M; ++q) {
                     const int c = reverse ? ord[q] : ord[M - 1 - q];
@@ -282,7 +290,7 @@ static int save_distance_impl(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_ind
         if (is_leaf(b, id)) {
             const int k = orc_leaf_loop_count(b, b->base[id]);
             count_add(counts, q, id, 1, k);
-            const int r = orc_tri_intersect(b->tri, b->base[id], k, ri, (int)b->G);
+            const int r = ORC_LEAF_TEST(b->tri, b->base[id], k, ri, (int)b->G);
             if (r != -1) {
                 result = 1;
                 *leaf_index = b->base[id];
@@ -292,7 +300,7 @@ static int save_distance_impl(const orc_bvh* b, orc_ray* ray, uint32_t* leaf_ind
             const int k = orc_node_loop_count(b, id);
             count_add(counts, q, id, 0, k);
             const float* bd = b->bounds + (size_t)id * 6 * M;
-            (void)orc_aabb_intersect(bd, k, dist, ri);
+            (void)ORC_NODE_TEST(bd, k, dist, ri);
             const int8_t* ord = b->order + (size_t)id * 3 * M + (size_t)code * M;
             for (uint32_t q = 0; q < M; ++q) {
                 const int c = reverse ? ord[q] : ord[M - 1 - q];
@@ -337,7 +345,7 @@ static int secondary_impl(const orc_bvh* b, orc_ray* ray, orc_counts* counts, ui
         if (is_leaf(b, id)) {
             const int k = orc_leaf_loop_count(b, b->base[id]);
             count_add(counts, q, id, 1, k);
-            if (orc_tri_anyhit(b->tri, b->base[id], k, ri)) {
+            if (ORC_LEAF_ANY(b->tri, b->base[id], k, ri)) {
                 result = 1;
                 break;
             }
@@ -345,7 +353,7 @@ static int secondary_impl(const orc_bvh* b, orc_ray* ray, orc_counts* counts, ui
             const int k = orc_node_loop_count(b, id);
             count_add(counts, q, id, 0, k);
             const float* bd = b->bounds + (size_t)id * 6 * M;
-            if (orc_aabb_intersect(bd, k, dist, ri)) {
+            if (ORC_NODE_TEST(bd, k, dist, ri)) {
                 for (uint32_t i = 0; i < M; ++i) {
                     if (!isnan(dist[i])) {
                         if (sp >= ORC_STACK_CAP) {
@@ -479,7 +487,7 @@ void orc_intersect_wide(const orc_bvh* b, orc_ray* rays, uint32_t count, uint32_
             } else {
                 const int k = orc_node_loop_count(b, id);
                 if (counts) count_add(&counts[r], NULL, 0, 0, k);
-                if (orc_aabb_intersect(b->bounds + (size_t)id * 6 * M, k, dist, ri)) {
+                if (ORC_NODE_TEST(b->bounds + (size_t)id * 6 * M, k, dist, ri)) {
                     const uint32_t axis = (flags & ORC_WIDE_AXIS_BUG) ? (major[i] >> 1) : (major[r] >> 1);
                     const int8_t* ord = b->order + (size_t)id * 3 * M + (size_t)axis * M;
                     const int fwd = major[r] & 1;
@@ -514,7 +522,7 @@ void orc_intersect_wide(const orc_bvh* b, orc_ray* rays, uint32_t count, uint32_
             } else {
                 const int k = orc_leaf_loop_count(b, b->base[id]);
                 if (counts) count_add(&counts[r], NULL, 0, 1, k);
-                const int res = orc_tri_intersect(b->tri, b->base[id], k, ri, (int)b->G);
+                const int res = ORC_LEAF_TEST(b->tri, b->base[id], k, ri, (int)b->G);
                 if (res != -1) {
                     leaf_index[r] = b->base[id];
                     tri_index[r] = res;
@@ -571,7 +579,7 @@ void orc_intersect_secondary_wide(const orc_bvh* b, orc_ray* rays, uint32_t coun
             } else {
                 const int k = orc_node_loop_count(b, id);
                 if (counts) count_add(&counts[r], NULL, 0, 0, k);
-                if (orc_aabb_intersect(b->bounds + (size_t)id * 6 * M, k, dist, ri)) {
+                if (ORC_NODE_TEST(b->bounds + (size_t)id * 6 * M, k, dist, ri)) {
                     for (uint32_t c = 0; c < M; ++c) {
                         if (!isnan(dist[c])) {
                             if (s.sp[r] >= ORC_STACK_CAP) {
@@ -602,7 +610,7 @@ void orc_intersect_secondary_wide(const orc_bvh* b, orc_ray* rays, uint32_t coun
             } else {
                 const int k = orc_leaf_loop_count(b, b->base[id]);
                 if (counts) count_add(&counts[r], NULL, 0, 1, k);
-                hit = orc_tri_anyhit(b->tri, b->base[id], k, ri);
+                hit = ORC_LEAF_ANY(b->tri, b->base[id], k, ri);
             }
             if (hit) {
                 result[r]++;
