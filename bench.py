@@ -165,11 +165,11 @@ def bench_batched(rt, torch, args, log):
     if not args.no_batch_autotune:
         out["autotuned"] = batched_autotuned(rt, torch, scenes, outs, stream, total, args, log)
     if not args.no_cpu:
-        # the CPU baseline of this workload: the oracle per scene, summed
-        # (one warm-up pass, then one timed pass over all 64 scenes)
+        # the CPU baseline of this workload: the SIMD oracle per scene,
+        # summed (one warm-up pass, then one timed pass over all 64 scenes)
         import oracle
 
-        orcs = [(oracle.Oracle.from_bvh(x[1]), x[3]) for x in scenes]
+        orcs = [(oracle.Oracle.from_bvh(x[1], simd=True), x[3]) for x in scenes]
         for o, rays in orcs:
             o.trace(rays, counts=False)
         t = time.perf_counter()
@@ -184,22 +184,33 @@ def bench_batched(rt, torch, args, log):
     return out
 
 
+BATCH_SHAPES = ((128, 8), (128, 16), (128, 32), (256, 8), (256, 16), (256, 32))
+
+
 def batched_autotuned(rt, torch, scenes, ref_outs, stream, total, args, log):
     """The same 64 scenes with N and L chosen per scene by the autotuner
-    (candidates N4L4, N4L8, N8L4, N8L8 timed on 8192 of the scene's own
-    rays), traced in one batched graph launch that mixes the chosen node and
-    leaf memories.  Tuning and the rebuilds are ingest cost, outside the
-    timing."""
+    (candidates N4L4, N4L8, N8L4, N8L8, each with block size 128/256 and LDS
+    stack window 8/16/32, timed on 8192 of the scene's own rays), traced in
+    one batched graph launch that mixes the chosen node and leaf memories.
+    Block size and stack window are properties of a launch (its blocks and
+    its LDS allocation), so the batch takes one shape: the fastest of
+    BATCH_SHAPES for the whole launch.  The per-scene shape choices are
+    reported with what they gain over the default shape on the scene alone.
+    Tuning and the rebuilds are ingest cost, outside the timing."""
     from bvhraytracer_amd.autotune import Autotuner
 
     t0 = time.time()
-    tuner = Autotuner(nl=((4, 4), (4, 8), (8, 4), (8, 8)), blocks=(256,), windows=(16,), schedules=(16,),
-                      sample=8192, reps=2)
-    tuned, choice = [], {}
+    tuner = Autotuner(nl=((4, 4), (4, 8), (8, 4), (8, 8)), blocks=(128, 256), windows=(8, 16, 32),
+                      schedules=(16,), sample=8192, reps=2)
+    tuned, choice, shapes, gains = [], {}, {}, []
     for tris, bvh0, sc0, rays, d_r, d_h in scenes:
         best = tuner.tune_scene(tris, rays)
         key = f"N{best['N']}L{best['L']}"
         choice[key] = choice.get(key, 0) + 1
+        shape = f"{best['block']}x{best['window']}"
+        shapes[shape] = shapes.get(shape, 0) + 1
+        if best.get("default_mrays"):
+            gains.append(best["mrays"] / best["default_mrays"])
         if (best["N"], best["L"]) == (bvh0.branching, bvh0.leaf_size):
             tuned.append((None, sc0, d_r, torch.empty_like(d_h)))
         else:
@@ -208,18 +219,31 @@ def batched_autotuned(rt, torch, scenes, ref_outs, stream, total, args, log):
     tune_s = time.time() - t0
     batch = rt.Batch(64)
     items = [(sc, d_r, o, "closest") for (bvh, sc, d_r, o) in tuned]
-    sec = median_wall(lambda: batch.trace(items, graph=True, stream=stream, sync=False), torch,
-                      max(3, args.steps // 2), args.warmup)
+    reps = max(3, args.steps // 2)
+    by_shape = {}
+    for bd, win in BATCH_SHAPES:
+        batch.set_launch(bd, win)
+        sec = median_wall(lambda: batch.trace(items, graph=True, stream=stream, sync=False), torch, reps,
+                          args.warmup)
+        by_shape[f"{bd}x{win}"] = round(total / sec / 1e6, 2)
+    best_shape = max(by_shape, key=by_shape.get)
+    bd, win = (int(v) for v in best_shape.split("x"))
+    batch.set_launch(bd, win)
+    sec = median_wall(lambda: batch.trace(items, graph=True, stream=stream, sync=False), torch, reps, args.warmup)
     mrays = total / sec / 1e6
     # the hit triangles are the same whatever the tree, up to exact ties
     same_tri = sum(int((o[:, 0] == r[:, 0]).sum().item()) for (b, s, d, o), r in zip(tuned, ref_outs)) / total
-    log(f"batched autotuned {mrays:.1f} Mrays/s, choices {choice}, tuning {tune_s:.1f}s")
+    log(f"batched autotuned {mrays:.1f} Mrays/s (launch shape {best_shape}; by shape {by_shape}), "
+        f"choices {choice}, per-scene shapes {shapes}, tuning {tune_s:.1f}s")
     batch.close()
     for bvh, sc, d_r, o in tuned:
         if bvh is not None:
             sc.close()
-    return {"batched_mrays": round(mrays, 2), "choices": choice, "tune_seconds": round(tune_s, 1),
-            "same_hit_triangle_fraction": round(same_tri, 6)}
+    return {"batched_mrays": round(mrays, 2), "launch_shape": best_shape, "batched_mrays_by_shape": by_shape,
+            "choices": choice, "per_scene_shapes": shapes,
+            "per_scene_shape_gain_median": round(float(np.median(gains)), 4) if gains else None,
+            "per_scene_shape_gain_max": round(float(np.max(gains)), 4) if gains else None,
+            "tune_seconds": round(tune_s, 1), "same_hit_triangle_fraction": round(same_tri, 6)}
 
 
 def bench_frame(rt, torch, scene, bvh, n_bounce, log, reps=3):
@@ -568,32 +592,36 @@ def main():
     if not args.no_cpu:
         import oracle
 
-        o = oracle.Oracle.from_bvh(bvh)
+        def cpu_time(o, sample, reps, **kw):
+            times = []
+            for _ in range(reps):
+                t = time.perf_counter()
+                r = o.trace(rays[:sample], counts=False, **kw)
+                times.append(time.perf_counter() - t)
+            return sample / float(np.median(times)) / 1e6, int(r["threads"])
+
+        # the traversal with 4-wide SSE node and leaf tests (the reference's
+        # ISPC kernel shape, sse4-i32x4) and the scalar restatement, both on
+        # every host thread with OpenMP dynamic(4) over 256-ray tiles
+        simd, scalar = oracle.Oracle.from_bvh(bvh, simd=True), oracle.Oracle.from_bvh(bvh)
         probe = rays[: min(n, 65536)]
         t = time.perf_counter()
-        r = o.trace(probe, counts=False)
+        simd.trace(probe, counts=False)
         rate = len(probe) / max(time.perf_counter() - t, 1e-6)
         sample = int(min(n, max(65536, rate * args.cpu_seconds)))
-        times = []
-        for _ in range(5):
-            t = time.perf_counter()
-            r = o.trace(rays[:sample], counts=False)
-            times.append(time.perf_counter() - t)
-        cpu_rate = sample / float(np.median(times)) / 1e6
+        cpu_rate, threads = cpu_time(simd, sample, 5)
+        scalar_rate, _ = cpu_time(scalar, sample, 5)
         # the stream form (intersectWideAlternative per 256-ray tile) beside it
-        st = []
-        for _ in range(3):
-            t = time.perf_counter()
-            o.trace(rays[:sample], stream=True, counts=False)
-            st.append(time.perf_counter() - t)
-        stream_rate = sample / float(np.median(st)) / 1e6
-        cpu = {"value": round(cpu_rate, 3), "unit": "Mrays/s", "cores": int(r["threads"]),
-               "kind": "port", "stream_value": round(stream_rate, 3),
+        stream_rate, _ = cpu_time(simd, sample, 3, stream=True)
+        cpu = {"value": round(cpu_rate, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "simd": "4-wide SSE node and leaf tests (gang 4, as the reference's sse4-i32x4 ISPC kernels)",
+               "scalar_value": round(scalar_rate, 3), "stream_value": round(stream_rate, 3),
                "sample": f"first {sample} of the {n} diffuse rays, closest-hit per-ray traversal, "
-                         f"1 warm-up + 5 timed, median (stream_value: the stream traversal per "
-                         f"256-ray tile, 3 timed, median); {cpu_model()}"}
-        log(f"cpu baseline {cpu_rate:.3f} Mrays/s per-ray, {stream_rate:.3f} stream, on {r['threads']} threads "
-            f"({sample} rays)")
+                         f"1 warm-up + 5 timed, median (scalar_value: the scalar restatement, same protocol; "
+                         f"stream_value: the SIMD stream traversal per 256-ray tile, 3 timed, median); "
+                         f"{cpu_model()}"}
+        log(f"cpu baseline {cpu_rate:.3f} Mrays/s per-ray SIMD, {scalar_rate:.3f} scalar, {stream_rate:.3f} "
+            f"stream, on {threads} threads ({sample} rays)")
 
     out = {
         "metric": METRIC,

@@ -55,9 +55,12 @@ class Autotuner:
             best = min(best, scene.stats()["last_kernel_ms"])
         return best
 
+    DEFAULT_LAUNCH = (256, 16, 16)  # block size, stack window, schedule of a new scene
+
     def tune_launch(self, scene, d_rays, d_out, mode="closest"):
         """Fastest (block size, stack window, wave schedule) for this scene
-        and ray set."""
+        and ray set; default_ms is the time of the default launch shape
+        (None when it is not among the candidates)."""
         res = []
         for w in self.windows:
             for b in self.blocks:
@@ -68,7 +71,8 @@ class Autotuner:
         ms, b, w, sc = min(res)
         scene.set_launch(b, w)
         scene.set_schedule(sc)
-        return {"block": b, "window": w, "schedule": sc, "ms": ms}
+        default = [r[0] for r in res if r[1:] == self.DEFAULT_LAUNCH]
+        return {"block": b, "window": w, "schedule": sc, "ms": ms, "default_ms": default[0] if default else None}
 
     def tune_scene(self, tris, rays, mode="closest", key=None):
         """Returns {"N", "L", "block", "window", "schedule", "mrays", "tried"}
@@ -93,7 +97,8 @@ class Autotuner:
             r = self.tune_launch(scene, d_rays, d_out, mode)
             mrays = len(sample) / r["ms"] / 1e3
             cfg = {"N": N, "L": L, "block": r["block"], "window": r["window"], "schedule": r["schedule"],
-                   "mrays": round(mrays, 2)}
+                   "mrays": round(mrays, 2),
+                   "default_mrays": round(len(sample) / r["default_ms"] / 1e3, 2) if r["default_ms"] else None}
             tried.append(cfg)
             if best is None or mrays > best["mrays"]:
                 best = dict(cfg)
