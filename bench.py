@@ -31,15 +31,18 @@ HBM_PEAK_GBPS = 8000.0
 # in KiB.  Keyed by (branching, leaf size, rays); the commands and the
 # per-dispatch values are in profiles/k_trace_N8L8_diffuse8M_hbm_traffic.md.
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
-MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 8383360, "write": 397396},
+MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 8008795, "write": 398439},
                         # batched leg (profiles/k_trace_batched_64scenes_hbm_traffic.md)
-                        ("batch", 8, 8, 9_872_996): {"fetch": 3175441, "write": 409272}}
+                        ("batch", 8, 8, 9_872_996): {"fetch": 3029947, "write": 410084}}
 
 # What binds the timed kernel, from rocprofv3 counter runs of this benchmark
 # (profiles/k_trace_N8L8_diffuse8M_l2_valu_occupancy.md): the VALU pipes are
-# busy 74% of the time with 38.8% of the lanes active, the rest is waits on
-# dependent loads; HBM carries a quarter of its peak.
-MEASURED_COUNTERS = {(8, 8, 8_000_000): {"valu_busy": 0.74, "active_lanes": 0.388, "l2_hit": 0.858,
+# busy 75% of the time with 41.7% of the lanes active, the rest is waits on
+# dependent loads; HBM carries a fifth of its peak, and the L1s request
+# 62.9 GB per step from L2 (9.836e8 64-byte reads), 86% of it hitting there.
+MEASURED_COUNTERS = {(8, 8, 8_000_000): {"valu_busy": 0.747, "active_lanes": 0.417, "l2_hit": 0.864,
+                                         "l2_read_bytes": 62_950_400_000,
+                                         "valu_insts_per_ray": 527.5,
                                          "source": "profiles/k_trace_N8L8_diffuse8M_l2_valu_occupancy.md"}}
 
 
@@ -648,7 +651,10 @@ def main():
                      "useful_bytes": int(useful), "useful_frac": round(useful / (k_ms / 1000.0) / 1e9 / HBM_PEAK_GBPS, 4),
                      "hbm_frac": hbm_frac(traffic, k_ms),
                      "limiter": "VALU issue and dependent-load latency, not HBM bandwidth",
-                     "counters": MEASURED_COUNTERS.get((N, L, n))},
+                     "counters": MEASURED_COUNTERS.get((N, L, n)),
+                     # bytes the L1s read from L2 per step (the counters' pass) over this run's kernel time
+                     "l2_read_gbps": (round(MEASURED_COUNTERS[(N, L, n)]["l2_read_bytes"] / (k_ms / 1000.0) / 1e9, 1)
+                                      if (N, L, n) in MEASURED_COUNTERS else None)},
         "cpu_baseline": cpu,
         "traversals_mrays": variants,
         "bounce_frame": frame,
