@@ -516,14 +516,57 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
         } else {
             for (uint32_t q = 0; q < c; ++q) pm |= ((live >> ((uint32_t)(row >> (4 * q)) & 15u)) & 1u) << q;
         }
-        if (!fits((uint32_t)__popc(pm) + (pad0 ? 1u : 0u))) {
+        const uint32_t npush = (uint32_t)__popc(pm) + (pad0 ? 1u : 0u);
+        const uint32_t rowLo = (uint32_t)row, rowHi = (uint32_t)(row >> 32);
+        if (MODE != kSaveDist && !HIST && __ballot(t.sp + npush > (uint32_t)S) == 0) {
+            // Every push of the wave lands in the LDS windows, so each entry
+            // is written straight to its stack position: push order o goes to
+            // sp + o, and the entry of order npush - 1 becomes the pending
+            // one.  Backward lanes push the row positions in increasing
+            // order, forward lanes (after the padding slot) in decreasing
+            // order; the loop takes the lowest position first for both and
+            // steps the stack position up or down.  Row positions q < c hold
+            // exactly the real slots 0..c-1 (a permutation, checked at
+            // upload), so the pushes clear the live bits of slots 0..c-1.
+            const uint32_t leafBaseF = kEntryLeaf | leafBase;
+            auto entry_fast = [&](uint32_t s) -> uint32_t {
+                const bool isLeaf = __builtin_amdgcn_ubfe(leafMask, s, 1) != 0;
+                const uint32_t m = isLeaf ? leafMask : innerMask;
+                const uint32_t b = isLeaf ? leafBaseF : innerBase;
+                const uint32_t kc = __builtin_amdgcn_ubfe(kcodes, 2 * s, 2);
+                return (kc << kEntryKShift) | (b + (uint32_t)__popc(__builtin_amdgcn_ubfe(m, 0, s)));
+            };
+            if (npush) {
+                const uint32_t last = t.sp + npush - 1u;
+                uint32_t lastE = 0;
+                if (pad0) {
+                    lastE = entry_fast(0);
+                    sk.stE[t.sp * bd] = lastE;
+                }
+                uint32_t pos = t.rev ? t.sp : last;
+                const uint32_t step = t.rev ? 1u : ~0u;
+                while (pm) {
+                    const uint32_t q = (uint32_t)__builtin_ctz(pm);
+                    pm &= pm - 1u;
+                    const uint32_t s = (MT > 0 && MT <= 8) ? __builtin_amdgcn_ubfe(rowLo, 4 * q, 4)
+                                                           : __builtin_amdgcn_ubfe(q < 8 ? rowLo : rowHi, 4 * (q & 7u), 4);
+                    const uint32_t e = entry_fast(s);
+                    sk.stE[pos * bd] = e;
+                    lastE = pos == last ? e : lastE;
+                    pos += step;
+                }
+                t.cur = lastE;
+                t.hasCur = true;
+                t.sp = last;
+            }
+            t.live &= ~((1u << c) - 1u);
+        } else if (!fits(npush)) {
             t.err = true;
         } else {
             if (pad0) push(entry_real(0), 0);
             // one loop for both directions (forward takes the lowest position
             // first, backward the highest), so lanes of either kind share
             // the iterations; row positions q < c hold real slots only
-            const uint32_t rowLo = (uint32_t)row, rowHi = (uint32_t)(row >> 32);
             while (pm) {
                 const uint32_t q = t.rev ? (uint32_t)__builtin_ctz(pm) : 31u - (uint32_t)__builtin_clz(pm);
                 pm &= ~(1u << q);
