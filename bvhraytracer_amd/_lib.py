@@ -119,7 +119,8 @@ STEP_DTYPE = np.dtype([("node_work", "<u4"), ("node_unique", "<u4"), ("leaf_work
                        ("leaf_unique", "<u4"), ("terminations", "<u4")])
 
 
-# (name, restype, argtypes) of every exported entry point in include/rt.h
+# (name, restype, argtypes) of every exported entry point in include/rt.h and
+# include/rt_report.h
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
 SIGNATURES = [
@@ -171,6 +172,22 @@ SIGNATURES = [
     ("rt_spawn_diffuse_device", ctypes.c_int, [_P, _P, _P, _P, _U32, ctypes.c_uint64, _U32,
                                                ctypes.c_float, _P, _P]),
     ("rt_sah_factor", ctypes.c_float, [_U32, _U32, _U32, ctypes.c_char_p]),
+    # include/rt_report.h
+    ("rt_report_read_config", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_size_t,
+                                             ctypes.POINTER(ctypes.c_size_t)]),
+    ("rt_report_write_perf", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _U32, _U32, _U32, _U32, _U32,
+                                            ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                            ctypes.c_uint64]),
+    ("rt_report_write_info", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _U32, _U32, _U32,
+                                            ctypes.POINTER(RtTraceHist), ctypes.POINTER(RtTraceHist)]),
+    ("rt_report_write_workgroup_analysis", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                                          ctypes.c_int32, _U32, _U32, _U32, _P, _P, _P, _P, _P, _P,
+                                                          _P, _P, ctypes.c_int32, ctypes.c_char_p,
+                                                          ctypes.c_size_t]),
+    ("rt_report_tile_order_to_rows", ctypes.c_int, [_P, _U32, _U32, _U32, _U32, _P]),
+    ("rt_report_write_png", ctypes.c_int, [ctypes.c_char_p, _P, _U32, _U32]),
+    ("rt_report_ambient_gray", ctypes.c_int, [_P, _P, _U32, _U32, _P]),
+    ("rt_report_ambient_distance", ctypes.c_int, [_P, _U32, ctypes.POINTER(ctypes.c_float)]),
 ]
 
 _lib = None

@@ -18,7 +18,7 @@ OUT = os.path.join(HERE, "librt.so")
 CLI = os.path.join(HERE, "bvhrt")
 OBJ = os.path.join(HERE, "_build")
 
-HOST_SOURCES = ["scene.cpp", "builder.cpp", "host_api.cpp", "renderer.cpp"]
+HOST_SOURCES = ["scene.cpp", "builder.cpp", "host_api.cpp", "renderer.cpp", "report_api.cpp"]
 HIP_SOURCES = ["rt_kernels.hip", "rt_api.hip", "rt_serve.hip", "rt_shade.hip", "rt_noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

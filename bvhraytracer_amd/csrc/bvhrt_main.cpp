@@ -10,35 +10,7 @@
 
 namespace {
 
-void print_config(const rt::RenderConfig& c) {
-    const std::pair<const char*, int> ints[] = {
-        {"minLeafSize", c.minLeafSize},     {"maxLeafSize", c.maxLeafSize},   {"leafStep", c.leafStep},
-        {"minBranch", c.minBranch},         {"maxBranch", c.maxBranch},       {"branchStep", c.branchStep},
-        {"subdivisionStart", c.subdivisionStart}, {"subdivisionEnd", c.subdivisionEnd},
-        {"subdivisionStep", c.subdivisionStep},   {"renderType", c.renderType},
-        {"ambientSampleCount", c.ambientSampleCount}, {"workGroupSize", c.workGroupSize},
-        {"leafSplitOption", c.leafSplitOption},   {"xRes", c.xRes},             {"yRes", c.yRes}};
-    const std::pair<const char*, bool> bools[] = {
-        {"saveImage", c.saveImage},         {"saveDepthDetailedImage", c.saveDepthDetailedImage},
-        {"bvhAnalysis", c.bvhAnalysis},     {"saveBvhImage", c.saveBvhImage},
-        {"sortEachSplit", c.sortEachSplit}, {"renderAnalysisImage", c.renderAnalysisImage},
-        {"castShadows", c.castShadows},     {"doPerformanceTest", c.doPerformanceTest},
-        {"doNodeMemoryTest", c.doNodeMemoryTest}, {"doLeafMemoryTest", c.doLeafMemoryTest},
-        {"saveRayTimes", c.saveRayTimes},   {"doWorkGroupAnalysis", c.doWorkGroupAnalysis},
-        {"saveDistance", c.saveDistance},   {"wideRender", c.wideRender},
-        {"wideAlternative", c.wideAlternative}, {"renderAllOptions", c.renderAllOptions},
-        {"doCacheAnalysis", c.doCacheAnalysis}};
-    for (const auto& kv : ints) std::cout << kv.first << "=" << kv.second << "\n";
-    for (const auto& kv : bools) std::cout << kv.first << "=" << (kv.second ? "true" : "false") << "\n";
-    auto list = [](const char* k, const std::vector<int>& v) {
-        std::cout << k << "=";
-        for (size_t i = 0; i < v.size(); ++i) std::cout << (i ? "," : "") << v[i];
-        std::cout << "\n";
-    };
-    list("scenarios", c.scenarios);
-    list("cacheSizes", c.cacheSizes);
-    for (const auto& w : c.warnings) std::cout << "warning=" << w << "\n";
-}
+void print_config(const rt::RenderConfig& c) { std::cout << rt::config_text(c); }
 
 }  // namespace
 

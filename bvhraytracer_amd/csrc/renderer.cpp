@@ -112,6 +112,38 @@ bool read_config(const std::string& path, RenderConfig& c) {
     return true;
 }
 
+std::string config_text(const RenderConfig& c) {
+    const std::pair<const char*, int> ints[] = {
+        {"minLeafSize", c.minLeafSize},     {"maxLeafSize", c.maxLeafSize},   {"leafStep", c.leafStep},
+        {"minBranch", c.minBranch},         {"maxBranch", c.maxBranch},       {"branchStep", c.branchStep},
+        {"subdivisionStart", c.subdivisionStart}, {"subdivisionEnd", c.subdivisionEnd},
+        {"subdivisionStep", c.subdivisionStep},   {"renderType", c.renderType},
+        {"ambientSampleCount", c.ambientSampleCount}, {"workGroupSize", c.workGroupSize},
+        {"leafSplitOption", c.leafSplitOption},   {"xRes", c.xRes},             {"yRes", c.yRes}};
+    const std::pair<const char*, bool> bools[] = {
+        {"saveImage", c.saveImage},         {"saveDepthDetailedImage", c.saveDepthDetailedImage},
+        {"bvhAnalysis", c.bvhAnalysis},     {"saveBvhImage", c.saveBvhImage},
+        {"sortEachSplit", c.sortEachSplit}, {"renderAnalysisImage", c.renderAnalysisImage},
+        {"castShadows", c.castShadows},     {"doPerformanceTest", c.doPerformanceTest},
+        {"doNodeMemoryTest", c.doNodeMemoryTest}, {"doLeafMemoryTest", c.doLeafMemoryTest},
+        {"saveRayTimes", c.saveRayTimes},   {"doWorkGroupAnalysis", c.doWorkGroupAnalysis},
+        {"saveDistance", c.saveDistance},   {"wideRender", c.wideRender},
+        {"wideAlternative", c.wideAlternative}, {"renderAllOptions", c.renderAllOptions},
+        {"doCacheAnalysis", c.doCacheAnalysis}};
+    std::string out;
+    for (const auto& kv : ints) out += std::string(kv.first) + "=" + std::to_string(kv.second) + "\n";
+    for (const auto& kv : bools) out += std::string(kv.first) + "=" + (kv.second ? "true" : "false") + "\n";
+    auto list = [&out](const char* k, const std::vector<int>& v) {
+        out += std::string(k) + "=";
+        for (size_t i = 0; i < v.size(); ++i) out += (i ? "," : "") + std::to_string(v[i]);
+        out += "\n";
+    };
+    list("scenarios", c.scenarios);
+    list("cacheSizes", c.cacheSizes);
+    for (const auto& w : c.warnings) out += "warning=" + w + "\n";
+    return out;
+}
+
 void sanity_clamp(RenderConfig& c) {
     c.maxBranch = std::max(c.maxBranch, c.minBranch);
     c.minBranch = std::min(c.maxBranch, c.minBranch);
@@ -628,19 +660,26 @@ std::pair<float, float> CameraFast::renderImages(bool saveImage, FastNodeManager
     }
     const std::string file = perfFile(wideAlternative);
     std::filesystem::create_directories(std::filesystem::path(file).parent_path(), ec);
-    std::ofstream f(file);
-    if (!f.is_open()) throw Error("CameraFast: cannot open " + file);
-    f << "scenario " << name_ << " with branching factor of " << m.branchingFactor << " and leafsize of " << m.leafSize
-      << std::endl;
-    f << "with a gang size of " << m.gangSize << ", a node meory of " << m.nodeMemory << ", and a leaf memory of "
-      << m.leafMemory << std::endl;
-    f << "Raytracer total time: " << totalTime << std::endl;
-    f << "Time for all rays (SUM): " << timeSumEachRay << std::endl;
-    f << "Time for triangle intersections (SUM): " << timeSumTriangle << std::endl;
-    f << "Time all rays(sum) - triangle(sum): " << timeSumEachRay - timeSumTriangle << std::endl;
-    f << "Average BVH depth: " << m.averageBvhDepth << std::endl;
-    f << "Triangle Count: " << m.triangleCount << std::endl;
+    write_perf(file, name_, m.branchingFactor, m.leafSize, m.gangSize, m.nodeMemory, m.leafMemory, totalTime,
+               timeSumEachRay, timeSumTriangle, m.averageBvhDepth, m.triangleCount);
     return {totalTime, timeSumEachRay};
+}
+
+void write_perf(const std::string& path, const std::string& name, unsigned branchingFactor, unsigned leafSize,
+                unsigned gangSize, unsigned nodeMemory, unsigned leafMemory, float totalTime, float raySum,
+                float triangleSum, float averageBvhDepth, uint64_t triangleCount) {
+    std::ofstream f(path);
+    if (!f.is_open()) throw Error("write_perf: cannot open " + path);
+    f << "scenario " << name << " with branching factor of " << branchingFactor << " and leafsize of " << leafSize
+      << std::endl;
+    f << "with a gang size of " << gangSize << ", a node meory of " << nodeMemory << ", and a leaf memory of "
+      << leafMemory << std::endl;
+    f << "Raytracer total time: " << totalTime << std::endl;
+    f << "Time for all rays (SUM): " << raySum << std::endl;
+    f << "Time for triangle intersections (SUM): " << triangleSum << std::endl;
+    f << "Time all rays(sum) - triangle(sum): " << raySum - triangleSum << std::endl;
+    f << "Average BVH depth: " << averageBvhDepth << std::endl;
+    f << "Triangle Count: " << triangleCount << std::endl;
 }
 
 // ---------------------------------------------------------------------------

@@ -13,76 +13,49 @@ Scenes are procedural (the reference loads glTF models that are not part of
 the repository); everything else keeps the reference's semantics: substring
 key matching, `#` comments, the 1 warm-up + 5 timed renders median protocol,
 and the `_Perf.txt` fields.  Traversal runs on the GPU through librt.so.
+
+The file formats and the config.txt parser have one implementation, the C++
+renderer layer (include/rt_renderer.hpp); the functions here call it through
+include/rt_report.h.
 """
+import ctypes
 import os
-import struct
 import time
-import zlib
 
 import numpy as np
 
 from .host import (Bvh, Scene, camera_rays, interior_scene, load_sah_factor, spawn_ambient, spheres_scene)
-from ._lib import RAY_DTYPE
-
-INT_KEYS = ["minLeafSize", "maxLeafSize", "leafStep", "minBranch", "maxBranch", "branchStep",
-            "subdivisionStart", "subdivisionEnd", "subdivisionStep", "renderType",
-            "ambientSampleCount", "workGroupSize", "leafSplitOption", "xRes", "yRes"]
-BOOL_KEYS = ["saveImage", "saveDepthDetailedImage", "bvhAnalysis", "saveBvhImage", "sortEachSplit",
-             "renderAnalysisImage", "castShadows", "doPerformanceTest", "doNodeMemoryTest",
-             "doLeafMemoryTest", "saveRayTimes", "doWorkGroupAnalysis", "saveDistance", "wideRender",
-             "wideAlternative", "renderAllOptions", "doCacheAnalysis"]
+from ._lib import RAY_DTYPE, STEP_DTYPE, RtTraceHist, check, lib
 
 
-def _stoi(s):
-    """std::stoi: skip leading whitespace, read an optional sign and digits."""
-    s = s.lstrip()
-    j = 0
-    if j < len(s) and s[j] in "+-":
-        j += 1
-    while j < len(s) and s[j].isdigit():
-        j += 1
-    if j == 0 or s[:j] in "+-":
-        raise ValueError(f"stoi: no conversion for {s!r}")
-    return int(s[:j])
-
-
-def read_config(path="config.txt", defaults=None):
-    """RayTracer::readConfig: for every non-empty line that does not start with
-    '#', every key that occurs anywhere in the line (substring match) takes the
-    value after the first '='.  Booleans look for "true" / "false" anywhere in
-    the line; a line with neither sets false (with a warning)."""
-    cfg = dict(defaults or {})
-    cfg.setdefault("scenarios", [])
-    cfg.setdefault("cacheSizes", [])
-    warnings = []
-    with open(path) as f:
-        for line in f.read().splitlines():
-            if not line or line[0] == "#":
-                continue
-            for k in INT_KEYS:
-                if k in line:
-                    cfg[k] = _stoi(line[line.find("=") + 1:])
-            for k in BOOL_KEYS:
-                if k in line:
-                    if "true" in line:
-                        cfg[k] = True
-                    elif "false" in line:
-                        cfg[k] = False
-                    else:
-                        warnings.append(f"{k}value written wrong -> default = false")
-                        cfg[k] = False
-            if "scenario" in line:
-                s = line[line.find("=") + 1:]
-                cfg["scenarios"] += [_stoi(t) for t in s.split(",")]
-            if cfg.get("doCacheAnalysis") and "cacheSize" in line:
-                for t in line[line.find("=") + 1:].split(","):
-                    v = _stoi(t)
-                    if v % 8 == 0:
-                        cfg["cacheSizes"].append(v)
-                    else:
-                        warnings.append(f"cachesize must be divideable by 8: {v} is ignored")
-                if not cfg["cacheSizes"]:
-                    cfg["cacheSizes"] = [256]
+def read_config(path="config.txt"):
+    """RayTracer::readConfig over the reference's defaults: for every
+    non-empty line that does not start with '#', every key that occurs
+    anywhere in the line (substring match) takes the value after the first
+    '='; booleans look for "true" / "false" anywhere in the line (neither:
+    false, with a warning).  Returns every setting plus "scenarios",
+    "cacheSizes" and "_warnings" (what the reference prints)."""
+    n = ctypes.c_size_t()
+    cap = 1 << 16
+    while True:
+        buf = ctypes.create_string_buffer(cap)
+        rc = lib().rt_report_read_config(os.fsencode(path), buf, cap, ctypes.byref(n))
+        if rc != 0 and n.value + 1 > cap:
+            cap = n.value + 1
+            continue
+        check(rc)
+        break
+    cfg, warnings = {}, []
+    for line in buf.value.decode().splitlines():
+        k, v = line.split("=", 1)
+        if k == "warning":
+            warnings.append(v)
+        elif k in ("scenarios", "cacheSizes"):
+            cfg[k] = [int(x) for x in v.split(",")] if v else []
+        elif v in ("true", "false"):
+            cfg[k] = v == "true"
+        else:
+            cfg[k] = int(v)
     cfg["_warnings"] = warnings
     return cfg
 
@@ -108,26 +81,18 @@ def tile_order_to_rows(values, width, height, wg):
     wg x wg tiles left to right, tile rows top to bottom, lanes row-major in a
     tile) to a (height, width, ...) row-major image: the reordering before
     cameraFast.h:833-849 saves the image, extended to a partial last tile
-    row (whose tiles hold height % wg rows)."""
-    values = np.asarray(values)
-    if width % wg:
+    row (whose tiles hold height % wg rows).  Values of any dtype and shape
+    (n, ...) move as whole pixels."""
+    values = np.ascontiguousarray(values)
+    if wg <= 0 or width % wg:
         raise ValueError("width must be a multiple of the workgroup size")
     if len(values) != width * height:
         raise ValueError(f"expected {width * height} pixels, got {len(values)}")
-    tiles_x, full_rows = width // wg, height // wg
-    h, w = np.meshgrid(np.arange(height, dtype=np.int64), np.arange(width, dtype=np.int64), indexing="ij")
-    lane = (h % wg) * wg + w % wg
-    k = ((h // wg) * tiles_x + w // wg) * wg * wg + lane
-    part = h >= full_rows * wg
-    if part.any():
-        rows_left = height - full_rows * wg
-        k[part] = (full_rows * tiles_x * wg * wg + (w[part] // wg) * rows_left * wg
-                   + (h[part] % wg) * wg + w[part] % wg)
-    return values[k]
-
-
-def _png_chunk(tag, data):
-    return struct.pack(">I", len(data)) + tag + data + struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF)
+    raw = values.view(np.uint8).reshape(width * height, -1) if values.size else values.reshape(0, 1)
+    out = np.empty_like(values)
+    check(lib().rt_report_tile_order_to_rows(raw.ctypes.data, width, height, wg, raw.shape[1],
+                                             out.view(np.uint8).ctypes.data))
+    return out.reshape((height, width) + values.shape[1:])
 
 
 def write_png(path, rgba):
@@ -137,34 +102,27 @@ def write_png(path, rgba):
     rgba = np.ascontiguousarray(rgba, dtype=np.uint8)
     if rgba.ndim != 3 or rgba.shape[2] != 4:
         raise ValueError("expected a (height, width, 4) image")
-    height, width = rgba.shape[:2]
-    raw = b"".join(b"\x00" + rgba[r].tobytes() for r in range(height))  # filter type 0 per row
-    png = (b"\x89PNG\r\n\x1a\n"
-           + _png_chunk(b"IHDR", struct.pack(">IIBBBBB", width, height, 8, 6, 0, 0, 0))
-           + _png_chunk(b"IDAT", zlib.compress(raw, 6))
-           + _png_chunk(b"IEND", b""))
-    with open(path, "wb") as f:
-        f.write(png)
+    check(lib().rt_report_write_png(os.fsencode(path), rgba.ctypes.data, rgba.shape[1], rgba.shape[0]))
 
 
 def ambient_distance(tris):
     """ambientDistance of RayTracer::renderImage (rayTracer.cpp:607): cbrt of
     the root box volume (Aabb::getVolume, aabb.h:136-140) / 10, in float32,
     the root box being the triangles' bounds."""
-    pts = np.asarray(tris, np.float32).reshape(-1, 3)
-    d = pts.max(0) - pts.min(0)
-    vol = np.float32(np.float32(d[0] * d[1]) * d[2])
-    return float(np.float32(np.cbrt(vol) / np.float32(10)))
+    t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    out = ctypes.c_float()
+    check(lib().rt_report_ambient_distance(t.ctypes.data, len(t), ctypes.byref(out)))
+    return float(out.value)
 
 
 def ambient_image(occluded, hit, samples):
     """Gray value per pixel (cameraFast.h:404-439): a hit pixel is
     (uint8)((1 - occluded / samples) * 255) in float32, a miss is 0.  With
     no samples the reference converts NaN, which gives 0 on x86."""
+    hit = np.ascontiguousarray(hit, np.uint8)
+    occ = np.ascontiguousarray(occluded, np.uint32)
     img = np.zeros(len(hit), np.uint8)
-    if samples:
-        factor = np.float32(1) - occluded.astype(np.float32) / np.float32(samples)
-        img[hit] = (factor[hit] * np.float32(255)).astype(np.uint8)
+    check(lib().rt_report_ambient_gray(hit.ctypes.data, occ.ctypes.data, len(hit), int(samples), img.ctypes.data))
     return img
 
 
@@ -241,57 +199,28 @@ class CameraFast:
 
 def write_perf(path, name, problem, gang, node_memory, leaf_memory, branching, leaf_size,
                total, ray_sum, tri_sum, avg_depth, tri_count):
-    """The _Perf.txt fields of cameraFast.h:147-161."""
-    with open(path, "w") as f:
-        f.write(f"scenario {name} with branching factor of {branching} and leafsize of {leaf_size}\n")
-        f.write(f"with a gang size of {gang}, a node meory of {node_memory}, and a leaf memory of {leaf_memory}\n")
-        # the reference streams floats: %g with 6 significant digits
-        total, ray_sum, tri_sum = np.float32(total), np.float32(ray_sum), np.float32(tri_sum)
-        f.write(f"Raytracer total time: {_g(total)}\n")
-        f.write(f"Time for all rays (SUM): {_g(ray_sum)}\n")
-        f.write(f"Time for triangle intersections (SUM): {_g(tri_sum)}\n")
-        f.write(f"Time all rays(sum) - triangle(sum): {_g(np.float32(ray_sum - tri_sum))}\n")
-        f.write(f"Average BVH depth: {_g(np.float32(avg_depth))}\n")
-        f.write(f"Triangle Count: {tri_count}\n")
+    """The _Perf.txt fields of cameraFast.h:147-161 (`problem` names the
+    file, the text does not hold it)."""
+    check(lib().rt_report_write_perf(os.fsencode(path), name.encode(), branching, leaf_size, gang, node_memory,
+                                     leaf_memory, float(total), float(ray_sum), float(tri_sum), float(avg_depth),
+                                     int(tri_count)))
 
 
-def _info_block(f, label, st, branching):
-    """One counter block of _Info.txt (cameraData.h:247-265), float32 as
-    there: factor = 1 / (float)rays, uint64 counts converted to float."""
-    f32 = np.float32
-    factor = f32(1) / f32(st["rays"])
-
-    def per_ray(v):
-        return _g(f32(f32(int(v)) * factor))
-
-    def ratio(a, b):
-        return _g(f32(f32(int(a)) / f32(int(b))))
-
-    f.write(f"{label} intersections node: {per_ray(st['node_visits'])}\n")
-    f.write(f"{label} aabb intersections: {per_ray(st['aabb_tests'])}\n")
-    f.write(f"{label} aabb success ration: {ratio(st['aabb_hits'], st['aabb_tests'])}\n")
-    waste = f32(f32(int(st["leaf_visits"] + st["node_visits"] - st["rays"])) / f32(int(st["node_visits"]) * branching))
-    f.write(f"{label} waste factor: {_g(f32(1) - waste)}\n")
-    f.write(f"{label} intersections leaf: {per_ray(st['leaf_visits'])}\n")
-    f.write(f"{label} primitive intersections: {per_ray(st['tri_tests'])}\n")
-    f.write(f"{label} primitive success ratio: {ratio(st['tri_hits'], st['tri_tests'])}\n")
-
-
-def _fullness(f, title, avg_key, counts, size, factor):
-    """A fullness section (cameraData.h:270-292): the average is a float sum
-    of count * i over the counts' int total, written with std::to_string."""
-    f32 = np.float32
-    c = [int(v) for v in counts[:size]] + [0] * max(0, size - len(counts))
-    acc = f32(0)
-    for i, v in enumerate(c):
-        acc = f32(acc + f32(v * i))
-    total = sum(c) & 0xFFFFFFFF  # std::accumulate(..., 0) sums into an int
-    total = total - (1 << 32) if total >= 1 << 31 else total
-    acc = f32(acc / f32(total))
-    f.write(f"{title}\n")
-    f.write("%s%f\n" % (avg_key, float(acc)))
-    for i, v in enumerate(c):
-        f.write(f"{i} : {_g(f32(f32(v) * factor))}\n")
+def _hist_struct(st):
+    """rt_trace_hist from the dict of Scene.histograms (summed or not)."""
+    h = RtTraceHist()
+    for f, _ in RtTraceHist._fields_:
+        v = st[f]
+        if hasattr(v, "__len__"):
+            arr = getattr(h, f)
+            vals = [int(x) for x in v]
+            if len(vals) > len(arr):
+                raise ValueError(f"{f}: {len(vals)} bins, the histogram holds {len(arr)}")
+            for i, x in enumerate(vals):
+                arr[i] = x
+        else:
+            setattr(h, f, int(v))
+    return h
 
 
 def write_info(path, name, branching, leaf_size, primary, secondary, bvh_depth=None):
@@ -300,89 +229,16 @@ def write_info(path, name, branching, leaf_size, primary, secondary, bvh_depth=N
     rays and of the secondary (ambient) rays: per-ray averages, success
     ratios, the waste factor, child fullness 0..N and leaf fullness 0..L of
     the primary rays, and visits per depth 0..bvh_depth-1 (bvhDepth, the
-    tree depth + 1), in float32 with the reference's line keys and number
-    formats, so its Analysis/DataManager.py reads it.  Every ray, test and
-    visit count must be nonzero (the reference divides by them)."""
-    f32 = np.float32
+    tree depth + 1; by default up to the deepest primary visit), in float32
+    with the reference's line keys and number formats, so its
+    Analysis/DataManager.py reads it.  Every ray, test and visit count must
+    be nonzero (the reference divides by them)."""
     if bvh_depth is None:
         nz = np.nonzero(np.asarray(primary["node_depth"]) + np.asarray(primary["leaf_depth"]))[0]
         bvh_depth = int(nz[-1]) + 1 if len(nz) else 1
-    with open(path, "w") as f:
-        f.write(f"scenario {name} with branching factor of {branching} and leafsize of {leaf_size}\n")
-        f.write("intersections counts are normalized per Ray\n\n")
-        _info_block(f, "primary", primary, branching)
-        f.write("\n")
-        _info_block(f, "secondary", secondary, branching)
-        f.write("\n")
-        factor = f32(1) / f32(primary["rays"])
-        _fullness(f, "intersections with nodes with x children :", "average child fullness: ",
-                  primary["child_fullness"], branching + 1, factor)
-        f.write("\n")
-        _fullness(f, "intersections with leaf nodes with x primitives :", "average leaf fullness: ",
-                  primary["leaf_fullness"], leaf_size + 1, factor)
-        for label, st, key in (("primary node", primary, "node_depth"), ("primary leaf", primary, "leaf_depth"),
-                               ("secondary node", secondary, "node_depth"),
-                               ("secondary leaf", secondary, "leaf_depth")):
-            fac = f32(1) / f32(st["rays"])
-            f.write("\n")
-            f.write(f"{label} intersections at depth x :\n")
-            for i in range(bvh_depth):
-                v = int(st[key][i]) if i < len(st[key]) else 0
-                f.write(f"{i} : {_g(f32(f32(v) * fac))}\n")
-
-
-def _g(x):
-    """std::ostream << float / double with the default format (%g, 6
-    significant digits)."""
-    return "%g" % float(x)
-
-
-def _whisker_rows(tiles, wgsq):
-    """Per-tile (median, min, max, avg, sd) of the step at which each ray
-    finished (cameraData.h:703-779): the depth list expands the
-    terminations per step; avg and the variance are float32 as there (each
-    square comes from std::pow in double and is added to the float sum, one
-    rounding per ray, in ray order; vectorised over tiles)."""
-    med_id = int(np.float32(wgsq) * np.float32(0.5))
-    depth = np.zeros((len(tiles), wgsq), np.int64)
-    work = np.zeros(len(tiles), np.int64)
-    for i, rec in enumerate(tiles):
-        term = rec["terminations"].astype(np.int64)
-        dl = np.repeat(np.arange(len(term), dtype=np.int64), term)
-        if len(dl) < wgsq:
-            raise ValueError(f"tile {i}: {len(dl)} terminations, expected at least {wgsq}")
-        depth[i] = dl[:wgsq]
-        work[i] = int((np.arange(len(term)) * term).sum())
-    avg = (work.astype(np.float32) / np.float32(wgsq)).astype(np.float32)
-    var = np.zeros(len(tiles), np.float32)
-    for j in range(wgsq):
-        d = (depth[:, j].astype(np.float32) - avg).astype(np.float32).astype(np.float64)
-        var = (var.astype(np.float64) + d * d).astype(np.float32)
-    var = (var / np.float32(wgsq)).astype(np.float32)
-    sd = np.sqrt(var).astype(np.float32)
-    rows = [(int(depth[i, med_id]), int(depth[i, 0]), int(depth[i, wgsq - 1]), avg[i], sd[i])
-            for i in range(len(tiles))]
-    # std::sort by (median, max); the reference leaves ties unordered, here
-    # they keep the tile order
-    return sorted(rows, key=lambda r: (r[0], r[2]))
-
-
-def _step_averages(tiles, j, n_tiles_f, wgsq):
-    """One row half of _WorkGroupData.txt for step j (cameraData.h:846-949):
-    averages over the tiles still working at step j, unique counts over the
-    tiles with a nonzero unique count, terminations over all tiles."""
-    live = [t for t in tiles if len(t) > j]
-    acc = {f: np.float32(0) for f in ("node_work", "leaf_work", "node_unique", "leaf_unique", "terminations")}
-    for t in live:
-        for f in acc:
-            acc[f] = np.float32(acc[f] + np.float32(int(t[f][j])))
-    un = [int(t["node_unique"][j]) for t in live if t["node_unique"][j] != 0]
-    ul = [int(t["leaf_unique"][j]) for t in live if t["leaf_unique"][j] != 0]
-    cnt = np.float32(max(len(live), 1))
-    return (np.float32(acc["node_work"] / cnt), np.float32(acc["node_unique"] / np.float32(max(len(un), 1))),
-            np.float32(acc["leaf_work"] / cnt), np.float32(acc["leaf_unique"] / np.float32(max(len(ul), 1))),
-            np.float32(acc["terminations"] / n_tiles_f),
-            max(un) if un else 0, min(un) if un else 0, max(ul) if ul else 0, min(ul) if ul else 0)
+    hp, hs = _hist_struct(primary), _hist_struct(secondary)
+    check(lib().rt_report_write_info(os.fsencode(path), name.encode(), branching, leaf_size, int(bvh_depth),
+                                     ctypes.byref(hp), ctypes.byref(hs)))
 
 
 def write_workgroup_analysis(path, name, problem, camera_id, width, height, wg, primary, secondary,
@@ -397,69 +253,32 @@ def write_workgroup_analysis(path, name, problem, camera_id, width, height, wg, 
     concatenated, as the reference appends them).  primary_visits /
     secondary_visits: (nodes, leaves) per ray in tile order (secondary summed
     over the samples).  Returns the four paths."""
-    wgsq = wg * wg
-    n_tiles = (width // wg) * (height // wg)
+    n_tiles = (width // wg) * (height // wg) if wg else 0
     if len(primary) != n_tiles or len(secondary) != n_tiles:
         raise ValueError(f"expected {n_tiles} tiles")
-    d = os.path.join(path, f"WorkGroupSize_{wg}_Version_{int(bool(wide_alternative))}")
-    os.makedirs(d, exist_ok=True)
-    stem = os.path.join(d, f"{name}{problem}_c{camera_id}")
-    out = [stem + "_PrimaryWorkGroupWiskerPlot.txt", stem + "_SecondaryWorkGroupWiskerPlot.txt",
-           stem + "_WorkGroupData.txt", stem + "_WorkGroupUniqueWork.txt"]
-    for fn, tiles in ((out[0], primary), (out[1], secondary)):
-        with open(fn, "w") as f:
-            f.write("median, min, max, lowerStdDeviation, upperStdDeviation\n")
-            for med, mn, mx, avg, sd in _whisker_rows(tiles, wgsq):
-                f.write(f"{med}, {mn}, {mx}, {_g(avg)}, {_g(sd)}\n")
-    # the reference's size loop tests the primary size twice, so the steps
-    # are those of the longest primary tile
-    max_size = max(len(t) for t in primary)
-    n_tiles_f = np.float32(np.float32(width // wg) * np.float32(height // wg))
-    with open(out[2], "w") as f:
-        f.write("stepId, avgPrimaryNodeWork, avgPrimaryNodeUnique, avgPrimaryLeafWork, avgPrimaryLeafUnique, "
-                "avgPrimaryRayTermination, primaryNodeWorkMax, primaryNodeWorkMin, primaryLeafWorkMax, "
-                "primaryLeafWorkMin, avgSecondaryNodeWork, avgSecondaryNodeUnique, avgSecondaryLeafWork, "
-                "avgSecondaryLeafUnique, avgSecondaryRayTermination, secondaryNodeWorkMax, secondaryNodeWorkMin, "
-                "secondaryLeafWorkMax, secondaryLeafWorkMin\n")
-        cum_p = cum_s = np.float32(0)
-        for j in range(max_size):
-            p = list(_step_averages(primary, j, n_tiles_f, wgsq))
-            s = list(_step_averages(secondary, j, n_tiles_f, wgsq))
-            if j:
-                p[4] = np.float32(p[4] + cum_p)
-                s[4] = np.float32(s[4] + cum_s)
-            cum_p, cum_s = p[4], s[4]
-            cells = [str(j)] + [_g(v) for v in p[:5]] + [str(v) for v in p[5:]] + \
-                [_g(v) for v in s[:5]] + [str(v) for v in s[5:]]
-            f.write(", ".join(cells) + "\n")
-    pn, pl = (np.asarray(a, np.int64) for a in primary_visits)
-    sn, sl = (np.asarray(a, np.int64) for a in secondary_visits)
-    with open(out[3], "w") as f:
-        f.write("loadedPrimaryNodes, loadedPrimaryLeafs, loadedPrimaryNodesMax, loadedPrimaryLeafsMax, "
-                "loadedPrimaryNodesMin, loadedPrimaryLeafsMin, loadedSecondaryNodes, loadedSecondaryLeafs, "
-                "loadedSecondaryNodesMax, loadedSecondaryLeafsMax, loadedSecondaryNodesMin, "
-                "loadedSecondaryLeafsMin, loadedWidePrimaryNodes, loadedWidePrimaryLeafs, "
-                "loadedWideSecondaryNodes, loadedWideSecondaryLeafs\n")
 
-        def mm(v):
-            # max from 0, min from 1000 (reset to 0 when it stays 1000)
-            mx, mn = max(0, int(v.max())), min(1000, int(v.min()))
-            return mx, (0 if mn == 1000 else mn)
+    def flat(tiles):
+        recs = [np.ascontiguousarray(t, STEP_DTYPE) for t in tiles]
+        lens = np.array([len(t) for t in recs], np.uint32)
+        cat = np.concatenate(recs) if recs else np.zeros(0, STEP_DTYPE)
+        return np.ascontiguousarray(cat), lens
 
-        for i in range(n_tiles):
-            sl_ = slice(i * wgsq, (i + 1) * wgsq)
-            cells = []
-            for nodes, leaves in ((pn[sl_], pl[sl_]), (sn[sl_], sl[sl_])):
-                nmx, nmn = mm(nodes)
-                lmx, lmn = mm(leaves)
-                cells += [_g(np.float32(nodes.sum())), _g(np.float32(leaves.sum())), str(nmx), str(lmx),
-                          str(nmn), str(lmn)]
-            cells += [_g(np.float32(int(primary[i]["node_unique"].sum()))),
-                      _g(np.float32(int(primary[i]["leaf_unique"].sum()))),
-                      _g(np.float32(int(secondary[i]["node_unique"].sum()))),
-                      _g(np.float32(int(secondary[i]["leaf_unique"].sum())))]
-            f.write(", ".join(cells) + "\n")
-    return out
+    p, pl = flat(primary)
+    s, sl = flat(secondary)
+    nrays = n_tiles * wg * wg
+    vis = []
+    for a in (*primary_visits, *secondary_visits):
+        v = np.ascontiguousarray(a, np.int64)
+        if len(v) < nrays or (len(v) and (v.min() < 0 or v.max() > 0xFFFFFFFF)):
+            raise ValueError(f"expected {nrays} visit counts per ray in uint32 range")
+        vis.append(np.ascontiguousarray(v[:nrays], np.uint32))
+    cap = 4 * (len(os.fsencode(path)) + len(name.encode()) + len(problem.encode()) + 256)
+    buf = ctypes.create_string_buffer(cap)
+    check(lib().rt_report_write_workgroup_analysis(
+        os.fsencode(path), name.encode(), problem.encode(), int(camera_id), width, height, wg, p.ctypes.data,
+        pl.ctypes.data, s.ctypes.data, sl.ctypes.data, vis[0].ctypes.data, vis[1].ctypes.data, vis[2].ctypes.data,
+        vis[3].ctypes.data, int(bool(wide_alternative)), buf, cap))
+    return buf.value.decode().splitlines()
 
 
 def workgroup_analysis(scene, bvh, pos, target, width, height, wg, ambient_samples, ambient_distance, path,

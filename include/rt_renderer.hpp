@@ -53,6 +53,11 @@ struct RenderConfig {
 // opened (cfg keeps its values).
 bool read_config(const std::string& path, RenderConfig& cfg);
 
+// The settings as "key=value" lines: the integers and booleans (true /
+// false) by their config.txt key, scenarios and cacheSizes comma-separated,
+// then one "warning=<text>" line per warning.
+std::string config_text(const RenderConfig& cfg);
+
 // The settings sanity checks of RayTracer::run (rayTracer.cpp:104-122).
 void sanity_clamp(RenderConfig& cfg);
 
@@ -138,6 +143,13 @@ std::vector<std::string> write_workgroup_analysis(const std::string& path, const
                                                   const std::vector<std::vector<rt_step_stat>>& secondary,
                                                   const TileVisits& primaryVisits, const TileVisits& secondaryVisits,
                                                   bool wideAlternative = true);
+
+// _Perf.txt of CameraFast::renderImages (cameraFast.h:147-161): the scenario
+// and tree settings, the median times in seconds, the average BVH depth and
+// the triangle count (floats streamed with the default 6 significant digits).
+void write_perf(const std::string& path, const std::string& name, unsigned branchingFactor, unsigned leafSize,
+                unsigned gangSize, unsigned nodeMemory, unsigned leafMemory, float totalTime, float raySum,
+                float triangleSum, float averageBvhDepth, uint64_t triangleCount);
 
 // Gray value of an ambient-occlusion pixel (cameraFast.h:404-439): a hit is
 // (uint8)((1 - occluded / samples) * 255) in float32, a miss 0; with no

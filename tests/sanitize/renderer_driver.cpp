@@ -1,11 +1,13 @@
 // Sanitizer driver for the host-only parts of the C++ renderer layer
 // (config parsing, sweep, SAH factors, _Info.txt and workgroup files, tile
-// order, PNG, scenario geometry); argv[1] is the output directory.
+// order, PNG, scenario geometry) and their C entry points (rt_report.h);
+// argv[1] is the output directory.
 #include <cstdio>
 #include <fstream>
 #include <random>
 
 #include "rt_renderer.hpp"
+#include "rt_report.h"
 
 int main(int argc, char** argv) {
     if (argc < 2) return 2;
@@ -83,6 +85,61 @@ int main(int argc, char** argv) {
     rt::Vec3 cp, ct;
     auto tris = rt::scenario_triangles(0, name, cp, ct);
     std::printf("scenario 0: %zu floats, ambient distance %g\n", tris.size(), rt::ambient_distance(tris));
+    // the C entry points over the same code: a config text buffer too small
+    // first, flattened step records, null arguments refused
+    {
+        size_t len = 0;
+        char small[8];
+        if (rt_report_read_config((dir + "/config.txt").c_str(), small, sizeof small, &len) != RT_ERR_ARG) return 3;
+        std::vector<char> text(len + 1);
+        if (rt_report_read_config((dir + "/config.txt").c_str(), text.data(), text.size(), &len) != RT_OK) return 4;
+        if (rt_report_read_config((dir + "/missing.txt").c_str(), text.data(), text.size(), &len) != RT_ERR_ARG)
+            return 5;
+        if (rt_report_write_perf((dir + "/x_Perf.txt").c_str(), "scene", 8, 8, 4, 8, 8, 1.5f, 1.0f, 0.25f, 9.5f,
+                                 100002) != RT_OK)
+            return 6;
+        rt_trace_hist a{};
+        a.rays = 10; a.node_visits = 300; a.leaf_visits = 70; a.aabb_tests = 2000; a.aabb_hits = 400;
+        a.tri_tests = 200; a.tri_hits = 9;
+        a.node_depth[3] = 5; a.child_fullness[4] = 2; a.leaf_fullness[2] = 3;
+        if (rt_report_write_info((dir + "/c_Info.txt").c_str(), "scene", 8, 8, 12, &a, &a) != RT_OK) return 7;
+        if (rt_report_write_info((dir + "/c_Info.txt").c_str(), "scene", 8, 8, 65, &a, &a) != RT_ERR_ARG) return 8;
+        const unsigned wg = 4, W = 8, H = 4, tiles = 2, wgsq = wg * wg;
+        std::vector<rt_step_stat> recs;
+        std::vector<uint32_t> lens;
+        for (unsigned t = 0; t < tiles; ++t) {
+            rt_step_stat r0{}, r1{};
+            r0.node_work = wgsq;
+            r0.node_unique = 1;
+            r1.leaf_work = wgsq;
+            r1.leaf_unique = 2;
+            r1.terminations = wgsq;
+            recs.push_back(r0);
+            recs.push_back(r1);
+            lens.push_back(2);
+        }
+        std::vector<uint32_t> visits(tiles * wgsq, 1);
+        char paths[4096];
+        if (rt_report_write_workgroup_analysis(dir.c_str(), "scene", "prob", 1, W, H, wg, recs.data(), lens.data(),
+                                               recs.data(), lens.data(), visits.data(), visits.data(), visits.data(),
+                                               visits.data(), 1, paths, sizeof paths) != RT_OK)
+            return 9;
+        if (rt_report_write_workgroup_analysis(dir.c_str(), "scene", "prob", 1, W, H, 3, recs.data(), lens.data(),
+                                               recs.data(), lens.data(), visits.data(), visits.data(), visits.data(),
+                                               visits.data(), 1, paths, sizeof paths) != RT_ERR_ARG)
+            return 10;
+        std::vector<uint8_t> px(W * H * 4), rows(W * H * 4);
+        for (auto& x : px) x = (uint8_t)g();
+        if (rt_report_tile_order_to_rows(px.data(), W, H, wg, 4, rows.data()) != RT_OK) return 11;
+        if (rt_report_write_png((dir + "/c.png").c_str(), rows.data(), W, H) != RT_OK) return 12;
+        std::vector<uint8_t> hit(W * H, 1), gray(W * H);
+        std::vector<uint32_t> occ(W * H, 1);
+        if (rt_report_ambient_gray(hit.data(), occ.data(), W * H, 3, gray.data()) != RT_OK) return 13;
+        float dist = 0;
+        if (rt_report_ambient_distance(tris.data(), (uint32_t)(tris.size() / 9), &dist) != RT_OK) return 14;
+        if (rt_report_write_png(nullptr, rows.data(), W, H) != RT_ERR_ARG) return 15;
+        std::printf("C entry points: config %zu bytes, ambient distance %g\n", len, dist);
+    }
     std::printf("renderer host parts clean\n");
     return 0;
 }

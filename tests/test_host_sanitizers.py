@@ -41,7 +41,8 @@ def test_renderer_host_parts_clean(tmp_path, rt):
     lib = os.path.join(ROOT, "bvhraytracer_amd")
     _run(["g++", *SAN, "-std=c++17", "-pthread", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
           os.path.join(ROOT, "tests", "sanitize", "renderer_driver.cpp"),
-          *[os.path.join(CSRC, f) for f in ("renderer.cpp", "scene.cpp", "builder.cpp", "host_api.cpp")],
+          *[os.path.join(CSRC, f) for f in ("renderer.cpp", "report_api.cpp", "scene.cpp", "builder.cpp",
+                                            "host_api.cpp")],
           "-L", lib, "-l:librt.so", f"-Wl,-rpath,{lib}", "-lz", "-o", str(exe)])
     out = _run([str(exe), str(tmp_path)], env=ENV, timeout=600)
     assert "renderer host parts clean" in out

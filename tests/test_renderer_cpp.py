@@ -1,5 +1,6 @@
-"""The C++ renderer layer (include/rt_renderer.hpp, the bvhrt program) reads
-config.txt exactly as the Python driver does; both follow
+"""config.txt parsing of the C++ renderer layer (include/rt_renderer.hpp),
+through the bvhrt program and through the C entry point the Python driver
+binds (rt_report_read_config), against settings worked out by hand from
 RayTracer::readConfig (rayTracer.cpp:756-877, configUtil.h:6-58)."""
 import os
 import subprocess
@@ -19,9 +20,13 @@ DEFAULTS = dict(minLeafSize=4, maxLeafSize=4, leafStep=4, minBranch=4, maxBranch
                 doLeafMemoryTest=False, saveRayTimes=False, doWorkGroupAnalysis=False, saveDistance=False,
                 wideRender=False, wideAlternative=True, renderAllOptions=False, doCacheAnalysis=False)
 
+# (config text, settings that differ from DEFAULTS, warnings)
 CONFIGS = [
-    # the reference's key semantics: substring keys, comments, bad booleans
-    """#Config file for raytracer.
+    # the reference's key semantics: substring keys ("doNodeMemoryTests" sets
+    # doNodeMemoryTest), '#' comments, a boolean that is neither true nor
+    # false (false, with a warning), cache sizes ignored without
+    # doCacheAnalysis, scenarios as a comma list
+    ("""#Config file for raytracer.
 xRes = 1920
 yRes = 1088
 minLeafSize = 4
@@ -37,23 +42,28 @@ doNodeMemoryTests = true
 cacheSize = 512, 256, 12
 wideRender = maybe
 scenario = 4,8, 9
-""",
-    # cache sizes only with doCacheAnalysis; a size not divisible by 8 is dropped
-    """doCacheAnalysis = true
+""", dict(maxLeafSize=16, maxBranch=16, doPerformanceTest=False, doNodeMemoryTest=True, scenarios=[4, 8, 9],
+          cacheSizes=[]), ["wideRendervalue written wrong -> default = false"]),
+    # cache sizes only with doCacheAnalysis; a size not divisible by 8 is
+    # dropped with a warning; std::stoi stops at the first non-digit
+    ("""doCacheAnalysis = true
 cacheSize = 512, 12, 256
 scenario = 0
 xRes=640
 yRes = 480 # trailing text after the number
 saveImage = true
-""",
-    # two keys in one line both take the value after the first '='
-    """minLeafSize maxLeafSize = 8
+""", dict(doCacheAnalysis=True, cacheSizes=[512, 256], scenarios=[0], xRes=640, yRes=480, saveImage=True),
+     ["cachesize must be divideable by 8: 12 is ignored"]),
+    # two keys in one line both take the value after the first '='; with
+    # every cache size dropped the list falls back to 256
+    ("""minLeafSize maxLeafSize = 8
 minBranch=12
 maxBranch = 4
 ambientSampleCount = 3
 doCacheAnalysis = true
 cacheSize = 20
-""",
+""", dict(minLeafSize=8, maxLeafSize=8, minBranch=12, maxBranch=4, ambientSampleCount=3, doCacheAnalysis=True,
+          cacheSizes=[256], scenarios=[]), ["cachesize must be divideable by 8: 20 is ignored"]),
 ]
 
 
@@ -75,16 +85,17 @@ def cpp_config(path):
 
 
 @pytest.mark.parametrize("i", range(len(CONFIGS)))
-def test_cpp_reads_config_like_python(tmp_path, i):
+def test_config_parsing_hand_derived(tmp_path, i):
+    text, changed, warnings = CONFIGS[i]
     p = tmp_path / "config.txt"
-    p.write_text(CONFIGS[i])
-    got, warnings = cpp_config(str(p))
-    want = raytracer.read_config(str(p))
-    for k, default in DEFAULTS.items():
-        assert got[k] == want.get(k, default), k
-    assert got["scenarios"] == want["scenarios"]
-    assert got["cacheSizes"] == want["cacheSizes"]
-    assert warnings == want["_warnings"]
+    p.write_text(text)
+    want = dict(DEFAULTS, **changed)
+    cli, cli_warnings = cpp_config(str(p))
+    bound = raytracer.read_config(str(p))
+    for k, v in want.items():
+        assert cli[k] == v, k
+        assert bound[k] == v, k
+    assert cli_warnings == warnings and bound["_warnings"] == warnings
 
 
 def test_cpp_config_errors(tmp_path):
