@@ -257,7 +257,6 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<floa
     for (uint64_t l = 0; l < nLeafBlocks; ++l) {
         float* dst = &tris[l * LS];
         const float* src = d->tri_points + l * block;
-        std::memcpy(dst, src, block * sizeof(float));
         const uint32_t k = blockK[l];
         // slots up to the last triangle that is not all zeros, in 2-slot
         // chunks: a zero triangle (e1 = e2 = 0) has divisor 0 for every
@@ -276,8 +275,9 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<floa
         for (uint32_t c = 0; c < 3; ++c)
             for (uint32_t s = 0; s < k; ++s) {
                 const float p0 = src[c * k + s];
-                dst[(3 + c) * k + s] = src[(3 + c) * k + s] - p0;
-                dst[(6 + c) * k + s] = src[(6 + c) * k + s] - p0;
+                dst[leaf_offset(s, c)] = p0;
+                dst[leaf_offset(s, 3 + c)] = src[(3 + c) * k + s] - p0;
+                dst[leaf_offset(s, 6 + c)] = src[(6 + c) * k + s] - p0;
             }
     }
     triIds.assign(d->tri_ids, d->tri_ids + nLeafBlocks * Lm);

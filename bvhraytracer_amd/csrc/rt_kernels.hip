@@ -128,29 +128,24 @@ __device__ __forceinline__ float comp(const float4& v, int j) {
     return j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
 }
 
-// Triangles of a leaf are read LEAF_CHUNK slots at a time: 9 plane loads of
-// LEAF_CHUNK consecutive floats (16- or 8-byte aligned: k and s0 are
-// multiples of 4 / 2).  Two slots per chunk keep 18 VGPRs of vertex data live
-// instead of 36.
-#ifndef LEAF_CHUNK
-#define LEAF_CHUNK 2
-#endif
+// Triangles of a leaf are read a pair of slots at a time: the pair's
+// 80-byte chunk (the leaf block layout) as four 16-byte loads and one
+// 8-byte load, 18 VGPRs of vertex data.
+constexpr uint32_t LEAF_CHUNK = 2;
 struct LeafChunk {
-#if LEAF_CHUNK == 4
-    float4 p[9];
-    __device__ __forceinline__ void load(const float* T, uint32_t k, uint32_t s0) {
+    float4 q[4];
+    float2 r;
+    __device__ __forceinline__ void load(const float* T, uint32_t s0) {
+        const float* c = T + (s0 >> 1) * 20;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) p[i] = ld4(T + i * k + s0);
+        for (int i = 0; i < 4; ++i) q[i] = ld4(c + 4 * i);
+        r = *reinterpret_cast<const float2*>(c + 16);
     }
-    __device__ __forceinline__ float v(int i, int j) const { return comp(p[i], j); }
-#else
-    float2 p[9];
-    __device__ __forceinline__ void load(const float* T, uint32_t k, uint32_t s0) {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) p[i] = *reinterpret_cast<const float2*>(T + i * k + s0);
+    // value i (0..8) of slot s0 + j
+    __device__ __forceinline__ float v(int i, int j) const {
+        const int w = 2 * i + j;
+        return w < 16 ? comp(q[w >> 2], w & 3) : (w == 16 ? r.x : r.y);
     }
-    __device__ __forceinline__ float v(int i, int j) const { return j == 0 ? p[i].x : p[i].y; }
-#endif
 };
 
 // maxAbsDimension (glmUtil.h:18-23)
@@ -310,7 +305,7 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
             bool any = false;
             for (uint32_t s0 = 0; s0 == 0 || s0 < kt; s0 += LEAF_CHUNK) {
                 LeafChunk a;
-                a.load(T, k, s0);
+                a.load(T, s0);
 #pragma unroll
                 for (int j = 0; j < LEAF_CHUNK; ++j) {
                     float tt, b1, b2;
@@ -343,7 +338,7 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
             const uint32_t inst = TRACK ? 0xFFFFu / ((1u << sc.G) - 1u) : 0u;
             for (uint32_t s0 = 0; s0 == 0 || s0 < kt; s0 += LEAF_CHUNK) {
                 LeafChunk a;
-                a.load(T, k, s0);
+                a.load(T, s0);
 #pragma unroll
                 for (int j = 0; j < LEAF_CHUNK; ++j) {
                     float tt, b1, b2;
@@ -596,11 +591,12 @@ __device__ __forceinline__ void trav_finish(const SceneCtx& sc, const Trav& t, r
             const uint32_t ord = t.hitEntry & kEntryIndexMask;
             float b1 = t.b1, b2 = t.b2;
             if constexpr (!TRACKED) {
-                const uint32_t k = ((t.hitEntry >> kEntryKShift) & 3u) * 4u + 4u;
-                const float* T = sc.tris + (size_t)ord * leaf_stride(sc.Lm) + t.hitSlot;
+                const float* T = sc.tris + (size_t)ord * leaf_stride(sc.Lm);
+                const uint32_t hs = (uint32_t)t.hitSlot;
                 float tt;
-                (void)mt(T[0], T[k], T[2 * k], T[3 * k], T[4 * k], T[5 * k], T[6 * k], T[7 * k], T[8 * k], t.ray,
-                         tt, b1, b2);
+                (void)mt(T[leaf_offset(hs, 0)], T[leaf_offset(hs, 1)], T[leaf_offset(hs, 2)], T[leaf_offset(hs, 3)],
+                         T[leaf_offset(hs, 4)], T[leaf_offset(hs, 5)], T[leaf_offset(hs, 6)], T[leaf_offset(hs, 7)],
+                         T[leaf_offset(hs, 8)], t.ray, tt, b1, b2);
             }
             h.tri = sc.triIds[(size_t)ord * sc.Lm + t.hitSlot];
             h.leaf = ord * 9u * sc.Lm;
@@ -1179,14 +1175,15 @@ __global__ void __launch_bounds__(256) k_coop(KParams p) {
             err = true;
         } else if (e & kEntryLeaf) {
             const uint32_t k = ((e >> kEntryKShift) & 3u) * 4u + 4u;
-            const float* T = sc.tris + (size_t)(e & kEntryIndexMask) * LSTRIDE + gl;
+            const float* T = sc.tris + (size_t)(e & kEntryIndexMask) * LSTRIDE;
             ++cL;
             cLS += k;
             bool hit = false;
             float t = 0.0f, b1, b2;
             if (gl < k)
-                hit = mt(T[0], T[k], T[2 * k], T[3 * k], T[4 * k], T[5 * k], T[6 * k], T[7 * k], T[8 * k], ray, t, b1,
-                         b2);
+                hit = mt(T[leaf_offset(gl, 0)], T[leaf_offset(gl, 1)], T[leaf_offset(gl, 2)], T[leaf_offset(gl, 3)],
+                         T[leaf_offset(gl, 4)], T[leaf_offset(gl, 5)], T[leaf_offset(gl, 6)], T[leaf_offset(gl, 7)],
+                         T[leaf_offset(gl, 8)], ray, t, b1, b2);
             if (MODE == kAny) {
                 if (group_ballot(hit)) {
                     occl = true;

@@ -21,11 +21,13 @@
 // innerBase/leafBase and a popcount of leafMask, without a table lookup.
 //
 // Leaf block (one per leaf, in FastNode order == trianglePoints order):
-//   LSTRIDE(Lm) floats (9*Lm rounded up to 16 floats = 64 bytes) holding the
-//   reference's 9*Lm-float block with its planes at stride k (the leaf's
-//   loop count): p0.x p0.y p0.z as stored, then e1 = p1 - p0 and
-//   e2 = p2 - p0 (x, y, z planes each), the FP32 differences the
-//   intersection kernels form first.
+//   LSTRIDE(Lm) = 20 * ceil(Lm / 2) floats: the leaf's slots in pairs, one
+//   80-byte chunk per pair (slots 2c and 2c+1).  A chunk holds, for each of
+//   the nine values p0.x p0.y p0.z e1.x e1.y e1.z e2.x e2.y e2.z, the two
+//   slots' floats side by side (18 floats), then 2 floats of padding, so a
+//   pair of triangles is four 16-byte loads and one 8-byte load.  e1 = p1 -
+//   p0 and e2 = p2 - p0 are the FP32 differences the intersection kernels
+//   form first.  Slots >= k (the leaf's loop count) are zero.
 //
 // Stack entry (32 bits):
 //   bit 31        leaf
@@ -47,6 +49,8 @@ __host__ __device__ constexpr uint32_t header_words(uint32_t M) { return 5 + 3 *
 __host__ __device__ constexpr uint32_t record_words(uint32_t M) {
     return ((6 * M + header_words(M)) + 15) / 16 * 16;
 }
-__host__ __device__ constexpr uint32_t leaf_stride(uint32_t Lm) { return (9 * Lm + 15) / 16 * 16; }
+__host__ __device__ constexpr uint32_t leaf_stride(uint32_t Lm) { return (Lm + 1) / 2 * 20; }
+// Float offset of value v (0..8: p0 xyz, e1 xyz, e2 xyz) of slot s in a leaf block.
+__host__ __device__ constexpr uint32_t leaf_offset(uint32_t s, uint32_t v) { return (s >> 1) * 20 + 2 * v + (s & 1); }
 
 }  // namespace rt

@@ -56,14 +56,16 @@ __global__ void __launch_bounds__(kShadeBlock) k_surface(const float* __restrict
     const float qnan = __builtin_nanf("");
     const uint32_t ord = h.leaf / (9u * Lm);
     const uint32_t lc = ord < nLeaves ? leafLc[ord] : 0u;
-    if (h.slot < 0 || lc == 0 || (uint32_t)h.slot >= Lm || h.leaf % (9u * Lm) != 0) {
+    if (h.slot < 0 || lc == 0 || (uint32_t)h.slot >= lc || h.leaf % (9u * Lm) != 0) {
         nrm[0] = nrm[1] = nrm[2] = pos[0] = pos[1] = pos[2] = qnan;
     } else {
-        // leaf block: p0 planes, then e1 = p1 - p0 and e2 = p2 - p0, stride lc
-        const float* T = tris + (size_t)ord * LS + h.slot;
-        const float p0[3] = {T[0], T[lc], T[2 * lc]};
-        const float e1[3] = {T[3 * lc], T[4 * lc], T[5 * lc]};
-        const float e2[3] = {T[6 * lc], T[7 * lc], T[8 * lc]};
+        // leaf block: p0, then e1 = p1 - p0 and e2 = p2 - p0 of the slot
+        // (slots < lc are the leaf's triangles)
+        const float* T = tris + (size_t)ord * LS;
+        const uint32_t s = (uint32_t)h.slot;
+        const float p0[3] = {T[leaf_offset(s, 0)], T[leaf_offset(s, 1)], T[leaf_offset(s, 2)]};
+        const float e1[3] = {T[leaf_offset(s, 3)], T[leaf_offset(s, 4)], T[leaf_offset(s, 5)]};
+        const float e2[3] = {T[leaf_offset(s, 6)], T[leaf_offset(s, 7)], T[leaf_offset(s, 8)]};
         surface(p0, e1, e2, rays[i], nrm, pos);
     }
     for (int a = 0; a < 3; ++a) {
