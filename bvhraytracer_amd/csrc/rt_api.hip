@@ -235,14 +235,37 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<floa
             }
         }
         uint32_t* h = rec + 6 * M;
-        h[0] = innerBase == 0xFFFFFFFFu ? 0 : innerBase;
-        h[1] = leafBase == 0xFFFFFFFFu ? 0 : leafBase;
-        h[2] = leafMask | (c << 16) | (k << 24);
-        h[3] = kcodes;
-        h[4] = cb;
+        const uint32_t ib = innerBase == 0xFFFFFFFFu ? 0 : innerBase;
+        const uint32_t lb = leafBase == 0xFFFFFFFFu ? 0 : leafBase;
+        uint64_t rows[3] = {0, 0, 0};
         for (uint32_t a = 0; a < 3; ++a)
-            for (uint32_t q = 0; q < M; ++q)
-                h[5 + a * OW + q / 8] |= ((uint32_t)ord[a * M + q] & 15u) << (4 * (q % 8));
+            for (uint32_t q = 0; q < M; ++q) rows[a] |= ((uint64_t)ord[a * M + q] & 15u) << (4 * q);
+        if (M <= 8 && (ib > kEntryIndexMask || lb > kEntryIndexMask))
+            return set_error(RT_ERR_LAYOUT, "rt_upload_bvh: more than 2^28 inner nodes or leaves");
+        if (M == 4) {
+            h[0] = ib | leafMask << 28;
+            h[1] = lb | (c - 1) << 28;
+            const uint64_t packed = kcodes | rows[0] << 8 | rows[1] << 24 | rows[2] << 40;
+            h[2] = (uint32_t)packed;
+            h[3] = (uint32_t)(packed >> 32);
+            h[4] = cb;
+        } else if (M == 8) {
+            for (uint32_t a = 0; a < 3; ++a) {
+                h[4 * a] = ib;
+                h[4 * a + 1] = lb;
+                h[4 * a + 2] = leafMask | c << 8 | kcodes << 16;
+                h[4 * a + 3] = (uint32_t)rows[a];
+            }
+            h[12] = cb;
+        } else {
+            h[0] = ib;
+            h[1] = lb;
+            h[2] = leafMask | (c << 16);
+            h[3] = kcodes;
+            h[4] = cb;
+            for (uint32_t a = 0; a < 3; ++a)
+                for (uint32_t w = 0; w < OW; ++w) h[5 + a * OW + w] = (uint32_t)(rows[a] >> (32 * w));
+        }
     }
     // leaf blocks: p0 planes as stored, then e1 = p1 - p0 and e2 = p2 - p0
     // (the edge vectors triIntersect/triAnyHit form first, in the same FP32

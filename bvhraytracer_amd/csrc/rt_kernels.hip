@@ -156,6 +156,47 @@ __device__ __forceinline__ int max_abs_dim(float x, float y, float z) {
     return (x > y) ? ((x > z) ? 0 : 2) : ((y > z) ? 1 : 2);
 }
 
+// Header fields of an inner node visit (the layout's header block) for a ray
+// whose major axis is `axis`: one 16-byte load when M <= 8.
+struct NodeHeader {
+    uint32_t innerBase, leafBase, leafMask, c, kcodes;
+    uint64_t row;  // the axis' order row, 4-bit slots
+};
+
+__device__ __forceinline__ NodeHeader load_header(const uint32_t* rec, uint32_t M, int axis) {
+    NodeHeader h;
+    const uint32_t* hw = rec + 6 * M;
+    if (M == 4) {
+        const uint4 v = ld4u(hw);
+        h.innerBase = v.x & kEntryIndexMask;
+        h.leafMask = v.x >> 28;
+        h.leafBase = v.y & kEntryIndexMask;
+        h.c = (v.y >> 28) + 1u;
+        h.kcodes = v.z & 0xFFu;
+        h.row = ((((uint64_t)v.w << 32) | v.z) >> (8 + 16 * axis)) & 0xFFFFu;
+    } else if (M == 8) {
+        const uint4 v = ld4u(hw + 4 * axis);
+        h.innerBase = v.x;
+        h.leafBase = v.y;
+        h.leafMask = v.z & 0xFFu;
+        h.c = (v.z >> 8) & 15u;
+        h.kcodes = v.z >> 16;
+        h.row = v.w;
+    } else {
+        // M = 12, 16: two order words per axis after childBase (w4)
+        const uint4 h0 = ld4u(hw), h1 = ld4u(hw + 4), h2 = ld4u(hw + 8);
+        h.innerBase = h0.x;
+        h.leafBase = h0.y;
+        h.leafMask = h0.z & 0xFFFFu;
+        h.c = (h0.z >> 16) & 31u;
+        h.kcodes = h0.w;
+        const uint32_t lo = axis == 0 ? h1.y : (axis == 1 ? h1.w : h2.y);
+        const uint32_t hi = axis == 0 ? h1.z : (axis == 1 ? h2.x : h2.z);
+        h.row = ((uint64_t)hi << 32) | lo;
+    }
+    return h;
+}
+
 // ---------------------------------------------------------------------------
 // Traversal state of one ray and the step function shared by the per-ray
 // kernel (state in registers) and the ray-stream kernel (state in LDS).
@@ -167,7 +208,7 @@ struct SceneCtx {
     const uint32_t* nodeEntry;
     const uint8_t* leafPrims;  // real triangles per leaf block (visit statistics)
     const uint8_t* leafTest;   // slots to test per leaf block for finite directions
-    uint32_t rootEntry, G, Lm, nodeCount, M, recw, ow;
+    uint32_t rootEntry, G, Lm, nodeCount, M, recw;
     bool boxesBounded;  // every tested box coordinate is within +-2^100
 };
 
@@ -262,7 +303,6 @@ template <int MT, int MODE, int S, bool HIST = false, bool TRACK = false>
 __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk, Trav& t) {
     const uint32_t M = MT ? (uint32_t)MT : sc.M;
     const uint32_t recw = MT ? record_words(MT) : sc.recw;
-    const uint32_t ow = MT ? order_words(MT) : sc.ow;
     const uint32_t bd = sk.stride;
     uint32_t e;
     float edist = 0.0f;
@@ -381,10 +421,7 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
     const uint32_t* rec = sc.nodes + (size_t)idx * recw;
     ++t.cN;
     t.cNS += k;
-    const uint4 h0 = ld4u(rec + 6 * M);
-    const uint4 h1 = ld4u(rec + 6 * M + 4);
-    uint4 h2 = make_uint4(0, 0, 0, 0);
-    if (ow == 2) h2 = ld4u(rec + 6 * M + 8);
+    const NodeHeader hd = load_header(rec, M, t.axis);
     uint32_t H = 0;
     const float* bp = reinterpret_cast<const float*>(rec);
     auto boxes = [&](auto fastTag) {
@@ -410,24 +447,17 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
     else boxes(std::false_type{});
     t.live |= H;
     if constexpr (HIST) {
-        const uint32_t cc = (h0.z >> 16) & 31u;
+        const uint32_t cc = hd.c;
         atomicAdd(&sk.hist[kHwNodeDepth + min(edep, kHistDepthBins - 1)], 1u);
         atomicAdd(&sk.hist[kHwChildFull + min(cc, kHistFullBins - 1)], 1u);
         atomicAdd(&sk.hist[kHwAabbTests], cc);
         atomicAdd(&sk.hist[kHwAabbHits], (uint32_t)__popc(H & ((1u << cc) - 1u)));
     }
     if (MODE != kSaveDist && H == 0) return;
-    const uint32_t innerBase = h0.x, leafBase = h0.y;
-    const uint32_t leafMask = h0.z & 0xFFFFu, c = (h0.z >> 16) & 31u;
-    const uint32_t kcodes = h0.w, childBase = h1.x;
-    uint64_t row;
-    if (ow == 1) {
-        row = t.axis == 0 ? h1.y : (t.axis == 1 ? h1.z : h1.w);
-    } else {
-        const uint32_t lo = t.axis == 0 ? h1.y : (t.axis == 1 ? h1.w : h2.y);
-        const uint32_t hi = t.axis == 0 ? h1.z : (t.axis == 1 ? h2.x : h2.z);
-        row = ((uint64_t)hi << 32) | lo;
-    }
+    const uint32_t innerBase = hd.innerBase, leafBase = hd.leafBase;
+    const uint32_t leafMask = hd.leafMask, c = hd.c;
+    const uint32_t kcodes = hd.kcodes;
+    const uint64_t row = hd.row;
     // stack entry of a real child slot s < c (no branches: both child kinds
     // are computed and selected)
     const uint32_t innerMask = ~leafMask;
@@ -484,7 +514,7 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
                 const uint32_t s = (uint32_t)__builtin_ctz(m);
                 uint32_t e;
                 if (s >= c) {
-                    const uint32_t id = childBase + s;
+                    const uint32_t id = rec[6 * M + child_base_word(M)] + s;
                     e = id < sc.nodeCount ? sc.nodeEntry[id] : kEntryInvalid;
                 } else {
                     e = entry_real(s);
@@ -640,7 +670,6 @@ __device__ __forceinline__ SceneCtx plain_scene(const KParams& p, uint32_t MT) {
     sc.boxesBounded = p.boxesBounded != 0;
     sc.M = MT;
     sc.recw = record_words(MT);
-    sc.ow = order_words(MT);
     return sc;
 }
 
@@ -759,7 +788,6 @@ __device__ __forceinline__ void trace_body(const KParams& p) {
                     sc.leafTest = bs.leafTest;
                     sc.M = bs.M;
                     sc.recw = record_words(bs.M);
-                    sc.ow = order_words(bs.M);
                     rays = it.rays;
                     hits = static_cast<rt_hit*>(it.out);
                     occluded = static_cast<uint8_t*>(it.out);
@@ -1118,7 +1146,6 @@ __global__ void __launch_bounds__(256) k_coop(KParams p) {
     SceneCtx sc = plain_scene(p, 0);
     sc.M = M;
     sc.recw = record_words(M);
-    sc.ow = order_words(M);
     const uint32_t LSTRIDE = leaf_stride(sc.Lm);
     // slots of the same program instance as this lane's slot
     uint32_t resMask = 0;
@@ -1222,21 +1249,11 @@ __global__ void __launch_bounds__(256) k_coop(KParams p) {
             const uint32_t H = group_ballot(hit);
             live |= H;
             if (H) {
-                const uint4 h0 = ld4u(rec + 6 * M);
-                const uint4 h1 = ld4u(rec + 6 * M + 4);
-                uint4 h2 = make_uint4(0, 0, 0, 0);
-                if (sc.ow == 2) h2 = ld4u(rec + 6 * M + 8);
-                const uint32_t innerBase = h0.x, leafBase = h0.y;
-                const uint32_t leafMask = h0.z & 0xFFFFu, c = (h0.z >> 16) & 31u;
-                const uint32_t kcodes = h0.w, childBase = h1.x;
-                uint64_t row;
-                if (sc.ow == 1) {
-                    row = axis == 0 ? h1.y : (axis == 1 ? h1.z : h1.w);
-                } else {
-                    const uint32_t lo = axis == 0 ? h1.y : (axis == 1 ? h1.w : h2.y);
-                    const uint32_t hi = axis == 0 ? h1.z : (axis == 1 ? h2.x : h2.z);
-                    row = ((uint64_t)hi << 32) | lo;
-                }
+                const NodeHeader hd = load_header(rec, M, axis);
+                const uint32_t innerBase = hd.innerBase, leafBase = hd.leafBase;
+                const uint32_t leafMask = hd.leafMask, c = hd.c;
+                const uint32_t kcodes = hd.kcodes;
+                const uint64_t row = hd.row;
                 // lane gl plays position gl of the push sequence source
                 uint32_t s;
                 bool valid;
@@ -1268,7 +1285,7 @@ __global__ void __launch_bounds__(256) k_coop(KParams p) {
                     if (liveAt) {
                         uint32_t en;
                         if (s >= c) {
-                            const uint32_t id = childBase + s;
+                            const uint32_t id = rec[6 * M + child_base_word(M)] + s;
                             en = id < sc.nodeCount ? sc.nodeEntry[id] : kEntryInvalid;
                         } else {
                             const uint32_t below = (1u << s) - 1u;

@@ -7,14 +7,27 @@
 //                  minX minY minZ maxX maxY maxZ (slot s of plane p at p*M+s).
 //                  Slots [childCount, k) are the reference's zero boxes,
 //                  slots >= k are never read.
-//   [6M, 6M+HW)    header block:
-//                  w0 innerBase   record index of the first inner child
-//                  w1 leafBase    leaf ordinal of the first leaf child
-//                  w2 leafMask (bits 0-15) | childCount << 16 | k << 24
-//                  w3 kcodes      2 bits per child slot: (k_child / 4) - 1
-//                  w4 childBase   FastNode id of child slot 0 (childIdBegin)
-//                  w5..           traverseOrderEachAxis, 4-bit slots,
-//                                 OW = ceil(M/8) words per axis
+//   [6M, ...)      header block, the fields
+//                  innerBase  record index of the first inner child
+//                  leafBase   leaf ordinal of the first leaf child
+//                  leafMask   bit s: child slot s is a leaf
+//                  c          childCount
+//                  kcodes     2 bits per child slot: (k_child / 4) - 1
+//                  childBase  FastNode id of child slot 0 (childIdBegin)
+//                  rows       traverseOrderEachAxis, 4-bit slots per axis
+//                  packed so that a lane reads what its visit needs with
+//                  one 16-byte load when M <= 8 (indices below 2^28):
+//    M = 4:  w0 innerBase | leafMask << 28
+//            w1 leafBase | (c - 1) << 28
+//            w2 kcodes | row_x << 8 | (row_y & 0xFF) << 24
+//            w3 row_y >> 8 | row_z << 8   (row_a = bits 8+16a.. of w3:w2)
+//            w4 childBase
+//    M = 8:  one block per axis a, w[4a .. 4a+3]:
+//            innerBase, leafBase, leafMask | c << 8 | kcodes << 16, row_a
+//            w12 childBase
+//    M > 8:  w0 innerBase, w1 leafBase, w2 leafMask | c << 16,
+//            w3 kcodes, w4 childBase, w5.. rows (OW = ceil(M/8) words
+//            per axis)
 // Children of one FastNode are contiguous (customTreeOrder), so the inner
 // children get consecutive record indices and the leaf children consecutive
 // leaf ordinals: the stack entry of child slot s is computed from
@@ -45,7 +58,11 @@ constexpr uint32_t kEntryIndexMask = 0x0FFFFFFFu;
 constexpr int kEntryKShift = 28;
 
 __host__ __device__ constexpr uint32_t order_words(uint32_t M) { return (M + 7) / 8; }
-__host__ __device__ constexpr uint32_t header_words(uint32_t M) { return 5 + 3 * order_words(M); }
+__host__ __device__ constexpr uint32_t header_words(uint32_t M) {
+    return M == 4 ? 5 : (M == 8 ? 13 : 5 + 3 * order_words(M));
+}
+// Header word holding childBase.
+__host__ __device__ constexpr uint32_t child_base_word(uint32_t M) { return M == 8 ? 12 : 4; }
 __host__ __device__ constexpr uint32_t record_words(uint32_t M) {
     return ((6 * M + header_words(M)) + 15) / 16 * 16;
 }
