@@ -128,7 +128,6 @@ struct rt_scene {
     uint32_t* nodeEntry = nullptr;
     uint8_t* leafLc = nullptr;  // getLeafLoopCount per leaf block (hit reconstruction)
     uint8_t* leafPrims = nullptr;  // real triangles per leaf block (visit statistics)
-    uint8_t* leafTest = nullptr;   // slots to test per leaf block for finite directions
     uint64_t bytes = 0;
     uint32_t blockSize = 256, stackWindow = 16;
     uint32_t cus = 0;
@@ -147,7 +146,7 @@ namespace {
 // the node/leaf memory).
 int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<float>& tris,
             std::vector<int32_t>& triIds, std::vector<uint32_t>& entry, std::vector<uint8_t>& leafLc,
-            std::vector<uint8_t>& leafTest, LayoutInfo& info) {
+            LayoutInfo& info) {
     const uint32_t M = d->node_memory, Lm = d->leaf_memory, G = d->gang, n = d->node_count;
     if (!(M == 4 || M == 8 || M == 12 || M == 16) || !(Lm == 4 || Lm == 8 || Lm == 12 || Lm == 16))
         return set_error(RT_ERR_UNSUPPORTED, "rt_upload_bvh: node/leaf memory must be 4, 8, 12 or 16");
@@ -267,16 +266,16 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<floa
                 for (uint32_t w = 0; w < OW; ++w) h[5 + a * OW + w] = (uint32_t)(rows[a] >> (32 * w));
         }
     }
-    // leaf blocks: p0 planes as stored, then e1 = p1 - p0 and e2 = p2 - p0
-    // (the edge vectors triIntersect/triAnyHit form first, in the same FP32
-    // subtraction) so the kernels do not recompute them for every ray
+    // leaf blocks (rt_layout.h): slot pairs of p0 as stored, e1 = p1 - p0
+    // and e2 = p2 - p0 (the edge vectors triIntersect/triAnyHit form first,
+    // in the same FP32 subtraction) so the kernels do not recompute them for
+    // every ray
     const uint32_t LS = leaf_stride(Lm);
     tris.assign((size_t)nLeafBlocks * LS, 0.0f);
     std::vector<uint32_t> blockK(nLeafBlocks, 0);
     for (uint32_t i = 0; i < n; ++i)
         if (isLeaf[i]) blockK[index[i]] = kOf[i];
     leafLc.assign(nLeafBlocks, 0);
-    leafTest.assign(nLeafBlocks, 0);
     for (uint64_t l = 0; l < nLeafBlocks; ++l) {
         float* dst = &tris[l * LS];
         const float* src = d->tri_points + l * block;
@@ -289,7 +288,10 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<floa
         for (uint32_t j = 0; j < k; ++j)
             for (uint32_t v = 0; v < 9; ++v)
                 if (!(src[v * k + j] == 0.0f)) last = j + 1;
-        leafTest[l] = (uint8_t)((last + 1) / 2 * 2);
+        // the count goes into chunk 0's first padding word, so the leaf loop
+        // reads it with the first pair of triangles
+        const uint32_t test = (last + 1) / 2 * 2;
+        std::memcpy(&dst[18], &test, sizeof test);
         // loop count as getLeafLoopCount reads it (fastNodeManager.h:1728-1746),
         // kept for hit reconstruction when it matches the layout's stride
         int lc = (int)Lm;
@@ -331,7 +333,6 @@ void free_scene(rt_scene* s) {
     if (s->nodeEntry) (void)hipFree(s->nodeEntry);
     if (s->leafLc) (void)hipFree(s->leafLc);
     if (s->leafPrims) (void)hipFree(s->leafPrims);
-    if (s->leafTest) (void)hipFree(s->leafTest);
     if (s->joinEv) (void)hipEventDestroy(s->joinEv);
     if (s->ownStream) (void)hipStreamDestroy(s->ownStream);
     (void)hipSetDevice(cur);
@@ -420,7 +421,7 @@ extern "C" int rt_scene_footprint(const rt_bvh_desc* d, uint64_t* bytes) {
         if (!std::isnan(d->bounds[(size_t)i * 6 * M])) ++inner;
     const uint64_t leaves = d->tri_float_count / (9ull * Lm);
     *bytes = inner * record_words(M) * 4ull + leaves * leaf_stride(Lm) * 4ull + leaves * Lm * 4ull +
-             (uint64_t)d->node_count * 4ull + 3 * std::max<uint64_t>(leaves, 64);
+             (uint64_t)d->node_count * 4ull + 2 * std::max<uint64_t>(leaves, 64);
     return RT_OK;
 }
 
@@ -429,11 +430,11 @@ extern "C" int rt_upload_bvh(const rt_bvh_desc* d, rt_scene_t* out) {
     std::vector<uint32_t> nodes, entry;
     std::vector<float> tris;
     std::vector<int32_t> triIds;
-    std::vector<uint8_t> leafLc, leafPrims, leafTest;
+    std::vector<uint8_t> leafLc, leafPrims;
     LayoutInfo info;
     int rc;
     try {
-        rc = convert(d, nodes, tris, triIds, entry, leafLc, leafTest, info);
+        rc = convert(d, nodes, tris, triIds, entry, leafLc, info);
     } catch (const std::bad_alloc&) {
         rc = set_error(RT_ERR_NOMEM, "rt_upload_bvh: out of host memory");
     }
@@ -464,8 +465,7 @@ extern "C" int rt_upload_bvh(const rt_bvh_desc* d, rt_scene_t* out) {
         (rc = dev_upload(&s->triIds, triIds, s->bytes)) != RT_OK ||
         (rc = dev_upload(&s->nodeEntry, entry, s->bytes)) != RT_OK ||
         (rc = dev_upload(&s->leafLc, leafLc, s->bytes)) != RT_OK ||
-        (rc = dev_upload(&s->leafPrims, leafPrims, s->bytes)) != RT_OK ||
-        (rc = dev_upload(&s->leafTest, leafTest, s->bytes)) != RT_OK) {
+        (rc = dev_upload(&s->leafPrims, leafPrims, s->bytes)) != RT_OK) {
         free_scene(s);
         return rc;
     }
@@ -615,7 +615,6 @@ extern "C" int rt_trace_steps(rt_scene_t s, const rt_ray* rays, uint32_t n, uint
     p.nodeCount = s->nodeCount;
     p.M = s->M;
     p.boxesBounded = s->boxesBounded ? 1u : 0u;
-    p.leafTest = s->leafTest;
     p.steps = dOut;
     p.stepLen = dLen;
     p.stepCap = step_cap;
@@ -738,7 +737,6 @@ int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, voi
     p.chunk = ray_chunk(n, grid, bd);
     p.boxesBounded = s->boxesBounded ? 1u : 0u;
     p.sched = s->sched;
-    p.leafTest = s->leafTest;
     constexpr size_t kHistWords = sizeof(rt_trace_hist) / 8;
     if (hist) {
         if (!w->hist) HIP_TRY(hipMalloc(reinterpret_cast<void**>(&w->hist), sizeof(rt_trace_hist)), "hipMalloc(hist)");
@@ -857,7 +855,6 @@ int rt::trace_in_scratch(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t 
     p.chunk = ray_chunk(n, grid, bd);
     p.boxesBounded = s->boxesBounded ? 1u : 0u;
     p.sched = s->sched;
-    p.leafTest = s->leafTest;
     HIP_TRY(launch_reset(p.work, st), "reset kernel");
     HIP_TRY(launch_trace(s->M, mode, S, bd, grid, p, st), "trace kernel launch");
     s->raysTraced += n;
@@ -1130,7 +1127,6 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
             bs.nodeCount = s->nodeCount;
             bs.M = s->M;
             bs.boxesBounded = s->boxesBounded ? 1u : 0u;
-            bs.leafTest = s->leafTest;
             bs.pad1 = bs.pad2 = 0;
             cap = std::max(cap, stack_capacity(s));
         }

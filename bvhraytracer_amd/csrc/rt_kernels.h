@@ -37,7 +37,6 @@ struct BatchScene {
     const float* tris;
     const int32_t* triIds;
     const uint32_t* nodeEntry;
-    const uint8_t* leafTest;  // slots to test per leaf block for finite directions
     uint32_t rootEntry, gang, Lm, nodeCount, M, boxesBounded, pad1, pad2;
 };
 
@@ -71,9 +70,6 @@ struct KParams {
     // rt_trace_hist counters (uint64) the kernel adds to
     const uint8_t* leafPrims;
     uint64_t* hist;
-    // slots to test per leaf block for rays with a finite direction (the
-    // trailing all-zero padding triangles are skipped)
-    const uint8_t* leafTest;
     // per-ray kernel wave schedule: 0 = every lane steps each iteration;
     // r > 0 = one phase per iteration, the leaf phase when
     // 16 * leafLanes >= r * nodeLanes

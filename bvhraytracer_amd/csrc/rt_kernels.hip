@@ -129,23 +129,20 @@ __device__ __forceinline__ float comp(const float4& v, int j) {
 }
 
 // Triangles of a leaf are read a pair of slots at a time: the pair's
-// 80-byte chunk (the leaf block layout) as four 16-byte loads and one
-// 8-byte load, 18 VGPRs of vertex data.
+// 80-byte chunk (the leaf block layout) as five 16-byte loads, the last
+// carrying chunk 0's count of slots to test.
 constexpr uint32_t LEAF_CHUNK = 2;
 struct LeafChunk {
-    float4 q[4];
-    float2 r;
+    float4 q[5];
     __device__ __forceinline__ void load(const float* T, uint32_t s0) {
         const float* c = T + (s0 >> 1) * 20;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) q[i] = ld4(c + 4 * i);
-        r = *reinterpret_cast<const float2*>(c + 16);
+        for (int i = 0; i < 5; ++i) q[i] = ld4(c + 4 * i);
     }
     // value i (0..8) of slot s0 + j
-    __device__ __forceinline__ float v(int i, int j) const {
-        const int w = 2 * i + j;
-        return w < 16 ? comp(q[w >> 2], w & 3) : (w == 16 ? r.x : r.y);
-    }
+    __device__ __forceinline__ float v(int i, int j) const { return comp(q[(2 * i + j) >> 2], (2 * i + j) & 3); }
+    // chunk 0 only: slots to test for a finite direction (the layout's word 18)
+    __device__ __forceinline__ uint32_t test_slots() const { return __float_as_uint(q[4].z); }
 };
 
 // maxAbsDimension (glmUtil.h:18-23)
@@ -207,7 +204,6 @@ struct SceneCtx {
     const int32_t* triIds;
     const uint32_t* nodeEntry;
     const uint8_t* leafPrims;  // real triangles per leaf block (visit statistics)
-    const uint8_t* leafTest;   // slots to test per leaf block for finite directions
     uint32_t rootEntry, G, Lm, nodeCount, M, recw;
     bool boxesBounded;  // every tested box coordinate is within +-2^100
 };
@@ -338,14 +334,15 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
         uint32_t prims = 0, triHits = 0;
         if constexpr (HIST) prims = sc.leafPrims[idx];
         // slots to test: all k, or for a finite direction up to the last
-        // triangle that is not all zeros (chunk 0 is always tested, so its
-        // loads do not wait for this count)
-        const uint32_t kt = t.fast ? min(k, (uint32_t)sc.leafTest[idx]) : k;
+        // triangle that is not all zeros; chunk 0 is always tested and
+        // carries that count
+        uint32_t kt = k;
         if (t.anyMode) {
             bool any = false;
             for (uint32_t s0 = 0; s0 == 0 || s0 < kt; s0 += LEAF_CHUNK) {
                 LeafChunk a;
                 a.load(T, s0);
+                if (s0 == 0 && t.fast) kt = min(k, a.test_slots());
 #pragma unroll
                 for (int j = 0; j < LEAF_CHUNK; ++j) {
                     float tt, b1, b2;
@@ -379,6 +376,7 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
             for (uint32_t s0 = 0; s0 == 0 || s0 < kt; s0 += LEAF_CHUNK) {
                 LeafChunk a;
                 a.load(T, s0);
+                if (s0 == 0 && t.fast) kt = min(k, a.test_slots());
 #pragma unroll
                 for (int j = 0; j < LEAF_CHUNK; ++j) {
                     float tt, b1, b2;
@@ -666,7 +664,6 @@ __device__ __forceinline__ SceneCtx plain_scene(const KParams& p, uint32_t MT) {
     sc.Lm = p.Lm;
     sc.nodeCount = p.nodeCount;
     sc.leafPrims = p.leafPrims;
-    sc.leafTest = p.leafTest;
     sc.boxesBounded = p.boxesBounded != 0;
     sc.M = MT;
     sc.recw = record_words(MT);
@@ -785,7 +782,6 @@ __device__ __forceinline__ void trace_body(const KParams& p) {
                     sc.Lm = bs.Lm;
                     sc.nodeCount = bs.nodeCount;
                     sc.boxesBounded = bs.boxesBounded != 0;
-                    sc.leafTest = bs.leafTest;
                     sc.M = bs.M;
                     sc.recw = record_words(bs.M);
                     rays = it.rays;

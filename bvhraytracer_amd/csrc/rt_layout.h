@@ -40,7 +40,11 @@
 //   slots' floats side by side (18 floats), then 2 floats of padding, so a
 //   pair of triangles is four 16-byte loads and one 8-byte load.  e1 = p1 -
 //   p0 and e2 = p2 - p0 are the FP32 differences the intersection kernels
-//   form first.  Slots >= k (the leaf's loop count) are zero.
+//   form first.  Slots >= k (the leaf's loop count) are zero.  Word 18 of
+//   chunk 0 (a uint32) is the number of leading slots to test for a ray
+//   with a finite direction: up to the last triangle that is not all
+//   zeros, rounded up to a pair (an all-zero triangle has divisor 0 for
+//   every finite direction and never passes the test).
 //
 // Stack entry (32 bits):
 //   bit 31        leaf
