@@ -31,19 +31,31 @@ HBM_PEAK_GBPS = 8000.0
 # in KiB.  Keyed by (branching, leaf size, rays); the commands and the
 # per-dispatch values are in profiles/k_trace_N8L8_diffuse8M_hbm_traffic.md.
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
-MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 8008795, "write": 398439},
+MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 6979199, "write": 402846},
                         # batched leg (profiles/k_trace_batched_64scenes_hbm_traffic.md)
-                        ("batch", 8, 8, 9_872_996): {"fetch": 3029947, "write": 410084}}
+                        ("batch", 8, 8, 9_872_996): {"fetch": 2713611, "write": 412098}}
 
 # What binds the timed kernel, from rocprofv3 counter runs of this benchmark
-# (profiles/k_trace_N8L8_diffuse8M_l2_valu_occupancy.md): the VALU pipes are
-# busy 75% of the time with 41.7% of the lanes active, the rest is waits on
-# dependent loads; HBM carries a fifth of its peak, and the L1s request
-# 62.9 GB per step from L2 (9.836e8 64-byte reads), 86% of it hitting there.
-MEASURED_COUNTERS = {(8, 8, 8_000_000): {"valu_busy": 0.747, "active_lanes": 0.417, "l2_hit": 0.864,
-                                         "l2_read_bytes": 62_950_400_000,
-                                         "valu_insts_per_ray": 527.5,
-                                         "source": "profiles/k_trace_N8L8_diffuse8M_l2_valu_occupancy.md"}}
+# (profiles/k_trace_N8L8_diffuse8M_l2_valu_occupancy.md and
+# profiles/k_trace_N8L8_diffuse8M_memory_pipeline.md): the texture data and
+# address units that serve the per-lane loads are 97% and 89% busy, the VALU
+# pipes 84% (43.9% of the lanes active); HBM carries a quarter of its peak,
+# and the L1s request 47.6 GB per step from L2 (7.437e8 64-byte reads), 84%
+# of it hitting there.
+MEASURED_COUNTERS = {(8, 8, 8_000_000): {"td_busy": 0.966, "ta_busy": 0.891, "valu_busy": 0.842,
+                                         "active_lanes": 0.439, "l2_hit": 0.843,
+                                         "l2_read_bytes": 47_598_500_000,
+                                         "valu_insts_per_ray": 495.2, "loads_per_ray": 15.5,
+                                         "source": "profiles/k_trace_N8L8_diffuse8M_memory_pipeline.md, "
+                                                   "profiles/k_trace_N8L8_diffuse8M_l2_valu_occupancy.md"}}
+
+
+def limiter_text(c):
+    """What binds the kernel, from the counter runs."""
+    if not c or "td_busy" not in c:
+        return None
+    return (f"per-lane vector-memory loads: TA {c['ta_busy']:.0%} and TD {c['td_busy']:.0%} busy, "
+            f"VALU {c['valu_busy']:.0%}; HBM carries a fraction of its peak (hbm_frac)")
 
 
 def measured_traffic(key):
@@ -57,13 +69,15 @@ def log(msg):
 
 
 def header_bytes(M):
-    # inner-node header block the kernel loads per visit (rt_layout.h)
-    return 32 if M <= 8 else 48
+    # inner-node header bytes the kernel loads per visit (rt_layout.h): one
+    # 16-byte block for M <= 8, three for M = 12, 16
+    return 16 if M <= 8 else 48
 
 
 def algorithmic_bytes(counts, M):
     """Bytes the traversal must move per the device layout: ray in (40 B),
-    hit out (24 B), 24*k box bytes + header per inner visit, 36*k per leaf."""
+    hit out (24 B), 24*k box bytes + header per inner visit, 36*k per leaf
+    (the count of slots to test rides in the leaf's first load)."""
     c = counts
     return (64 * len(c) + 24 * c["node_slots"].astype(np.float64).sum()
             + header_bytes(M) * c["nodes"].astype(np.float64).sum()
@@ -650,7 +664,7 @@ def main():
                      "algorithmic_bytes": int(alg_bytes),
                      "useful_bytes": int(useful), "useful_frac": round(useful / (k_ms / 1000.0) / 1e9 / HBM_PEAK_GBPS, 4),
                      "hbm_frac": hbm_frac(traffic, k_ms),
-                     "limiter": "VALU issue and dependent-load latency, not HBM bandwidth",
+                     "limiter": limiter_text(MEASURED_COUNTERS.get((N, L, n))),
                      "counters": MEASURED_COUNTERS.get((N, L, n)),
                      # bytes the L1s read from L2 per step (the counters' pass) over this run's kernel time
                      "l2_read_gbps": (round(MEASURED_COUNTERS[(N, L, n)]["l2_read_bytes"] / (k_ms / 1000.0) / 1e9, 1)
