@@ -32,11 +32,14 @@ def scene_key(tris, rays):
 
 class Autotuner:
     def __init__(self, cache_path=None, nl=DEFAULT_NL, blocks=DEFAULT_BLOCKS, windows=DEFAULT_WINDOWS,
-                 schedules=DEFAULT_SCHEDULES, sample=1 << 16, reps=3):
+                 schedules=DEFAULT_SCHEDULES, sample=1 << 16, reps=3, fill=1 << 20):
+        """sample: distinct rays of the scene timed; fill: rays per timed
+        launch (the sample repeated), enough to fill the GPU so the timing
+        measures throughput rather than the latency of the longest ray."""
         self.cache_path = cache_path
         self.nl, self.blocks, self.windows = tuple(nl), tuple(blocks), tuple(windows)
         self.schedules = tuple(schedules)
-        self.sample, self.reps = int(sample), int(reps)
+        self.sample, self.reps, self.fill = int(sample), int(reps), int(fill)
         self.cache = {}
         if cache_path and os.path.exists(cache_path):
             with open(cache_path) as f:
@@ -81,24 +84,25 @@ class Autotuner:
         import torch
 
         sample = rays[: self.sample]
-        key = key or scene_key(tris, sample) + f":{mode}"
+        key = key or scene_key(tris, sample) + f":{mode}:{self.fill}"
         if key in self.cache:
             return dict(self.cache[key], cached=True)
-        d_rays = torch.from_numpy(np.ascontiguousarray(sample).view(np.float32).reshape(-1, 10)).cuda()
+        timed = np.resize(np.ascontiguousarray(sample), max(len(sample), self.fill))
+        d_rays = torch.from_numpy(np.ascontiguousarray(timed).view(np.float32).reshape(-1, 10)).cuda()
         if mode == "any":
-            d_out = torch.empty(len(sample), dtype=torch.uint8, device="cuda")
+            d_out = torch.empty(len(timed), dtype=torch.uint8, device="cuda")
         else:
-            d_out = torch.empty((len(sample), 6), dtype=torch.int32, device="cuda")
+            d_out = torch.empty((len(timed), 6), dtype=torch.int32, device="cuda")
         t0 = time.time()
         best, tried = None, []
         for N, L in self.nl:
             bvh = Bvh(tris, N, L)
             scene = Scene(bvh)
             r = self.tune_launch(scene, d_rays, d_out, mode)
-            mrays = len(sample) / r["ms"] / 1e3
+            mrays = len(timed) / r["ms"] / 1e3
             cfg = {"N": N, "L": L, "block": r["block"], "window": r["window"], "schedule": r["schedule"],
                    "mrays": round(mrays, 2),
-                   "default_mrays": round(len(sample) / r["default_ms"] / 1e3, 2) if r["default_ms"] else None}
+                   "default_mrays": round(len(timed) / r["default_ms"] / 1e3, 2) if r["default_ms"] else None}
             tried.append(cfg)
             if best is None or mrays > best["mrays"]:
                 best = dict(cfg)
