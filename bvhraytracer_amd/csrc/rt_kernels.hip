@@ -829,12 +829,16 @@ __device__ __forceinline__ void trace_body(const KParams& p) {
     }
 }
 
-// Register budget of the plain launch: at most 96 VGPRs, 5 waves per SIMD
-// (measured faster than the unconstrained 100-VGPR build at 4 waves; 6 waves
-// spills).  The batched launch keeps its per-lane scene context in registers
-// and stays at 4 waves.
+// Register budget: at most 96 VGPRs, 5 waves per SIMD (measured faster than
+// the unconstrained builds at 4 waves: 100 VGPRs for the plain launch, 108
+// for the batched one, whose lanes keep their scene's pointers; the batched
+// kernel spills a little at 5 waves and still gains 7%; 6 waves spills more
+// and is slower).  The batched kernel for scenes of different node memories
+// (MT = 0) spills 59 words at 5 waves and runs a third slower, so it keeps
+// 4 waves.
 template <int MT, int MODE, int S, bool BATCH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BATCH ? 4 : 5))) k_trace(KParams p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BATCH && MT == 0 ? 4 : 5)))
+k_trace(KParams p) {
     trace_body<MT, MODE, S, BATCH, false>(p);
 }
 
