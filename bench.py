@@ -33,7 +33,7 @@ HBM_PEAK_GBPS = 8000.0
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 6979199, "write": 402846},
                         # batched leg (profiles/k_trace_batched_64scenes_hbm_traffic.md)
-                        ("batch", 8, 8, 9_872_996): {"fetch": 2713611, "write": 412098}}
+                        ("batch", 8, 8, 9_872_996): {"fetch": 2632015, "write": 412558}}
 
 # What binds the timed kernel, from rocprofv3 counter runs of this benchmark
 # (profiles/k_trace_N8L8_diffuse8M_l2_valu_occupancy.md and
