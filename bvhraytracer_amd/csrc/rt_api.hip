@@ -1136,7 +1136,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
         bi.counts = it.counts;
         bi.scene = si;
         bi.mode = (it.flags & RT_ANY) ? kAny : kClosest;
-        bi.start = (uint32_t)total;
+        bi.start = 0;
         bi.n = it.n;
         total += it.n;
         if (total >= 0x80000000ull) return set_error(RT_ERR_ARG, "rt_batch_trace: at most 2^31-1 rays per batch");
@@ -1144,10 +1144,31 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     if (used == 0) return RT_OK;
     const uint32_t S = b->stackWindow, bd = b->blockSize;
     const bool graph = (flags & RT_GRAPH) != 0;
-    // one node memory for the whole batch: the kernel specialised on it
+    // One node memory for the whole batch: the kernel specialised on it.
+    // Several: the items are ordered by node memory (stable, ascending) into
+    // groups that the grouped kernel runs one after another.  Item starts
+    // are the running ray offsets in that order.
     uint32_t batchM = b->hScenes[0].M;
     for (uint32_t i = 1; i < nScenes; ++i)
         if (b->hScenes[i].M != batchM) batchM = 0;
+    uint32_t nGroups = 0, groupM[kMaxGroups] = {}, groupEnd[kMaxGroups] = {};
+    {
+        std::stable_sort(b->hItems, b->hItems + used, [&](const BatchItem& x, const BatchItem& y) {
+            return b->hScenes[x.scene].M < b->hScenes[y.scene].M;
+        });
+        uint32_t off = 0;
+        for (uint32_t i = 0; i < used; ++i) {
+            BatchItem& bi = b->hItems[i];
+            const uint32_t m = b->hScenes[bi.scene].M;
+            if (nGroups == 0 || groupM[nGroups - 1] != m) {
+                if (nGroups == kMaxGroups) return set_error(RT_ERR_LAYOUT, "rt_batch_trace: more than 4 node memories");
+                groupM[nGroups++] = m;
+            }
+            bi.start = off;
+            off += bi.n;
+            groupEnd[nGroups - 1] = off;
+        }
+    }
     const int occ = std::max(1, trace_occupancy(batchM, kMixed, S, bd));
     const uint64_t full = (uint64_t)b->cus * (uint64_t)occ;
     const uint32_t grid = graph ? (uint32_t)full
@@ -1168,6 +1189,11 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     p.items = b->dItems;
     p.scenes = b->dScenes;
     p.nItems = used;
+    p.nGroups = nGroups;
+    for (uint32_t g = 0; g < nGroups; ++g) {
+        p.groupM[g] = groupM[g];
+        p.groupEnd[g] = groupEnd[g];
+    }
     p.chunk = ray_chunk(total, grid, bd);
     p.sched = kDefaultSched;
     if (graph) {

@@ -28,6 +28,13 @@ enum TraceMode { kClosest = 0, kAny = 1, kSaveDist = 2, kMixed = 3 };
 constexpr uint32_t kRayCounter = 0;
 constexpr uint32_t kErrWord = 32;  // rays stopped by a guard
 constexpr uint32_t kWorkWords = 64;
+// Batched launch grouped by node memory: group g's ray counter is word
+// kGroupCounter + kGroupCounterStride * g (inside the line launch_reset
+// zeroes; group 0 shares the plain counter's word).
+constexpr uint32_t kMaxGroups = 4;
+constexpr uint32_t kGroupCounter = kRayCounter;
+constexpr uint32_t kGroupCounterStride = 8;
+static_assert(kGroupCounter + kGroupCounterStride * (kMaxGroups - 1) < kErrWord, "group counters overlap");
 
 // Batched launch tables (device memory).  A batch is a list of items, each a
 // contiguous ray range of one scene in one mode; the global ray index space
@@ -86,10 +93,16 @@ struct KParams {
     const BatchItem* items;
     const BatchScene* scenes;
     uint32_t nItems;
+    // batched launch over several node memories (launch_trace with M = 0):
+    // items ordered by node memory into nGroups groups, group g holding
+    // global rays [groupEnd[g-1], groupEnd[g]) of node memory groupM[g]
+    uint32_t nGroups;
+    uint32_t groupM[kMaxGroups];
+    uint32_t groupEnd[kMaxGroups];
 };
 
 // Launches the per-ray kernel for node memory M on `stream`; mode kMixed
-// launches the batched kernel (M is then read per scene).
+// launches the batched kernel (M = 0: the grouped one, p.nGroups set).
 hipError_t launch_trace(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize,
                         uint32_t grid, const KParams& p, hipStream_t stream);
 

@@ -717,8 +717,11 @@ struct WaveQueue {
 // reads the node memory from the scene; MODE = kMixed picks closest/any per
 // item).
 // Body of the per-ray kernel; HIST adds the visit statistics.
+// Rays come from the counter `counter`, which hands out indices 0 .. n-1;
+// local index i is global ray base + i (the batched launch runs its groups
+// of scenes through one counter each).
 template <int MT, int MODE, int S, bool BATCH, bool HIST>
-__device__ __forceinline__ void trace_body(const KParams& p) {
+__device__ __forceinline__ void trace_body(const KParams& p, uint32_t* counter, uint32_t base, uint32_t n) {
     extern __shared__ uint32_t smem[];
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
     const size_t gslot = (size_t)blockIdx.x * bd + tid;
@@ -759,8 +762,9 @@ __device__ __forceinline__ void trace_body(const KParams& p) {
     t.sp = 0;
     while (true) {
         if (!q.drained()) {
-            const uint32_t rr = q.fetch(p.work + kRayCounter, p.n, !has, p.chunk);
-            if (!has && rr < p.n) {
+            const uint32_t local = q.fetch(counter, n, !has, p.chunk);
+            if (!has && local < n) {
+                const uint32_t rr = base + local;
                 r = rr;
                 bool anyMode = (MODE == kAny);
                 if constexpr (BATCH) {
@@ -833,20 +837,47 @@ __device__ __forceinline__ void trace_body(const KParams& p) {
 // the unconstrained builds at 4 waves: 100 VGPRs for the plain launch, 108
 // for the batched one, whose lanes keep their scene's pointers; the batched
 // kernel spills a little at 5 waves and still gains 7%; 6 waves spills more
-// and is slower).  The batched kernel for scenes of different node memories
-// (MT = 0) spills 59 words at 5 waves and runs a third slower, so it keeps
-// 4 waves.
+// and is slower).
 template <int MT, int MODE, int S, bool BATCH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BATCH && MT == 0 ? 4 : 5)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 k_trace(KParams p) {
-    trace_body<MT, MODE, S, BATCH, false>(p);
+    trace_body<MT, MODE, S, BATCH, false>(p, p.work + kRayCounter, 0, p.n);
+}
+
+// Batched launch over scenes of different node memories.  The host orders
+// the items by node memory into groups (p.groupM[g], global rays
+// [p.groupEnd[g-1], p.groupEnd[g])), each with its own ray counter.  Every
+// wave runs the groups in turn, each with the body specialised on that
+// group's node memory: it takes rays from group g until the group's counter
+// is exhausted and its own rays are finished, then moves to group g + 1.
+// Lanes of one wave therefore always run the same node memory's code (a
+// body that reads M per lane keeps every memory's registers live and
+// spilled 59 words at 5 waves, a third slower).  Only the last group's end
+// is the launch's tail; a wave that reaches a drained group leaves it after
+// one atomic.
+template <int S>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+k_trace_groups(KParams p) {
+    uint32_t base = 0;
+    for (uint32_t g = 0; g < p.nGroups; ++g) {
+        const uint32_t end = p.groupEnd[g];
+        uint32_t* counter = p.work + kGroupCounter + kGroupCounterStride * g;
+        switch (p.groupM[g]) {
+            case 4: trace_body<4, kMixed, S, true, false>(p, counter, base, end - base); break;
+            case 8: trace_body<8, kMixed, S, true, false>(p, counter, base, end - base); break;
+            case 12: trace_body<12, kMixed, S, true, false>(p, counter, base, end - base); break;
+            case 16: trace_body<16, kMixed, S, true, false>(p, counter, base, end - base); break;
+            default: break;
+        }
+        base = end;
+    }
 }
 
 // Instrumented per-ray kernel (rt_trace_histograms): same traversal, plus
 // visit counts by depth, child/leaf fullness and test/success counts.
 template <int MT, int MODE>
 __global__ void __launch_bounds__(256) k_trace_hist(KParams p) {
-    trace_body<MT, MODE, kHistWindow, false, true>(p);
+    trace_body<MT, MODE, kHistWindow, false, true>(p, p.work + kRayCounter, 0, p.n);
 }
 
 // Per-step workload of ray tiles (rt_trace_steps; the instrumented wide
@@ -1320,8 +1351,9 @@ __global__ void __launch_bounds__(256) k_coop(KParams p) {
     }
 }
 
-// Zeroes the ray counter's line before a launch.  The guard counter (its
-// own line, kErrWord) is not reset: it accumulates over the launches that
+// Zeroes the ray counters (the plain one and the batched launch's group
+// counters) before a launch.  The guard counter (its own line, kErrWord) is
+// not reset: it accumulates over the launches that
 // share the work block and the host reads it when it reports statistics.
 __global__ void k_reset(uint32_t* work) {
     for (uint32_t i = threadIdx.x; i < kErrWord; i += blockDim.x) work[i] = 0;
@@ -1414,14 +1446,21 @@ void* pick_batch_s(uint32_t S) {
 }
 
 // Batched kernel: specialised on the node memory when every scene of the
-// batch has the same M, generic (M read per scene) otherwise (M = 0).
+// batch has the same M; otherwise (M = 0) the grouped kernel, which runs
+// each group of scenes with the body specialised on its node memory.
 void* pick_batch(uint32_t S, uint32_t M) {
     switch (M) {
         case 4: return pick_batch_s<4>(S);
         case 8: return pick_batch_s<8>(S);
         case 12: return pick_batch_s<12>(S);
         case 16: return pick_batch_s<16>(S);
-        default: return pick_batch_s<0>(S);
+        default:
+            switch (S) {
+                case 8: return reinterpret_cast<void*>(&k_trace_groups<8>);
+                case 16: return reinterpret_cast<void*>(&k_trace_groups<16>);
+                case 32: return reinterpret_cast<void*>(&k_trace_groups<32>);
+                default: return nullptr;
+            }
     }
 }
 

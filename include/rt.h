@@ -282,7 +282,9 @@ int rt_scene_footprint(const rt_bvh_desc* desc, uint64_t* bytes);
 /* Many scenes and ray batches of uneven size in ONE persistent launch: the
  * rays of all items form one index space that the waves drain through a
  * shared work counter (work stealing at ray granularity); every lane carries
- * the scene of its current ray.  No reference counterpart: the reference
+ * the scene of its current ray.  Scenes of different N may share a batch:
+ * the items are grouped by node memory, one counter per group, and each
+ * group runs code specialised on its node memory.  No reference counterpart: the reference
  * traces one scene per CameraFast render (cameraFast.h:318-582). */
 typedef struct rt_batch* rt_batch_t;
 
