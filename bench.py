@@ -597,6 +597,7 @@ def main():
         return round(count / (float(np.median(ks)) / 1000.0) / 1e6, 1)
 
     variants = {"closest_per_ray": round(n / (k_ms / 1000.0) / 1e6, 1),
+                "closest_exact_boxes": kernel_mrays(d_rays, d_hits, n, exact_boxes=True),
                 "save_distance": kernel_mrays(d_rays, d_hits, n, mode="save_distance"),
                 "closest_stream": kernel_mrays(d_rays, d_hits, n, stream_sched=True),
                 "closest_cooperative": kernel_mrays(d_rays, d_hits, n, coop=True)}
@@ -604,6 +605,7 @@ def main():
     d_sh = torch.from_numpy(shadow.view(np.float32).reshape(-1, 10)).to(dev)
     d_occ = torch.empty(len(shadow), dtype=torch.uint8, device=dev)
     variants["shadow_any_hit"] = kernel_mrays(d_sh, d_occ, len(shadow), mode="any")
+    variants["shadow_any_hit_exact_boxes"] = kernel_mrays(d_sh, d_occ, len(shadow), mode="any", exact_boxes=True)
     variants["shadow_any_hit_stream"] = kernel_mrays(d_sh, d_occ, len(shadow), mode="any", stream_sched=True)
     log(f"traversal variants (Mrays/s, kernel time): {variants}")
     del d_sh, d_occ

@@ -22,6 +22,7 @@ RT_COUNTS = 8
 RT_DEVICE_PTRS = 16
 RT_NO_SYNC = 32
 RT_COOP = 64
+RT_EXACT_BOXES = 256
 
 RT_GRAPH = 128
 
@@ -73,7 +74,7 @@ class RtStats(ctypes.Structure):
         ("leaf_block_bytes", ctypes.c_uint32), ("last_kernel_ms", ctypes.c_float),
         ("rays_traced", ctypes.c_uint64), ("block_size", ctypes.c_uint32),
         ("stack_window", ctypes.c_uint32), ("max_stack", ctypes.c_uint32),
-        ("error_rays", ctypes.c_uint32),
+        ("error_rays", ctypes.c_uint32), ("quantized_record_bytes", ctypes.c_uint32),
     ]
 
 

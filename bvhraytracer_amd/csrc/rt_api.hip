@@ -90,7 +90,42 @@ void free_workspace(Workspace& w) {
 struct LayoutInfo {
     uint32_t inner = 0, leaves = 0, maxDepth = 0;
     bool boxesBounded = true;  // every stored box coordinate is within +-2^100
+    bool quantized = true;     // the quantized records are valid (rt_layout.h)
 };
+
+// Frame scale of one axis of a quantized record: the smallest power of two
+// s = 2^e with F + 255 s >= hi (real arithmetic), e >= -149 and e no more
+// than 30 below F's exponent, so that F + q s is exact in long double for
+// every q in [0, 255].
+int quant_exponent(float F, float hi) {
+    int eF = 0;
+    (void)std::frexp(F, &eF);
+    int e = std::max(-149, (F == 0.0f ? -149 : eF - 30));
+    const long double ext = (long double)hi - (long double)F;
+    while ((long double)F + 255.0L * std::ldexp(1.0L, e) < (long double)hi) ++e;
+    (void)ext;
+    return e;
+}
+
+// Quantized planes of a child interval [lo, hi] in the frame (F, s = 2^e):
+// the largest q_lo with F + q_lo s <= lo and the smallest q_hi with
+// F + q_hi s >= hi.  False when the interval leaves [F, F + 255 s].
+bool quantize(float F, int e, float lo, float hi, uint32_t& qlo, uint32_t& qhi) {
+    const long double s = std::ldexp(1.0L, e), f = F;
+    if ((long double)lo < f || (long double)hi > f + 255.0L * s || lo > hi) return false;
+    long double a = std::floor(((long double)lo - f) / s);
+    long double b = std::ceil(((long double)hi - f) / s);
+    a = std::min(255.0L, std::max(0.0L, a));
+    b = std::min(255.0L, std::max(0.0L, b));
+    while (a > 0 && f + a * s > (long double)lo) a -= 1;
+    while (a < 255 && f + (a + 1) * s <= (long double)lo) a += 1;
+    while (b < 255 && f + b * s < (long double)hi) b += 1;
+    while (b > 0 && f + (b - 1) * s >= (long double)hi) b -= 1;
+    if (f + a * s > (long double)lo || f + b * s < (long double)hi) return false;
+    qlo = (uint32_t)a;
+    qhi = (uint32_t)b;
+    return true;
+}
 
 uint32_t decode_node_k(const rt_bvh_desc* d, uint32_t i) {
     const uint32_t M = d->node_memory;
@@ -123,6 +158,7 @@ struct rt_scene {
     bool boxesBounded = false;
     uint32_t sched = kDefaultSched;
     uint32_t* nodes = nullptr;
+    uint32_t* qnodes = nullptr;  // quantized records (null when the scene does not qualify)
     float* tris = nullptr;
     int32_t* triIds = nullptr;
     uint32_t* nodeEntry = nullptr;
@@ -130,6 +166,7 @@ struct rt_scene {
     uint8_t* leafPrims = nullptr;  // real triangles per leaf block (visit statistics)
     uint64_t bytes = 0;
     uint32_t blockSize = 256, stackWindow = 16;
+    bool launchSet = false;  // rt_set_launch was called: its shape serves both node formats
     uint32_t cus = 0;
     std::unordered_map<hipStream_t, Workspace> ws;
     uint64_t useClock = 0;
@@ -144,9 +181,9 @@ namespace {
 // Builds the device layout on the host.  Validates the layout contract the
 // reference's traversal relies on (contiguous children, loop counts within
 // the node/leaf memory).
-int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<float>& tris,
-            std::vector<int32_t>& triIds, std::vector<uint32_t>& entry, std::vector<uint8_t>& leafLc,
-            LayoutInfo& info) {
+int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<uint32_t>& qnodes,
+            std::vector<float>& tris, std::vector<int32_t>& triIds, std::vector<uint32_t>& entry,
+            std::vector<uint8_t>& leafLc, LayoutInfo& info) {
     const uint32_t M = d->node_memory, Lm = d->leaf_memory, G = d->gang, n = d->node_count;
     if (!(M == 4 || M == 8 || M == 12 || M == 16) || !(Lm == 4 || Lm == 8 || Lm == 12 || Lm == 16))
         return set_error(RT_ERR_UNSUPPORTED, "rt_upload_bvh: node/leaf memory must be 4, 8, 12 or 16");
@@ -184,6 +221,10 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<floa
         entry[i] = (isLeaf[i] ? kEntryLeaf : 0u) | (((kOf[i] / 4) - 1) << kEntryKShift) | index[i];
     const uint32_t RECW = record_words(M), OW = order_words(M);
     nodes.assign((size_t)nInner * RECW, 0u);
+    // own box of every FastNode as its parent's slot holds it (the root: the
+    // union of its children, filled below)
+    std::vector<float> own((size_t)n * 6, 0.0f);
+    std::vector<uint8_t> hasOwn(n, 0);
     std::vector<uint32_t> depth(n, 0);
     uint32_t maxDepth = 0;
     for (uint32_t i = 0; i < n; ++i) {
@@ -219,6 +260,9 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<floa
         uint32_t ni = 0, nl = 0;
         for (uint32_t s = 0; s < c; ++s) {
             const uint32_t ch = cb + s;
+            for (uint32_t p = 0; p < 6; ++p) own[(size_t)ch * 6 + p] = bd[p * k + s];
+            if (hasOwn[ch]) info.quantized = false;  // a FastNode with two parents
+            hasOwn[ch] = 1;
             kcodes |= ((kOf[ch] / 4) - 1) << (2 * s);
             depth[ch] = depth[i] + 1;
             maxDepth = std::max(maxDepth, depth[ch]);
@@ -266,7 +310,72 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<floa
                 for (uint32_t w = 0; w < OW; ++w) h[5 + a * OW + w] = (uint32_t)(rows[a] >> (32 * w));
         }
     }
-    // leaf blocks (rt_layout.h): slot pairs of p0 as stored, e1 = p1 - p0
+    if (!info.boxesBounded || nInner >= (1u << 27) || nLeafBlocks >= (1u << 27)) info.quantized = false;
+    // the root's own box: the union of its children (it is never tested)
+    if (!isLeaf[0] && kOf[0] > 0) {
+        const float* bd = d->bounds;
+        int mx = 0;
+        for (uint32_t q = 0; q < M; ++q) mx = std::max<int>(mx, d->order[q]);
+        for (uint32_t p = 0; p < 6; ++p) {
+            float v = bd[p * kOf[0]];
+            for (uint32_t s = 1; s <= (uint32_t)mx; ++s)
+                v = p < 3 ? std::min(v, bd[p * kOf[0] + s]) : std::max(v, bd[p * kOf[0] + s]);
+            own[p] = v;
+        }
+        hasOwn[0] = 1;
+    }
+    const uint32_t QW = qrecord_words(M), QP = qplanes_word(M);
+    if (info.quantized) qnodes.assign((size_t)nInner * QW, 0u);
+    for (uint32_t i = 0; i < n && info.quantized; ++i) {
+        if (isLeaf[i]) continue;
+        const float* ob = &own[(size_t)i * 6];
+        const float* bd = d->bounds + (size_t)i * 6 * M;
+        const uint32_t k = kOf[i];
+        const uint32_t* rec = &nodes[(size_t)index[i] * RECW];
+        uint32_t* q = &qnodes[(size_t)index[i] * QW];
+        for (uint32_t p = 0; p < 6; ++p) {
+            if (!(std::fabs(ob[p]) <= kQMaxCoord)) info.quantized = false;
+            std::memcpy(&q[p], &ob[p], 4);
+        }
+        if (!info.quantized) break;
+        int mx = 0;
+        for (uint32_t qq = 0; qq < M; ++qq) mx = std::max<int>(mx, d->order[(size_t)i * 3 * M + qq]);
+        const uint32_t c = (uint32_t)mx + 1;
+        uint8_t* planes = reinterpret_cast<uint8_t*>(q + QP);
+        for (uint32_t a = 0; a < 3 && info.quantized; ++a) {
+            const int e = quant_exponent(ob[a], ob[3 + a]);
+            q[6] |= (uint32_t)(e + kQExpBias) << (8 * a);
+            for (uint32_t s = 0; s < c; ++s) {
+                uint32_t qlo = 0, qhi = 0;
+                const float lo = bd[a * k + s], hi = bd[(3 + a) * k + s];
+                if (!(std::fabs(lo) <= kQMaxCoord && std::fabs(hi) <= kQMaxCoord) ||
+                    !quantize(ob[a], e, lo, hi, qlo, qhi)) {
+                    info.quantized = false;
+                    break;
+                }
+                planes[qplane_byte(M, a, 0) + s] = (uint8_t)qlo;
+                planes[qplane_byte(M, a, 1) + s] = (uint8_t)qhi;
+            }
+        }
+        // the padded slots [c, k) must be zero boxes: the kernel tests them as such
+        for (uint32_t p = 0; p < 6; ++p)
+            for (uint32_t s = c; s < k; ++s)
+                if (!(bd[p * k + s] == 0.0f)) info.quantized = false;
+        // header: the exact record's, without childBase (w7 here)
+        const uint32_t* h = rec + 6 * M;
+        q[7] = h[child_base_word(M)];
+        if (M == 4) {
+            for (uint32_t w = 0; w < 4; ++w) q[kQHeader + w] = h[w];
+        } else if (M == 8) {
+            for (uint32_t w = 0; w < 12; ++w) q[kQHeader + w] = h[w];
+        } else {
+            for (uint32_t w = 0; w < 4; ++w) q[kQHeader + w] = h[w];
+            for (uint32_t w = 0; w < 3 * OW; ++w) q[kQHeader + 4 + w] = h[5 + w];
+        }
+    }
+    if (!info.quantized) qnodes.clear();
+    // leaf blocks (rt_layout.h): the leaf's own box in the head, then slot
+    // pairs of p0 as stored, e1 = p1 - p0
     // and e2 = p2 - p0 (the edge vectors triIntersect/triAnyHit form first,
     // in the same FP32 subtraction) so the kernels do not recompute them for
     // every ray
@@ -274,7 +383,11 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<floa
     tris.assign((size_t)nLeafBlocks * LS, 0.0f);
     std::vector<uint32_t> blockK(nLeafBlocks, 0);
     for (uint32_t i = 0; i < n; ++i)
-        if (isLeaf[i]) blockK[index[i]] = kOf[i];
+        if (isLeaf[i]) {
+            blockK[index[i]] = kOf[i];
+            float* head = &tris[(size_t)index[i] * LS];
+            for (uint32_t p = 0; p < 6; ++p) head[p] = own[(size_t)i * 6 + p];
+        }
     leafLc.assign(nLeafBlocks, 0);
     for (uint64_t l = 0; l < nLeafBlocks; ++l) {
         float* dst = &tris[l * LS];
@@ -291,7 +404,7 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<floa
         // the count goes into chunk 0's first padding word, so the leaf loop
         // reads it with the first pair of triangles
         const uint32_t test = (last + 1) / 2 * 2;
-        std::memcpy(&dst[18], &test, sizeof test);
+        std::memcpy(&dst[kLeafHead + 18], &test, sizeof test);
         // loop count as getLeafLoopCount reads it (fastNodeManager.h:1728-1746),
         // kept for hit reconstruction when it matches the layout's stride
         int lc = (int)Lm;
@@ -328,6 +441,7 @@ void free_scene(rt_scene* s) {
     (void)hipSetDevice(s->device);
     for (auto& kv : s->ws) free_workspace(kv.second);
     if (s->nodes) (void)hipFree(s->nodes);
+    if (s->qnodes) (void)hipFree(s->qnodes);
     if (s->tris) (void)hipFree(s->tris);
     if (s->triIds) (void)hipFree(s->triIds);
     if (s->nodeEntry) (void)hipFree(s->nodeEntry);
@@ -356,6 +470,17 @@ uint32_t ray_chunk(uint64_t n, uint64_t grid, uint32_t bd) {
 // Stack capacity per ray: every level keeps at most M-1 pending siblings
 // plus the children of the node being expanded.
 uint32_t stack_capacity(const rt_scene* s) { return ((1 + (s->maxDepth + 1) * s->M + 8) + 7) / 8 * 8; }
+
+// The per-ray closest / any-hit kernels run on the quantized records when the
+// scene has them, unless the caller asks for the exact-box records.
+bool use_quantized(const rt_scene* s, int mode, uint32_t flags) {
+    return s->qnodes && (mode == kClosest || mode == kAny) && !(flags & RT_EXACT_BOXES);
+}
+
+// Launch shape of the quantized records until rt_set_launch is called: their
+// stack carries a float beside each entry, so an 8-entry LDS window keeps the
+// LDS of 6 waves per SIMD (the exact-box records default to 256 x 16).
+constexpr uint32_t kQBlock = 128, kQWindow = 8;
 
 int get_workspace(rt_scene* s, hipStream_t st, Workspace*& out) {
     auto it = s->ws.find(st);
@@ -420,21 +545,21 @@ extern "C" int rt_scene_footprint(const rt_bvh_desc* d, uint64_t* bytes) {
     for (uint32_t i = 0; i < d->node_count; ++i)
         if (!std::isnan(d->bounds[(size_t)i * 6 * M])) ++inner;
     const uint64_t leaves = d->tri_float_count / (9ull * Lm);
-    *bytes = inner * record_words(M) * 4ull + leaves * leaf_stride(Lm) * 4ull + leaves * Lm * 4ull +
+    *bytes = inner * (record_words(M) + qrecord_words(M)) * 4ull + leaves * leaf_stride(Lm) * 4ull + leaves * Lm * 4ull +
              (uint64_t)d->node_count * 4ull + 2 * std::max<uint64_t>(leaves, 64);
     return RT_OK;
 }
 
 extern "C" int rt_upload_bvh(const rt_bvh_desc* d, rt_scene_t* out) {
     if (!d || !out) return set_error(RT_ERR_ARG, "rt_upload_bvh: null argument");
-    std::vector<uint32_t> nodes, entry;
+    std::vector<uint32_t> nodes, qnodes, entry;
     std::vector<float> tris;
     std::vector<int32_t> triIds;
     std::vector<uint8_t> leafLc, leafPrims;
     LayoutInfo info;
     int rc;
     try {
-        rc = convert(d, nodes, tris, triIds, entry, leafLc, info);
+        rc = convert(d, nodes, qnodes, tris, triIds, entry, leafLc, info);
     } catch (const std::bad_alloc&) {
         rc = set_error(RT_ERR_NOMEM, "rt_upload_bvh: out of host memory");
     }
@@ -461,7 +586,8 @@ extern "C" int rt_upload_bvh(const rt_bvh_desc* d, rt_scene_t* out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, s->device) == hipSuccess) s->cus = (uint32_t)prop.multiProcessorCount;
     if (!s->cus) s->cus = 256;
-    if ((rc = dev_upload(&s->nodes, nodes, s->bytes)) != RT_OK || (rc = dev_upload(&s->tris, tris, s->bytes)) != RT_OK ||
+    if ((!qnodes.empty() && (rc = dev_upload(&s->qnodes, qnodes, s->bytes)) != RT_OK) ||
+        (rc = dev_upload(&s->nodes, nodes, s->bytes)) != RT_OK || (rc = dev_upload(&s->tris, tris, s->bytes)) != RT_OK ||
         (rc = dev_upload(&s->triIds, triIds, s->bytes)) != RT_OK ||
         (rc = dev_upload(&s->nodeEntry, entry, s->bytes)) != RT_OK ||
         (rc = dev_upload(&s->leafLc, leafLc, s->bytes)) != RT_OK ||
@@ -482,6 +608,7 @@ extern "C" int rt_set_launch(rt_scene_t s, uint32_t block_size, uint32_t stack_w
     std::lock_guard<std::mutex> lk(s->mu);
     if (block_size) s->blockSize = block_size;
     if (stack_window) s->stackWindow = stack_window;
+    if (block_size || stack_window) s->launchSet = true;
     return RT_OK;
 }
 
@@ -634,7 +761,7 @@ namespace {
 int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, void* out, rt_counts* counts,
                void* hip_stream, rt_trace_hist* hist) {
     if (!s) return set_error(RT_ERR_ARG, "rt_trace: null scene");
-    if (flags & ~(RT_ANY | RT_SAVE_DIST | RT_STREAM | RT_COUNTS | RT_DEVICE_PTRS | RT_NO_SYNC | RT_COOP))
+    if (flags & ~(RT_ANY | RT_SAVE_DIST | RT_STREAM | RT_COUNTS | RT_DEVICE_PTRS | RT_NO_SYNC | RT_COOP | RT_EXACT_BOXES))
         return set_error(RT_ERR_ARG, "rt_trace: unknown flag bits");
     if (n == 0) return RT_OK;
     if (!rays || !out) return set_error(RT_ERR_ARG, "rt_trace: null rays or output");
@@ -671,17 +798,19 @@ int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, voi
 
     const bool stream = (flags & RT_STREAM) != 0;
     const bool coop = (flags & RT_COOP) != 0;
-    const uint32_t S = hist ? kHistWindow : s->stackWindow;
+    const bool quant = !hist && !stream && !coop && use_quantized(s, mode, flags);
+    const bool qShape = quant && !s->launchSet;
+    const uint32_t S = hist ? kHistWindow : (qShape ? kQWindow : s->stackWindow);
     const uint32_t W = coop_width(s->M, s->Lm);
     const uint32_t coopDepth = (stack_capacity(s) + 63) / 64 * 64;
     if (coop && (size_t)(256 / W) * coopDepth * 4 > 65536)
         return set_error(RT_ERR_UNSUPPORTED, "rt_trace: tree too deep for the cooperative kernel's LDS stack");
-    uint32_t bd = hist ? trace_hist_block(s->M, mode) : ((stream || coop) ? 256u : s->blockSize);
-    while (!stream && !coop && !hist && bd > 64 && trace_lds_bytes(s->M, mode, S, bd) > 65536) bd -= 64;
+    uint32_t bd = hist ? trace_hist_block(s->M, mode) : ((stream || coop) ? 256u : (qShape ? kQBlock : s->blockSize));
+    while (!stream && !coop && !hist && bd > 64 && trace_lds_bytes(s->M, mode, S, bd, quant) > 65536) bd -= 64;
     const int occ = std::max(1, hist ? trace_hist_occupancy(s->M, mode)
                                      : stream ? stream_occupancy(s->M, mode, S)
                                               : (coop ? coop_occupancy(W, mode, coopDepth)
-                                                      : trace_occupancy(s->M, mode, S, bd)));
+                                                      : trace_occupancy(s->M, mode, S, bd, quant)));
     const uint64_t wantBlocks = (uint64_t)s->cus * (uint64_t)occ;
     const uint64_t raysPerBlock = coop ? bd / W : bd;
     const uint64_t needBlocks = ((uint64_t)n + raysPerBlock - 1) / raysPerBlock;
@@ -690,7 +819,7 @@ int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, voi
     const uint32_t spillCap = cap > S ? cap - S : 0;
     const size_t slots = (size_t)grid * bd;
     const size_t spillNeed =
-        std::max<size_t>(64, slots * spillCap * 4 * (hist ? 3 : (mode == kSaveDist ? 2 : 1)));
+        std::max<size_t>(64, slots * spillCap * 4 * (hist ? 3 : ((mode == kSaveDist || quant) ? 2 : 1)));
     const bool dev = (flags & RT_DEVICE_PTRS) != 0;
     const size_t outBytes = (mode == kAny) ? (size_t)n : (size_t)n * sizeof(rt_hit);
     const size_t rayBytes = (size_t)n * sizeof(rt_ray);
@@ -716,7 +845,7 @@ int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, voi
         HIP_TRY(hipMemcpyAsync(const_cast<rt_ray*>(dRays), rays, rayBytes, hipMemcpyHostToDevice, st), "hipMemcpy(rays)");
     }
     KParams p{};
-    p.nodes = s->nodes;
+    p.nodes = quant ? s->qnodes : s->nodes;
     p.tris = s->tris;
     p.triIds = s->triIds;
     p.nodeEntry = s->nodeEntry;
@@ -749,7 +878,7 @@ int trace_impl(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags, voi
     if (hist) HIP_TRY(launch_trace_hist(s->M, mode, grid, p, st), "instrumented kernel launch");
     else if (coop) HIP_TRY(launch_coop(W, mode, grid, p, st), "cooperative kernel launch");
     else if (stream) HIP_TRY(launch_stream(s->M, mode, S, grid, p, st), "stream kernel launch");
-    else HIP_TRY(launch_trace(s->M, mode, S, bd, grid, p, st), "trace kernel launch");
+    else HIP_TRY(launch_trace(s->M, mode, S, bd, grid, p, st, quant), "trace kernel launch");
     HIP_TRY(hipEventRecord(w->ev1, st), "hipEventRecord");
     w->timed = true;
     s->raysTraced += n;
@@ -797,7 +926,7 @@ int rt::trace_in_scratch(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t 
                          const char* who) {
     const std::string w(who);
     if (!s) return set_error(RT_ERR_ARG, w + ": null scene");
-    if (flags & ~(RT_ANY | RT_SAVE_DIST | RT_COUNTS | RT_DEVICE_PTRS | RT_NO_SYNC | RT_STREAM | RT_COOP))
+    if (flags & ~(RT_ANY | RT_SAVE_DIST | RT_COUNTS | RT_DEVICE_PTRS | RT_NO_SYNC | RT_STREAM | RT_COOP | RT_EXACT_BOXES))
         return set_error(RT_ERR_ARG, w + ": unknown flag bits");
     if (flags & (RT_STREAM | RT_COOP))
         return set_error(RT_ERR_UNSUPPORTED, w + ": per-ray traversal only (no RT_STREAM or RT_COOP)");
@@ -818,13 +947,15 @@ int rt::trace_in_scratch(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t 
             if (dev != prev) (void)hipSetDevice(prev);
         }
     } restore{s->device, prevDev};
-    const uint32_t S = s->stackWindow;
-    uint32_t bd = s->blockSize;
-    while (bd > 64 && trace_lds_bytes(s->M, mode, S, bd) > 65536) bd -= 64;
-    const int occ = std::max(1, trace_occupancy(s->M, mode, S, bd));
+    const bool quant = use_quantized(s, mode, flags);
+    const bool qShape = quant && !s->launchSet;
+    const uint32_t S = qShape ? kQWindow : s->stackWindow;
+    uint32_t bd = qShape ? kQBlock : s->blockSize;
+    while (bd > 64 && trace_lds_bytes(s->M, mode, S, bd, quant) > 65536) bd -= 64;
+    const int occ = std::max(1, trace_occupancy(s->M, mode, S, bd, quant));
     const uint32_t cap = stack_capacity(s);
     const uint32_t spillCap = cap > S ? cap - S : 0;
-    const size_t perBlock = (size_t)bd * spillCap * 4 * (mode == kSaveDist ? 2 : 1);
+    const size_t perBlock = (size_t)bd * spillCap * 4 * ((mode == kSaveDist || quant) ? 2 : 1);
     const size_t spillBytes = scratchBytes - kWorkWords * 4;
     if (perBlock > spillBytes)
         return set_error(RT_ERR_UNSUPPORTED, w + ": the scene's stack spill for one block of " + std::to_string(bd) +
@@ -835,7 +966,7 @@ int rt::trace_in_scratch(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t 
     const uint32_t grid = (uint32_t)std::max<uint64_t>(
         1, std::min(std::min((uint64_t)s->cus * (uint64_t)occ, needBlocks), fit));
     KParams p{};
-    p.nodes = s->nodes;
+    p.nodes = quant ? s->qnodes : s->nodes;
     p.tris = s->tris;
     p.triIds = s->triIds;
     p.nodeEntry = s->nodeEntry;
@@ -856,7 +987,7 @@ int rt::trace_in_scratch(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t 
     p.boxesBounded = s->boxesBounded ? 1u : 0u;
     p.sched = s->sched;
     HIP_TRY(launch_reset(p.work, st), "reset kernel");
-    HIP_TRY(launch_trace(s->M, mode, S, bd, grid, p, st), "trace kernel launch");
+    HIP_TRY(launch_trace(s->M, mode, S, bd, grid, p, st, quant), "trace kernel launch");
     s->raysTraced += n;
     return RT_OK;
 }
@@ -904,6 +1035,7 @@ extern "C" int rt_stats(rt_scene_t s, rt_stats_t* o) {
     o->leaves = s->leaves;
     o->node_record_bytes = record_words(s->M) * 4;
     o->leaf_block_bytes = leaf_stride(s->Lm) * 4;
+    o->quantized_record_bytes = s->qnodes ? qrecord_words(s->M) * 4 : 0;
     o->last_kernel_ms = s->lastMs;
     o->rays_traced = s->raysTraced;
     o->block_size = s->blockSize;
@@ -945,6 +1077,7 @@ struct rt_batch {
     uint32_t* spill = nullptr;
     size_t spillBytes = 0;
     uint32_t blockSize = 256, stackWindow = 16, cus = 0;
+    bool launchSet = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool pending = false;
     hipStream_t ownStream = nullptr;
@@ -956,6 +1089,7 @@ struct rt_batch {
     hipKernelNodeParams traceNodeParams{};
     hipStream_t graphStream = nullptr;
     uint32_t graphGrid = 0, graphBd = 0, graphS = 0, graphSpillCap = 0, graphM = 0;
+    bool graphQ = false;
     uint32_t* graphSpill = nullptr;
     float lastMs = 0;
     uint64_t raysTraced = 0;
@@ -993,11 +1127,11 @@ void free_batch(rt_batch* b) {
 }
 
 int capture_graph(rt_batch* b, hipStream_t st, const KParams& p, uint32_t grid, uint32_t bd, uint32_t S,
-                  uint32_t M) {
+                  uint32_t M, bool q) {
     drop_graph(b);
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipError_t e1 = launch_reset(p.work, st);
-    hipError_t e2 = launch_trace(M, kMixed, S, bd, grid, p, st);
+    hipError_t e2 = launch_trace(M, kMixed, S, bd, grid, p, st, q);
     hipGraph_t g = nullptr;
     hipError_t e3 = hipStreamEndCapture(st, &g);
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
@@ -1028,6 +1162,7 @@ int capture_graph(rt_batch* b, hipStream_t st, const KParams& p, uint32_t grid, 
     b->graphSpill = b->spill;
     b->graphSpillCap = p.spillCap;
     b->graphM = M;
+    b->graphQ = q;
     return RT_OK;
 }
 
@@ -1067,13 +1202,14 @@ extern "C" int rt_batch_set_launch(rt_batch_t b, uint32_t block_size, uint32_t s
     std::lock_guard<std::mutex> lk(b->mu);
     if (block_size) b->blockSize = block_size;
     if (stack_window) b->stackWindow = stack_window;
+    if (block_size || stack_window) b->launchSet = true;
     return RT_OK;
 }
 
 extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t n_items, uint32_t flags,
                               void* hip_stream) {
     if (!b) return set_error(RT_ERR_ARG, "rt_batch_trace: null batch");
-    if (flags & ~(RT_GRAPH | RT_NO_SYNC)) return set_error(RT_ERR_ARG, "rt_batch_trace: unknown flag bits");
+    if (flags & ~(RT_GRAPH | RT_NO_SYNC | RT_EXACT_BOXES)) return set_error(RT_ERR_ARG, "rt_batch_trace: unknown flag bits");
     if (n_items == 0) return RT_OK;
     if (!items) return set_error(RT_ERR_ARG, "rt_batch_trace: null items");
     if (n_items > b->maxItems) return set_error(RT_ERR_ARG, "rt_batch_trace: more items than the batch was created for");
@@ -1117,7 +1253,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
             rt_scene* s = it.scene;
             scenes.push_back(s);
             BatchScene& bs = b->hScenes[nScenes++];
-            bs.nodes = s->nodes;
+            bs.nodes = s->nodes;  // the quantized records replace these below when every scene has them
             bs.tris = s->tris;
             bs.triIds = s->triIds;
             bs.nodeEntry = s->nodeEntry;
@@ -1142,7 +1278,6 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
         if (total >= 0x80000000ull) return set_error(RT_ERR_ARG, "rt_batch_trace: at most 2^31-1 rays per batch");
     }
     if (used == 0) return RT_OK;
-    const uint32_t S = b->stackWindow, bd = b->blockSize;
     const bool graph = (flags & RT_GRAPH) != 0;
     // One node memory for the whole batch: the kernel specialised on it.
     // Several: the items are ordered by node memory (stable, ascending) into
@@ -1151,6 +1286,13 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     uint32_t batchM = b->hScenes[0].M;
     for (uint32_t i = 1; i < nScenes; ++i)
         if (b->hScenes[i].M != batchM) batchM = 0;
+    // the quantized records when every scene of the batch has them
+    bool batchQ = !(flags & RT_EXACT_BOXES);
+    for (uint32_t i = 0; i < nScenes; ++i) batchQ = batchQ && scenes[i]->qnodes;
+    if (batchQ)
+        for (uint32_t i = 0; i < nScenes; ++i) b->hScenes[i].nodes = scenes[i]->qnodes;
+    const bool qShape = batchQ && !b->launchSet;
+    const uint32_t S = qShape ? kQWindow : b->stackWindow, bd = qShape ? kQBlock : b->blockSize;
     uint32_t nGroups = 0, groupM[kMaxGroups] = {}, groupEnd[kMaxGroups] = {};
     {
         std::stable_sort(b->hItems, b->hItems + used, [&](const BatchItem& x, const BatchItem& y) {
@@ -1169,13 +1311,13 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
             groupEnd[nGroups - 1] = off;
         }
     }
-    const int occ = std::max(1, trace_occupancy(batchM, kMixed, S, bd));
+    const int occ = std::max(1, trace_occupancy(batchM, kMixed, S, bd, batchQ));
     const uint64_t full = (uint64_t)b->cus * (uint64_t)occ;
     const uint32_t grid = graph ? (uint32_t)full
                                 : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(full, (total + bd - 1) / bd));
     uint32_t spillCap = cap > S ? cap - S : 0;
     if (graph && b->exec && b->graphSpillCap > spillCap && b->graphGrid == grid) spillCap = b->graphSpillCap;
-    const size_t spillNeed = std::max<size_t>(64, (size_t)grid * bd * spillCap * 4);
+    const size_t spillNeed = std::max<size_t>(64, (size_t)grid * bd * spillCap * 4 * (batchQ ? 2 : 1));
     int rc;
     if ((rc = ensure(reinterpret_cast<void**>(&b->spill), b->spillBytes, spillNeed, "hipMalloc(spill)")) != RT_OK)
         return rc;
@@ -1198,8 +1340,9 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     p.sched = kDefaultSched;
     if (graph) {
         if (!b->exec || b->graphStream != st || b->graphGrid != grid || b->graphBd != bd || b->graphS != S ||
-            b->graphSpill != b->spill || b->graphSpillCap != spillCap || b->graphM != batchM) {
-            if ((rc = capture_graph(b, st, p, grid, bd, S, batchM)) != RT_OK) return rc;
+            b->graphSpill != b->spill || b->graphSpillCap != spillCap || b->graphM != batchM ||
+            b->graphQ != batchQ) {
+            if ((rc = capture_graph(b, st, p, grid, bd, S, batchM, batchQ)) != RT_OK) return rc;
         } else {
             // re-arm: same graph, new kernel arguments (ray count, item count)
             hipKernelNodeParams kp = b->traceNodeParams;
@@ -1216,7 +1359,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     } else {
         HIP_TRY(launch_reset(b->work, st), "reset kernel");
         HIP_TRY(hipEventRecord(b->ev0, st), "hipEventRecord");
-        HIP_TRY(launch_trace(batchM, kMixed, S, bd, grid, p, st), "batched trace kernel");
+        HIP_TRY(launch_trace(batchM, kMixed, S, bd, grid, p, st, batchQ), "batched trace kernel");
         HIP_TRY(hipEventRecord(b->ev1, st), "hipEventRecord");
     }
     b->pending = true;

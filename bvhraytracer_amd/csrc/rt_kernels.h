@@ -103,15 +103,19 @@ struct KParams {
 
 // Launches the per-ray kernel for node memory M on `stream`; mode kMixed
 // launches the batched kernel (M = 0: the grouped one, p.nGroups set).
+// quantized: p.nodes (every scene's nodes, batched) are the quantized
+// records and p.spill has a second region of spillCap floats per slot
+// (closest, any and batched launches).
 hipError_t launch_trace(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize,
-                        uint32_t grid, const KParams& p, hipStream_t stream);
+                        uint32_t grid, const KParams& p, hipStream_t stream, bool quantized = false);
 
 // Blocks per CU the per-ray kernel reaches with this configuration.
-int trace_occupancy(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize);
+int trace_occupancy(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize, bool quantized = false);
 
 // Dynamic LDS bytes of the per-ray kernel (stack window, and for the
-// save-distance mode the stack distances and aabbDistances).
-size_t trace_lds_bytes(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize);
+// save-distance mode the stack distances and aabbDistances, for the
+// quantized records the entries' tmax).
+size_t trace_lds_bytes(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize, bool quantized = false);
 
 // Ray-stream kernel (closest / any): blocks of stream_block() ray slots.
 hipError_t launch_stream(uint32_t M, int mode, uint32_t stackWindow, uint32_t grid, const KParams& p,

@@ -65,10 +65,14 @@ __device__ __forceinline__ bool slab_fast(float bnx, float bny, float bnz, float
 // such a triangle has e1 = e2 = 0, so s1 = 0 and divisor = 0, and it never
 // passes the test (an infinite direction component would make the divisor
 // NaN, which passes).
+// Q: the quantized records' condition, which also bounds the origin and the
+// inverse direction by 2^60 (their boxes are within 2^60, so slab_fast's
+// condition holds too).
+template <bool Q = false>
 __device__ __forceinline__ bool ray_fast(const Ray& r) {
-    constexpr float kMaxO = 0x1p100f, kMaxF = 3.40282347e38f;
-    return fabsf(r.ox) <= kMaxO && fabsf(r.oy) <= kMaxO && fabsf(r.oz) <= kMaxO && fabsf(r.ix) <= kMaxF &&
-           fabsf(r.iy) <= kMaxF && fabsf(r.iz) <= kMaxF && fabsf(r.dx) <= kMaxF && fabsf(r.dy) <= kMaxF &&
+    constexpr float kMaxO = Q ? kQMaxCoord : 0x1p100f, kMaxF = 3.40282347e38f, kMaxI = Q ? kQMaxCoord : kMaxF;
+    return fabsf(r.ox) <= kMaxO && fabsf(r.oy) <= kMaxO && fabsf(r.oz) <= kMaxO && fabsf(r.ix) <= kMaxI &&
+           fabsf(r.iy) <= kMaxI && fabsf(r.iz) <= kMaxI && fabsf(r.dx) <= kMaxF && fabsf(r.dy) <= kMaxF &&
            fabsf(r.dz) <= kMaxF && r.tmax == r.tmax;
 }
 
@@ -135,7 +139,7 @@ constexpr uint32_t LEAF_CHUNK = 2;
 struct LeafChunk {
     float4 q[5];
     __device__ __forceinline__ void load(const float* T, uint32_t s0) {
-        const float* c = T + (s0 >> 1) * 20;
+        const float* c = T + kLeafHead + (s0 >> 1) * 20;
 #pragma unroll
         for (int i = 0; i < 5; ++i) q[i] = ld4(c + 4 * i);
     }
@@ -210,7 +214,8 @@ struct SceneCtx {
 
 struct StackCtx {
     uint32_t* stE;   // LDS stack entries of this ray, row stride `stride`
-    float* stD;      // LDS stack distances (save dist)
+    float* stD;      // LDS stack distances (save dist); tmax at the push
+                     // (quantized records)
     float* aabbD;    // LDS aabbDistances (save dist)
     uint32_t* spE;   // global spill of this ray
     float* spD;
@@ -236,8 +241,12 @@ struct Trav {
     bool hasCur, occl, err, anyMode, rev, fast;
     int axis;
     uint32_t curDepth;  // visit statistics: depth of `cur` (root = 0)
+    // quantized records: tmax at the push of `cur` (used when its entry is
+    // tentative)
+    float curT;
 };
 
+template <bool Q = false>
 __device__ __forceinline__ void trav_init(Trav& t, const rt_ray& in, uint32_t rootEntry, bool anyMode,
                                           bool boxesBounded) {
     t.ray.tmax = in.tmax;
@@ -253,7 +262,7 @@ __device__ __forceinline__ void trav_init(Trav& t, const rt_ray& in, uint32_t ro
     t.axis = max_abs_dim(t.ray.dx, t.ray.dy, t.ray.dz);
     const float da = t.axis == 0 ? t.ray.dx : (t.axis == 1 ? t.ray.dy : t.ray.dz);
     t.rev = da <= 0.0f;
-    t.fast = boxesBounded && ray_fast(t.ray);
+    t.fast = boxesBounded && ray_fast<Q>(t.ray);
     t.cur = rootEntry;
     t.curDist = 0.0f;
     t.curDepth = 0;
@@ -267,6 +276,7 @@ __device__ __forceinline__ void trav_init(Trav& t, const rt_ray& in, uint32_t ro
     t.err = false;
     t.anyMode = anyMode;
     t.cN = t.cL = t.cNS = t.cLS = 0;
+    t.curT = t.ray.tmax;
 }
 
 // Entry the next step visits (without taking it).
@@ -605,6 +615,382 @@ __device__ __forceinline__ void trav_step(const SceneCtx& sc, const StackCtx& sk
     }
 }
 
+// ---------------------------------------------------------------------------
+// Traversal step over the quantized node records (closest and any hit).
+//
+// A node's real children are tested on their quantized planes (rt_layout.h).
+// The test is conservative: for a ray with |o|, |inv| <= 2^60 (t.fast) its
+// near distance is at most, and its far distance at least, what the
+// reference computes on the exact box, so every child the reference's
+// aabbIntersect accepts passes.  Such a child is pushed tentatively with the
+// current tmax; its pop tests the exact box (the child's own box, stored at
+// the start of its record or leaf block) with that tmax, exactly as the
+// reference tested it at the parent, and skips the visit when the test fails
+// (nothing is counted, nothing pushed).  The visit sequence and the counts
+// are therefore the reference's.
+//
+// The quantized distance of plane q on axis a is fma(q, s*inv, (F - o)*inv)
+// with a margin m = ((|F| + |o|) + 256 s) * |inv| * 2^-21 + 2^-120 taken off
+// the near and added to the far side.  With K = |F| + |o| + 256 s bounding
+// every plane, origin and partial sum, the reference's two roundings and the
+// five of this form are each within u * K * |inv| (u = 2^-24), so 8 u K |inv|
+// covers them; 2^-120 covers results in the denormal range.
+//
+// Padded slots [c, k) hold the zero box, tested exactly as the reference does.
+// Rays outside the t.fast condition push every real child tentatively.
+
+// Exact box test of an entry's own box with the tmax of its push.  The
+// fast form is computed on every lane without a branch (so the box loads
+// issue with the rest of the visit's loads and are waited for once); lanes
+// outside its condition redo it in the exact form.
+__device__ __forceinline__ bool own_box_hit(const float4& b0, const float4& b1, const Ray& r, float tmax,
+                                            bool fast) {
+    Ray rr = r;
+    rr.tmax = tmax;
+    float t0;
+    bool hit = slab_fast(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, rr, t0);
+    if (!fast) hit = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, rr, t0);
+    return hit;
+}
+
+// Header fields of a quantized record (header at word kQHeader).
+__device__ __forceinline__ NodeHeader qload_header(const uint32_t* rec, uint32_t M, int axis) {
+    NodeHeader h;
+    const uint32_t* hw = rec + kQHeader;
+    if (M == 4) {
+        const uint4 v = ld4u(hw);
+        h.innerBase = v.x & kEntryIndexMask;
+        h.leafMask = v.x >> 28;
+        h.leafBase = v.y & kEntryIndexMask;
+        h.c = (v.y >> 28) + 1u;
+        h.kcodes = v.z & 0xFFu;
+        h.row = ((((uint64_t)v.w << 32) | v.z) >> (8 + 16 * axis)) & 0xFFFFu;
+    } else if (M == 8) {
+        const uint4 v = ld4u(hw + 4 * axis);
+        h.innerBase = v.x;
+        h.leafBase = v.y;
+        h.leafMask = v.z & 0xFFu;
+        h.c = (v.z >> 8) & 15u;
+        h.kcodes = v.z >> 16;
+        h.row = v.w;
+    } else {
+        const uint4 h0 = ld4u(hw), h1 = ld4u(hw + 4), h2 = ld4u(hw + 8);
+        h.innerBase = h0.x;
+        h.leafBase = h0.y;
+        h.leafMask = h0.z & 0xFFFFu;
+        h.c = (h0.z >> 16) & 31u;
+        h.kcodes = h0.w;
+        const uint32_t lo = axis == 0 ? h1.x : (axis == 1 ? h1.z : h2.x);
+        const uint32_t hi = axis == 0 ? h1.y : (axis == 1 ? h1.w : h2.y);
+        h.row = ((uint64_t)hi << 32) | lo;
+    }
+    return h;
+}
+
+// The quantized planes of a record, loaded whole: dword (a, j, hi) holds the
+// lo (hi = 0) or hi bytes of axis a for slots 4j .. 4j+3.
+template <int M>
+struct QPlanes {
+    static constexpr int kLoads = M == 4 ? 2 : (M == 8 ? 3 : 6);
+    uint4 v[kLoads];
+    __device__ __forceinline__ void load(const uint32_t* rec) {
+        const uint32_t* p = rec + qplanes_word(M);
+#pragma unroll
+        for (int i = 0; i < kLoads; ++i) v[i] = ld4u(p + 4 * i);
+    }
+    __device__ __forceinline__ uint32_t word(int w) const {
+        const uint4& q = v[w >> 2];
+        const int c = w & 3;
+        return c == 0 ? q.x : (c == 1 ? q.y : (c == 2 ? q.z : q.w));
+    }
+    __device__ __forceinline__ uint32_t dword(int a, int j, int hi) const {
+        return word((int)(qplane_byte(M, a, hi) / 4) + j);
+    }
+};
+
+template <int MT, int S>
+__device__ __forceinline__ void trav_step_q(const SceneCtx& sc, const StackCtx& sk, Trav& t) {
+    constexpr uint32_t M = (uint32_t)MT;
+    const uint32_t bd = sk.stride;
+    uint32_t e;
+    float tp;
+    if (t.hasCur) {
+        e = t.cur;
+        tp = t.curT;
+        t.hasCur = false;
+    } else {
+        --t.sp;
+        const uint32_t slot = (t.sp & (S - 1)) * bd;
+        e = sk.stE[slot];
+        tp = sk.stD[slot];
+        if (t.sp >= S) {
+            sk.stE[slot] = sk.spE[t.sp - S];
+            sk.stD[slot] = sk.spD[t.sp - S];
+        }
+    }
+    if (e & kEntryInvalid) {
+        t.err = true;
+        return;
+    }
+    const bool tent = (e & kEntryTent) != 0;  // pushed tentatively: test the exact box first
+    const uint32_t k = ((e >> kEntryKShift) & 3u) * 4u + 4u;
+    const uint32_t idx = e & kEntryIndexMaskQ;
+    if (e & kEntryLeaf) {
+        const float* T = sc.tris + (size_t)idx * leaf_stride(sc.Lm);
+        const float4 h0 = ld4(T), h1 = ld4(T + 4);
+        LeafChunk a0;
+        a0.load(T, 0);
+        const bool ob = own_box_hit(h0, h1, t.ray, tp, t.fast);
+        if (tent && !ob) return;
+        ++t.cL;
+        t.cLS += k;
+        const uint32_t kt = t.fast ? min(k, a0.test_slots()) : k;
+        if (t.anyMode) {
+            bool any = false;
+            for (uint32_t s0 = 0; s0 == 0 || s0 < kt; s0 += LEAF_CHUNK) {
+                LeafChunk a = a0;
+                if (s0) a.load(T, s0);
+#pragma unroll
+                for (int j = 0; j < LEAF_CHUNK; ++j) {
+                    float tt, b1, b2;
+                    if (mt(a.v(0, j), a.v(1, j), a.v(2, j), a.v(3, j), a.v(4, j), a.v(5, j), a.v(6, j), a.v(7, j),
+                           a.v(8, j), t.ray, tt, b1, b2))
+                        any = true;
+                }
+            }
+            if (any) {
+                t.occl = true;
+                t.sp = 0;
+                t.hasCur = false;
+            }
+        } else {
+            // gang-G tie rule of triIntersect, followed slot by slot (see
+            // trav_step): the largest kept slot at the minimum wins
+            float bt = 1000000.0f;
+            uint32_t hm = 0;
+            int win = -1;
+            float wb1 = 0.0f, wb2 = 0.0f;
+            const uint32_t inst = 0xFFFFu / ((1u << sc.G) - 1u);
+            for (uint32_t s0 = 0; s0 == 0 || s0 < kt; s0 += LEAF_CHUNK) {
+                LeafChunk a = a0;
+                if (s0) a.load(T, s0);
+#pragma unroll
+                for (int j = 0; j < LEAF_CHUNK; ++j) {
+                    float tt, b1, b2;
+                    const bool h = mt(a.v(0, j), a.v(1, j), a.v(2, j), a.v(3, j), a.v(4, j), a.v(5, j), a.v(6, j),
+                                      a.v(7, j), a.v(8, j), t.ray, tt, b1, b2);
+                    const uint32_t s = s0 + (uint32_t)j;
+                    const bool less = h && tt < bt;
+                    const bool same = h && tt == bt && hm != 0;
+                    const bool take = less || (same && (hm & (inst << (s & (sc.G - 1u)))) == 0);
+                    win = take ? (int)s : win;
+                    wb1 = take ? b1 : wb1;
+                    wb2 = take ? b2 : wb2;
+                    bt = less ? tt : bt;
+                    hm = less ? (1u << s) : (same ? (hm | (1u << s)) : hm);
+                }
+            }
+            if (hm) {
+                t.ray.tmax = bt;
+                t.hitEntry = e & ~kEntryTent;
+                t.hitSlot = win;
+                t.b1 = wb1;
+                t.b2 = wb2;
+            }
+        }
+        return;
+    }
+    const uint32_t* rec = sc.nodes + (size_t)idx * qrecord_words(M);
+    const float4 ob0 = ld4(reinterpret_cast<const float*>(rec)), ob1 = ld4(reinterpret_cast<const float*>(rec) + 4);
+    const NodeHeader hd = qload_header(rec, M, t.axis);
+    QPlanes<MT> pl;
+    pl.load(rec);
+    const bool ownHit = own_box_hit(ob0, ob1, t.ray, tp, t.fast);
+    const bool vis = !tent || ownHit;
+    const uint32_t c = hd.c;
+    const uint32_t realMask = (1u << c) - 1u;
+    const uint32_t padMask = ((1u << k) - 1u) & ~realMask;
+    // padded slots: the zero box
+    bool zhit = false;
+    if (padMask) {
+        float z;
+        zhit = t.fast ? slab_fast(0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, t.ray, z)
+                      : slab(0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, t.ray, z);
+    }
+    // real slots: the quantized test
+    uint32_t qh = 0;
+    {
+        const uint32_t ex = __float_as_uint(ob1.z);
+        const float F[3] = {ob0.x, ob0.y, ob0.z};
+        const float o[3] = {t.ray.ox, t.ray.oy, t.ray.oz};
+        const float inv[3] = {t.ray.ix, t.ray.iy, t.ray.iz};
+        float qa[3], bn[3], bf[3];
+        bool pos[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float sa = __builtin_ldexpf(1.0f, (int)__builtin_amdgcn_ubfe(ex, 8 * a, 8) - kQExpBias);
+            qa[a] = sa * inv[a];
+            const float bb = (F[a] - o[a]) * inv[a];
+            const float kk = (fabsf(F[a]) + fabsf(o[a])) + 256.0f * sa;
+            const float m = __builtin_fmaf(kk * fabsf(inv[a]), 0x1p-21f, 0x1p-120f);
+            bn[a] = bb - m;
+            bf[a] = bb + m;
+            pos[a] = inv[a] >= 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < (MT + 3) / 4; ++j) {
+            if ((uint32_t)(4 * j) >= c) break;
+            uint32_t nd[3], fd[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const uint32_t lo = pl.dword(a, j, 0), hi = pl.dword(a, j, 1);
+                nd[a] = pos[a] ? lo : hi;
+                fd[a] = pos[a] ? hi : lo;
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                float tn[3], tf[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    tn[a] = __builtin_fmaf((float)((nd[a] >> (8 * b)) & 0xFFu), qa[a], bn[a]);
+                    tf[a] = __builtin_fmaf((float)((fd[a] >> (8 * b)) & 0xFFu), qa[a], bf[a]);
+                }
+                const float t0 = fmaxf(fmaxf(tn[0], tn[1]), fmaxf(tn[2], 0.0f));
+                const float t1 = fminf(fminf(tf[0], tf[1]), fminf(tf[2], t.ray.tmax));
+                qh |= (t0 <= t1 ? 1u : 0u) << (4 * j + b);
+            }
+        }
+    }
+    const uint32_t H = ((t.fast ? qh : ~0u) & realMask) | (zhit ? padMask : 0u);
+    t.cN += vis ? 1u : 0u;
+    t.cNS += vis ? k : 0u;
+    if (!vis || H == 0) return;
+    // real slots pushed because of this node's test are tentative; slots
+    // whose aabbDistances entry was stale are pushed for certain
+    const uint32_t tentMask = realMask & ~t.live;
+    t.live |= H;
+    // every entry carries the tmax of its push; only tentative ones use it
+    const float tnow = t.ray.tmax;
+    const uint32_t innerBase = hd.innerBase, leafBase = hd.leafBase;
+    const uint32_t leafMask = hd.leafMask;
+    const uint32_t kcodes = hd.kcodes;
+    const uint64_t row = hd.row;
+    const uint32_t innerMask = ~leafMask;
+    // entry of real slot s, tentative unless its slot was stale
+    auto entry_real = [&](uint32_t s) -> uint32_t {
+        const bool isLeaf = __builtin_amdgcn_ubfe(leafMask, s, 1) != 0;
+        const uint32_t m = isLeaf ? leafMask : innerMask;
+        const uint32_t b = isLeaf ? (kEntryLeaf | leafBase) : innerBase;
+        const uint32_t kc = __builtin_amdgcn_ubfe(kcodes, 2 * s, 2);
+        const uint32_t tb = __builtin_amdgcn_ubfe(tentMask, s, 1) << 27;
+        return (kc << kEntryKShift) | tb | (b + (uint32_t)__popc(__builtin_amdgcn_ubfe(m, 0, s)));
+    };
+    static_assert(kEntryTent == 1u << 27, "tentative bit");
+    // push(e, s): the previous pending child goes to the stack, child s
+    // becomes the pending one
+    auto push = [&](uint32_t en, uint32_t s) {
+        if (t.hasCur) {
+            const uint32_t sl = (t.sp & (S - 1)) * bd;
+            if (t.sp >= S) {
+                sk.spE[t.sp - S] = sk.stE[sl];
+                sk.spD[t.sp - S] = sk.stD[sl];
+            }
+            sk.stE[sl] = t.cur;
+            sk.stD[sl] = t.curT;
+            ++t.sp;
+        }
+        t.cur = en;
+        t.curT = tnow;
+        t.hasCur = true;
+        t.live &= ~(1u << s);
+    };
+    auto fits = [&](uint32_t npush) { return npush == 0 || t.sp + npush <= sk.cap; };
+    if (t.anyMode) {
+        // intersectSecondary: slot order, padded slots included (their
+        // FastNode id resolves through the entry table, pushed for certain)
+        const uint32_t m0 = t.live;
+        if (!fits((uint32_t)__popc(m0))) {
+            t.err = true;
+        } else {
+            for (uint32_t m = m0; m; m &= m - 1u) {
+                const uint32_t s = (uint32_t)__builtin_ctz(m);
+                uint32_t en;
+                if (s >= c) {
+                    const uint32_t id = rec[7] + s;
+                    en = id < sc.nodeCount ? sc.nodeEntry[id] : kEntryInvalid;
+                } else {
+                    en = entry_real(s);
+                }
+                push(en, s);
+            }
+        }
+    } else {
+        uint32_t live = t.live;
+        const bool pad0 = !t.rev && c < M && (live & 1u);
+        if (pad0) live &= ~1u;
+        uint32_t pm = 0;
+        {
+            const uint32_t lo = (uint32_t)row, hi = (uint32_t)(row >> 32);
+#pragma unroll
+            for (uint32_t q = 0; q < M; ++q) {
+                const uint32_t slotq = __builtin_amdgcn_ubfe(q < 8 ? lo : hi, 4 * (q & 7u), 4);
+                pm |= __builtin_amdgcn_ubfe(live, slotq, 1) << q;
+            }
+            pm &= realMask;
+        }
+        const uint32_t npush = (uint32_t)__popc(pm) + (pad0 ? 1u : 0u);
+        const uint32_t rowLo = (uint32_t)row, rowHi = (uint32_t)(row >> 32);
+        auto slot_at = [&](uint32_t q) -> uint32_t {
+            return M <= 8 ? __builtin_amdgcn_ubfe(rowLo, 4 * q, 4)
+                          : __builtin_amdgcn_ubfe(q < 8 ? rowLo : rowHi, 4 * (q & 7u), 4);
+        };
+        if (__ballot(t.sp + npush > (uint32_t)S) == 0) {
+            // every push of the wave lands in the LDS window: each entry goes
+            // straight to its stack position (see trav_step)
+            if (npush) {
+                const uint32_t last = t.sp + npush - 1u;
+                uint32_t lastE = 0;
+                if (pad0) {
+                    lastE = entry_real(0);
+                    sk.stE[t.sp * bd] = lastE;
+                    sk.stD[t.sp * bd] = tnow;
+                }
+                uint32_t pos = t.rev ? t.sp : last;
+                const uint32_t step = t.rev ? 1u : ~0u;
+                while (pm) {
+                    const uint32_t q = (uint32_t)__builtin_ctz(pm);
+                    pm &= pm - 1u;
+                    const uint32_t sq = slot_at(q);
+                    const uint32_t en = entry_real(sq);
+                    sk.stE[pos * bd] = en;
+                    sk.stD[pos * bd] = tnow;
+                    lastE = pos == last ? en : lastE;
+                    pos += step;
+                }
+                t.cur = lastE;
+                t.curT = tnow;
+                t.hasCur = true;
+                t.sp = last;
+            }
+            t.live &= ~realMask;
+        } else if (!fits(npush)) {
+            t.err = true;
+        } else {
+            if (pad0) push(entry_real(0), 0);
+            while (pm) {
+                const uint32_t q = t.rev ? (uint32_t)__builtin_ctz(pm) : 31u - (uint32_t)__builtin_clz(pm);
+                pm &= ~(1u << q);
+                const uint32_t sq = slot_at(q);
+                push(entry_real(sq), sq);
+            }
+        }
+    }
+    if (t.err) {
+        t.sp = 0;
+        t.hasCur = false;
+    }
+}
+
 // Writes the result of a finished ray.
 // TRACKED: the step kept the hit's barycentrics (trav_step<..., TRACK>);
 // otherwise they are recomputed from the hit triangle.
@@ -720,7 +1106,9 @@ struct WaveQueue {
 // Rays come from the counter `counter`, which hands out indices 0 .. n-1;
 // local index i is global ray base + i (the batched launch runs its groups
 // of scenes through one counter each).
-template <int MT, int MODE, int S, bool BATCH, bool HIST>
+// Q: the scene's node array holds the quantized records (trav_step_q;
+// closest and any hit, the tmax of each stack entry in the distance arrays).
+template <int MT, int MODE, int S, bool BATCH, bool HIST, bool Q = false>
 __device__ __forceinline__ void trace_body(const KParams& p, uint32_t* counter, uint32_t base, uint32_t n) {
     extern __shared__ uint32_t smem[];
     const uint32_t tid = threadIdx.x, bd = blockDim.x;
@@ -795,7 +1183,7 @@ __device__ __forceinline__ void trace_body(const KParams& p, uint32_t* counter, 
                     anyMode = it.mode == kAny;
                     r = rr - it.start;
                 }
-                trav_init(t, rays[r], sc.rootEntry, anyMode, sc.boxesBounded);
+                trav_init<Q>(t, rays[r], sc.rootEntry, anyMode, sc.boxesBounded);
                 if (MODE == kSaveDist)
                     for (int s = 0; s < MT; ++s) sk.aabbD[s * bd] = __builtin_nanf("");
                 has = true;
@@ -811,7 +1199,8 @@ __device__ __forceinline__ void trace_body(const KParams& p, uint32_t* counter, 
             const uint32_t nn = (uint32_t)__popcll(__ballot(!leafNext));
             if (leafNext != (16u * nl >= p.sched * nn)) continue;
         }
-        trav_step<MT, MODE, S, HIST, !HIST>(sc, sk, t);
+        if constexpr (Q) trav_step_q<MT, S>(sc, sk, t);
+        else trav_step<MT, MODE, S, HIST, !HIST>(sc, sk, t);
         if (!trav_done(t)) continue;
         trav_finish<!HIST>(sc, t, hits, occluded, counts, r, p.work + kErrWord);
         has = false;
@@ -837,11 +1226,12 @@ __device__ __forceinline__ void trace_body(const KParams& p, uint32_t* counter, 
 // the unconstrained builds at 4 waves: 100 VGPRs for the plain launch, 108
 // for the batched one, whose lanes keep their scene's pointers; the batched
 // kernel spills a little at 5 waves and still gains 7%; 6 waves spills more
-// and is slower).
-template <int MT, int MODE, int S, bool BATCH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+// and is slower).  The plain kernel over quantized records fits 80 VGPRs
+// without spilling for node memories up to 8: 6 waves per SIMD.
+template <int MT, int MODE, int S, bool BATCH, bool Q = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q && !BATCH && MT <= 8 ? 6 : 5)))
 k_trace(KParams p) {
-    trace_body<MT, MODE, S, BATCH, false>(p, p.work + kRayCounter, 0, p.n);
+    trace_body<MT, MODE, S, BATCH, false, Q>(p, p.work + kRayCounter, 0, p.n);
 }
 
 // Batched launch over scenes of different node memories.  The host orders
@@ -855,7 +1245,7 @@ k_trace(KParams p) {
 // spilled 59 words at 5 waves, a third slower).  Only the last group's end
 // is the launch's tail; a wave that reaches a drained group leaves it after
 // one atomic.
-template <int S>
+template <int S, bool Q>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 k_trace_groups(KParams p) {
     uint32_t base = 0;
@@ -863,10 +1253,10 @@ k_trace_groups(KParams p) {
         const uint32_t end = p.groupEnd[g];
         uint32_t* counter = p.work + kGroupCounter + kGroupCounterStride * g;
         switch (p.groupM[g]) {
-            case 4: trace_body<4, kMixed, S, true, false>(p, counter, base, end - base); break;
-            case 8: trace_body<8, kMixed, S, true, false>(p, counter, base, end - base); break;
-            case 12: trace_body<12, kMixed, S, true, false>(p, counter, base, end - base); break;
-            case 16: trace_body<16, kMixed, S, true, false>(p, counter, base, end - base); break;
+            case 4: trace_body<4, kMixed, S, true, false, Q>(p, counter, base, end - base); break;
+            case 8: trace_body<8, kMixed, S, true, false, Q>(p, counter, base, end - base); break;
+            case 12: trace_body<12, kMixed, S, true, false, Q>(p, counter, base, end - base); break;
+            case 16: trace_body<16, kMixed, S, true, false, Q>(p, counter, base, end - base); break;
             default: break;
         }
         base = end;
@@ -1397,9 +1787,9 @@ __global__ void k_zero(uint32_t* p, uint32_t words) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
-template <int M, int MODE, int S>
+template <int M, int MODE, int S, bool Q>
 void* kernel_ptr() {
-    return reinterpret_cast<void*>(&k_trace<M, MODE, S, false>);
+    return reinterpret_cast<void*>(&k_trace<M, MODE, S, false, Q>);
 }
 
 template <int M, int MODE>
@@ -1435,12 +1825,12 @@ void* pick_coop(uint32_t W, int mode) {
 
 size_t stream_lds(uint32_t S) { return (size_t)(10 + 11 + S + 2) * kStreamSlots * 4 + 16; }
 
-template <int M>
+template <int M, bool Q>
 void* pick_batch_s(uint32_t S) {
     switch (S) {
-        case 8: return reinterpret_cast<void*>(&k_trace<M, kMixed, 8, true>);
-        case 16: return reinterpret_cast<void*>(&k_trace<M, kMixed, 16, true>);
-        case 32: return reinterpret_cast<void*>(&k_trace<M, kMixed, 32, true>);
+        case 8: return reinterpret_cast<void*>(&k_trace<M, kMixed, 8, true, Q>);
+        case 16: return reinterpret_cast<void*>(&k_trace<M, kMixed, 16, true, Q>);
+        case 32: return reinterpret_cast<void*>(&k_trace<M, kMixed, 32, true, Q>);
         default: return nullptr;
     }
 }
@@ -1448,71 +1838,79 @@ void* pick_batch_s(uint32_t S) {
 // Batched kernel: specialised on the node memory when every scene of the
 // batch has the same M; otherwise (M = 0) the grouped kernel, which runs
 // each group of scenes with the body specialised on its node memory.
-void* pick_batch(uint32_t S, uint32_t M) {
+template <bool Q>
+void* pick_batch_q(uint32_t S, uint32_t M) {
     switch (M) {
-        case 4: return pick_batch_s<4>(S);
-        case 8: return pick_batch_s<8>(S);
-        case 12: return pick_batch_s<12>(S);
-        case 16: return pick_batch_s<16>(S);
+        case 4: return pick_batch_s<4, Q>(S);
+        case 8: return pick_batch_s<8, Q>(S);
+        case 12: return pick_batch_s<12, Q>(S);
+        case 16: return pick_batch_s<16, Q>(S);
         default:
             switch (S) {
-                case 8: return reinterpret_cast<void*>(&k_trace_groups<8>);
-                case 16: return reinterpret_cast<void*>(&k_trace_groups<16>);
-                case 32: return reinterpret_cast<void*>(&k_trace_groups<32>);
+                case 8: return reinterpret_cast<void*>(&k_trace_groups<8, Q>);
+                case 16: return reinterpret_cast<void*>(&k_trace_groups<16, Q>);
+                case 32: return reinterpret_cast<void*>(&k_trace_groups<32, Q>);
                 default: return nullptr;
             }
     }
 }
 
-template <int M, int MODE>
+void* pick_batch(uint32_t S, uint32_t M, bool q) { return q ? pick_batch_q<true>(S, M) : pick_batch_q<false>(S, M); }
+
+template <int M, int MODE, bool Q>
 void* pick_s(uint32_t S) {
     switch (S) {
-        case 8: return kernel_ptr<M, MODE, 8>();
-        case 16: return kernel_ptr<M, MODE, 16>();
-        case 32: return kernel_ptr<M, MODE, 32>();
+        case 8: return kernel_ptr<M, MODE, 8, Q>();
+        case 16: return kernel_ptr<M, MODE, 16, Q>();
+        case 32: return kernel_ptr<M, MODE, 32, Q>();
         default: return nullptr;
     }
 }
 
 template <int M>
-void* pick_mode(int mode, uint32_t S) {
+void* pick_mode(int mode, uint32_t S, bool q) {
     switch (mode) {
-        case kClosest: return pick_s<M, kClosest>(S);
-        case kAny: return pick_s<M, kAny>(S);
-        case kSaveDist: return pick_s<M, kSaveDist>(S);
+        case kClosest: return q ? pick_s<M, kClosest, true>(S) : pick_s<M, kClosest, false>(S);
+        case kAny: return q ? pick_s<M, kAny, true>(S) : pick_s<M, kAny, false>(S);
+        case kSaveDist: return q ? nullptr : pick_s<M, kSaveDist, false>(S);
         default: return nullptr;
     }
 }
 
-void* pick(uint32_t M, int mode, uint32_t S) {
-    if (mode == kMixed) return pick_batch(S, M);
+void* pick(uint32_t M, int mode, uint32_t S, bool q) {
+    if (mode == kMixed) return pick_batch(S, M, q);
     switch (M) {
-        case 4: return pick_mode<4>(mode, S);
-        case 8: return pick_mode<8>(mode, S);
-        case 12: return pick_mode<12>(mode, S);
-        case 16: return pick_mode<16>(mode, S);
+        case 4: return pick_mode<4>(mode, S, q);
+        case 8: return pick_mode<8>(mode, S, q);
+        case 12: return pick_mode<12>(mode, S, q);
+        case 16: return pick_mode<16>(mode, S, q);
         default: return nullptr;
     }
 }
 
-size_t lds_bytes(uint32_t M, int mode, uint32_t S, uint32_t bd) {
+// Stack window (entries), for the save-distance mode its distances and
+// aabbDistances, for the quantized records the tmax of each entry.
+size_t lds_bytes(uint32_t M, int mode, uint32_t S, uint32_t bd, bool q) {
     size_t words = (size_t)S * bd;
     if (mode == kSaveDist) words += (size_t)S * bd + (size_t)M * bd;
+    else if (q) words += (size_t)S * bd;
     return words * 4;
 }
 
 }  // namespace
 
 hipError_t launch_trace(uint32_t M, int mode, uint32_t S, uint32_t bd, uint32_t grid, const KParams& p,
-                        hipStream_t stream) {
-    void* fn = pick(M, mode, S);
+                        hipStream_t stream, bool quantized) {
+    void* fn = pick(M, mode, S, quantized);
     if (!fn) return hipErrorInvalidValue;
     KParams copy = p;
     void* args[] = {&copy};
-    return hipLaunchKernel(fn, dim3(grid), dim3(bd), args, lds_bytes(M, mode, S, bd), stream);
+    return hipLaunchKernel(fn, dim3(grid), dim3(bd), args, lds_bytes(M, mode, S, bd, quantized), stream);
 }
 
-size_t trace_lds_bytes(uint32_t M, int mode, uint32_t S, uint32_t bd) { return lds_bytes(M, mode, S, bd); }
+size_t trace_lds_bytes(uint32_t M, int mode, uint32_t S, uint32_t bd, bool quantized) {
+    return lds_bytes(M, mode, S, bd, quantized);
+}
 
 namespace {
 
@@ -1539,7 +1937,7 @@ void* pick_hist(uint32_t M, int mode) {
 }  // namespace
 
 size_t trace_hist_lds_bytes(uint32_t M, int mode, uint32_t bd) {
-    return lds_bytes(M, mode, kHistWindow, bd) + (size_t)kHistWindow * bd * 4 + (size_t)kHwWords * 4;
+    return lds_bytes(M, mode, kHistWindow, bd, false) + (size_t)kHistWindow * bd * 4 + (size_t)kHwWords * 4;
 }
 
 uint32_t trace_hist_block(uint32_t M, int mode) {
@@ -1613,12 +2011,12 @@ hipError_t launch_trace_hist(uint32_t M, int mode, uint32_t grid, const KParams&
     return hipLaunchKernel(fn, dim3(grid), dim3(bd), args, trace_hist_lds_bytes(M, mode, bd), stream);
 }
 
-int trace_occupancy(uint32_t M, int mode, uint32_t S, uint32_t bd) {
-    void* fn = pick(M, mode, S);
+int trace_occupancy(uint32_t M, int mode, uint32_t S, uint32_t bd, bool quantized) {
+    void* fn = pick(M, mode, S, quantized);
     if (!fn) return 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(fn), (int)bd,
-                                                     lds_bytes(M, mode, S, bd)) != hipSuccess)
+                                                     lds_bytes(M, mode, S, bd, quantized)) != hipSuccess)
         return 0;
     return blocks;
 }

@@ -183,12 +183,15 @@ class Scene:
         """Wave schedule of the per-ray kernel (rt_set_schedule)."""
         check(lib().rt_set_schedule(self._h, int(leaf_ratio16)))
 
-    def intersect(self, rays, mode="closest", counts=False, stream=False, coop=False):
+    def intersect(self, rays, mode="closest", counts=False, stream=False, coop=False, exact_boxes=False):
         """Host-array form: rays (n,) RAY_DTYPE -> hits (n,) HIT_DTYPE, or
-        occluded (n,) uint8 for mode='any'; plus counts when asked."""
+        occluded (n,) uint8 for mode='any'; plus counts when asked.
+        exact_boxes: the exact-box node records (RT_EXACT_BOXES)."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
         n = len(rays)
         flags = MODES[mode] | (_lib.RT_STREAM if stream else 0) | (_lib.RT_COOP if coop else 0)
+        if exact_boxes:
+            flags |= _lib.RT_EXACT_BOXES
         out = np.empty(n, np.uint8) if mode == "any" else np.empty(n, HIT_DTYPE)
         cnt = np.zeros(n, COUNTS_DTYPE) if counts else None
         if counts:
@@ -197,11 +200,13 @@ class Scene:
         return (out, cnt) if counts else out
 
     def intersect_device(self, rays, out, mode="closest", counts=None, stream=None, sync=True,
-                         stream_sched=False, coop=False):
+                         stream_sched=False, coop=False, exact_boxes=False):
         """Device-buffer form: rays/out/counts are torch CUDA tensors (or
         device addresses); stream is a torch.cuda.Stream or None."""
         n = rays.shape[0] if hasattr(rays, "shape") else int(rays[1])
         flags = MODES[mode] | _lib.RT_DEVICE_PTRS
+        if exact_boxes:
+            flags |= _lib.RT_EXACT_BOXES
         if counts is not None:
             flags |= _lib.RT_COUNTS
         if not sync:
@@ -315,7 +320,7 @@ class Batch:
     def set_launch(self, block_size=0, stack_window=0):
         check(lib().rt_batch_set_launch(self._h, int(block_size), int(stack_window)))
 
-    def trace(self, items, graph=True, stream=None, sync=True):
+    def trace(self, items, graph=True, stream=None, sync=True, exact_boxes=False):
         arr = (RtBatchItem * len(items))()
         for i, it in enumerate(items):
             scene, rays, out, mode = it[:4]
@@ -323,6 +328,8 @@ class Batch:
             arr[i] = RtBatchItem(scene._h, ptr(rays), ptr(out), ptr(counts) if counts is not None else None,
                                  int(rays.shape[0]), MODES[mode])
         flags = (_lib.RT_GRAPH if graph else 0) | (0 if sync else _lib.RT_NO_SYNC)
+        if exact_boxes:
+            flags |= _lib.RT_EXACT_BOXES
         s = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
         check(lib().rt_batch_trace(self._h, arr, len(items), flags, s))
 

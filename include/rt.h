@@ -121,6 +121,10 @@ typedef struct rt_scene* rt_scene_t;
 #define RT_DEVICE_PTRS 16u /* rays/out/counts are device pointers                 */
 #define RT_NO_SYNC 32u     /* with RT_DEVICE_PTRS: return without waiting         */
 #define RT_COOP 64u        /* wavefront-cooperative child tests (lanes = slots)   */
+#define RT_EXACT_BOXES 256u /* per-ray closest / any hit: test every child on its
+                               exact box at the parent (the exact-box node
+                               records), not on the quantized records; results
+                               are the same either way                            */
 
 /* ---- scene lifetime ----------------------------------------------------------- */
 
@@ -188,6 +192,12 @@ typedef struct rt_stats_t {
                                  or a padded child id past the node array; the
                                  counters accumulate on the device and rt_stats
                                  waits for launches still in flight to read them */
+    uint32_t quantized_record_bytes; /* inner record of the quantized node format
+                                 the per-ray closest / any-hit kernels use; 0 when
+                                 the scene does not qualify (box coordinates
+                                 beyond +-2^60, a child box outside its parent's,
+                                 padded slots that are not zero boxes) and every
+                                 kernel tests the exact boxes                     */
 } rt_stats_t;
 
 int rt_stats(rt_scene_t s, rt_stats_t* out);

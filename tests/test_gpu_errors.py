@@ -50,7 +50,7 @@ def test_trace_argument_errors(rt, small):
         ((rays, n, _lib.RT_COOP | _lib.RT_SAVE_DIST, hits, None), -4, b"cooperative"),
         ((rays, n, _lib.RT_COOP | _lib.RT_STREAM, hits, None), -4, b"cooperative"),
         ((rays, n, _lib.RT_STREAM | _lib.RT_SAVE_DIST, hits, None), -4, b"save-distance"),
-        ((rays, n, 256, hits, None), -1, b"unknown flag bits"),
+        ((rays, n, 512, hits, None), -1, b"unknown flag bits"),
         ((rays, n, _lib.RT_GRAPH, hits, None), -1, b"unknown flag bits"),
         ((rays, 0x80000000, _lib.RT_CLOSEST, hits, None), -1, b"2^31-1"),
     ]
