@@ -29,33 +29,34 @@ HBM_PEAK_GBPS = 8000.0
 # each scaled by what the same counter reads for a plain 1 GiB device copy
 # (bench.py --calibrate-copy): bytes = FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0,
 # in KiB.  Keyed by (branching, leaf size, rays); the commands and the
-# per-dispatch values are in profiles/k_trace_N8L8_diffuse8M_hbm_traffic.md.
+# per-dispatch values are in profiles/k_trace_quantized_N8L8_diffuse8M_counters.md
+# and profiles/k_trace_quantized_batched_64scenes.md (the calibration in
+# profiles/k_trace_exact_N8L8_diffuse8M_hbm_traffic.md).
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
-MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 6979199, "write": 402846},
-                        # batched leg (profiles/k_trace_batched_64scenes_hbm_traffic.md)
-                        ("batch", 8, 8, 9_872_996): {"fetch": 2632015, "write": 412558}}
+MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 6777475, "write": 742866},
+                        ("batch", 8, 8, 9_872_996): {"fetch": 2292753, "write": 502988}}
 
-# What binds the timed kernel, from rocprofv3 counter runs of this benchmark
-# (profiles/k_trace_N8L8_diffuse8M_l2_valu_occupancy.md and
-# profiles/k_trace_N8L8_diffuse8M_memory_pipeline.md): the texture data and
-# address units that serve the per-lane loads are 97% and 89% busy, the VALU
-# pipes 84% (43.9% of the lanes active); HBM carries a quarter of its peak,
-# and the L1s request 47.6 GB per step from L2 (7.437e8 64-byte reads), 84%
-# of it hitting there.
-MEASURED_COUNTERS = {(8, 8, 8_000_000): {"td_busy": 0.966, "ta_busy": 0.891, "valu_busy": 0.842,
-                                         "active_lanes": 0.439, "l2_hit": 0.843,
-                                         "l2_read_bytes": 47_598_500_000,
-                                         "valu_insts_per_ray": 495.2, "loads_per_ray": 15.5,
-                                         "source": "profiles/k_trace_N8L8_diffuse8M_memory_pipeline.md, "
-                                                   "profiles/k_trace_N8L8_diffuse8M_l2_valu_occupancy.md"}}
+# What the timed kernel spends, from rocprofv3 counter runs of this benchmark
+# (profiles/k_trace_quantized_N8L8_diffuse8M_counters.md): 573 VALU and 12.5
+# load instructions per ray at 41% active lanes and 6 waves per SIMD; the
+# texture address unit 70% busy and the data unit 95% (55% of it stalled on
+# the L1), 3.44e8 L1-to-L2 requests (22 GB) per step, 68% L2 hits, and HBM
+# carrying about a quarter of its peak.
+MEASURED_COUNTERS = {(8, 8, 8_000_000): {"td_busy": 0.951, "td_stalled": 0.550, "ta_busy": 0.704,
+                                         "valu_busy_metric": 1.154, "active_lanes": 0.414, "l2_hit": 0.678,
+                                         "l2_read_bytes": 22_015_520_000,
+                                         "valu_insts_per_ray": 573.2, "loads_per_ray": 12.46,
+                                         "source": "profiles/k_trace_quantized_N8L8_diffuse8M_counters.md"}}
 
 
 def limiter_text(c):
     """What binds the kernel, from the counter runs."""
     if not c or "td_busy" not in c:
         return None
-    return (f"per-lane vector-memory loads: TA {c['ta_busy']:.0%} and TD {c['td_busy']:.0%} busy, "
-            f"VALU {c['valu_busy']:.0%}; HBM carries a fraction of its peak (hbm_frac)")
+    return (f"VALU issue ({c['valu_insts_per_ray']:.0f} instructions per ray at {c['active_lanes']:.0%} active lanes; "
+            f"the VALUBusy metric reads {c['valu_busy_metric']:.0%}) and the latency of each node's dependent "
+            f"loads (TD {c['td_busy']:.0%} busy, {c['td_stalled']:.0%} stalled on the L1; TA {c['ta_busy']:.0%}); "
+            f"HBM carries a fraction of its peak (hbm_frac)")
 
 
 def measured_traffic(key):
@@ -82,6 +83,24 @@ def algorithmic_bytes(counts, M):
     return (64 * len(c) + 24 * c["node_slots"].astype(np.float64).sum()
             + header_bytes(M) * c["nodes"].astype(np.float64).sum()
             + 36 * c["leaf_slots"].astype(np.float64).sum())
+
+
+def quantized_plane_bytes(M):
+    # bytes of quantized child planes a quantized inner record visit loads
+    # (rt_layout.h): 2, 3 or 6 sixteen-byte blocks
+    return 32 if M == 4 else (48 if M == 8 else 96)
+
+
+def layout_bytes(counts, M):
+    """Bytes the quantized node records move: ray in and hit out (64 B), per
+    inner visit the own box and frame (32 B), the header block (16 B, 48 B
+    for M > 8) and the quantized planes; per leaf visit the 32-byte head and
+    40 B per slot (a pair of triangles is one 80-byte chunk).  Tentative pops
+    whose exact box test fails (2-3% of the pops on these rays) are not
+    counted."""
+    c = counts
+    return (64 * len(c) + (32 + header_bytes(M) + quantized_plane_bytes(M)) * c["nodes"].astype(np.float64).sum()
+            + 32 * c["leaves"].astype(np.float64).sum() + 40 * c["leaf_slots"].astype(np.float64).sum())
 
 
 def useful_bytes(n, hist, M):
@@ -162,8 +181,9 @@ def bench_batched(rt, torch, args, log):
     # algorithmic bytes of the same launch from a counts pass
     cnts = [torch.empty((len(x[3]), 4), dtype=torch.int32, device="cuda") for x in scenes]
     batch.trace([(x[2], x[4], o, "closest", c) for x, o, c in zip(scenes, outs, cnts)], graph=True, stream=stream)
-    alg = sum(algorithmic_bytes(c.cpu().numpy().view(rt.COUNTS_DTYPE).reshape(-1), x[1].node_memory)
-              for x, c in zip(scenes, cnts))
+    host_cnts = [c.cpu().numpy().view(rt.COUNTS_DTYPE).reshape(-1) for c in cnts]
+    alg = sum(algorithmic_bytes(c, x[1].node_memory) for x, c in zip(scenes, host_cnts))
+    lay = sum(layout_bytes(c, x[1].node_memory) for x, c in zip(scenes, host_cnts))
     achieved = alg / (k_ms / 1000.0) / 1e9
     traffic = measured_traffic(("batch", N, L, total))
     log(f"batched {res['batched']:.1f} Mrays/s (kernel {k_ms:.3f} ms, {achieved:.0f} GB/s algorithmic) vs "
@@ -178,7 +198,8 @@ def bench_batched(rt, torch, args, log):
            "kernel_ms": round(k_ms, 4), "kernel_mrays": round(total / k_ms / 1e3, 2),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                        "hbm_frac": hbm_frac(traffic, k_ms), "algorithmic_bytes": int(alg)}}
+                        "hbm_frac": hbm_frac(traffic, k_ms), "algorithmic_bytes": int(alg),
+                        "layout_bytes": int(lay), "layout_frac": round(lay / (k_ms / 1000.0) / 1e9 / HBM_PEAK_GBPS, 4)}}
     if not args.no_batch_autotune:
         out["autotuned"] = batched_autotuned(rt, torch, scenes, outs, stream, total, args, log)
     if not args.no_cpu:
@@ -206,29 +227,32 @@ BATCH_SHAPES = ((128, 8), (128, 16), (128, 32), (256, 8), (256, 16), (256, 32))
 
 def batched_autotuned(rt, torch, scenes, ref_outs, stream, total, args, log):
     """The same 64 scenes with N and L chosen per scene by the autotuner
-    (candidates N4L4, N4L8, N8L4, N8L8, each with block size 128/256 and LDS
-    stack window 8/16/32, timed on 8192 of the scene's own rays repeated to
+    (candidates N4L4, N4L8, N8L4, N8L8, each with both node formats, block
+    size 128/256 and LDS stack window 8/16, timed on 8192 of the scene's own
+    rays repeated to
     2^20 per launch, so the GPU is full), traced in one batched graph launch
     that mixes the chosen node and leaf memories.  Block size and stack
     window are properties of a launch (its blocks and its LDS allocation),
-    so the batch takes one shape: the fastest of BATCH_SHAPES for the whole
-    launch.  The tuner's last step compares that batch with N8L8 for every
-    scene, at each shape, and keeps the faster (per-scene timings alone do
-    not see how the scenes share the GPU in one launch).  The per-scene
+    and so is the node format (one kernel per launch), so the batch takes one
+    format and shape: the fastest of the quantized and exact-box records at
+    BATCH_SHAPES for the whole launch.  The tuner's last step compares that
+    batch with N8L8 for every scene, at each format and shape, and keeps the
+    faster (per-scene timings alone do not see how the scenes share the GPU
+    in one launch).  The per-scene
     shape choices are reported with what they gain over the default shape
     on the scene alone.  Tuning and the rebuilds are ingest cost, outside
     the timing."""
     from bvhraytracer_amd.autotune import Autotuner
 
     t0 = time.time()
-    tuner = Autotuner(nl=((4, 4), (4, 8), (8, 4), (8, 8)), blocks=(128, 256), windows=(8, 16, 32),
+    tuner = Autotuner(nl=((4, 4), (4, 8), (8, 4), (8, 8)), blocks=(128, 256), windows=(8, 16),
                       schedules=(16,), sample=8192, reps=2, fill=1 << 20)
     tuned, choice, shapes, gains = [], {}, {}, []
     for tris, bvh0, sc0, rays, d_r, d_h in scenes:
         best = tuner.tune_scene(tris, rays)
         key = f"N{best['N']}L{best['L']}"
         choice[key] = choice.get(key, 0) + 1
-        shape = f"{best['block']}x{best['window']}"
+        shape = f"{'x' if best['exact_boxes'] else 'q'}{best['block']}x{best['window']}"
         shapes[shape] = shapes.get(shape, 0) + 1
         if best.get("default_mrays"):
             gains.append(best["mrays"] / best["default_mrays"])
@@ -244,12 +268,15 @@ def batched_autotuned(rt, torch, scenes, ref_outs, stream, total, args, log):
     reps = max(3, args.steps // 2)
 
     def by_shape_of(items):
+        """Mrays/s by node format and launch shape ("q" quantized records,
+        "x" exact-box records)."""
         out = {}
-        for bd, win in BATCH_SHAPES:
-            batch.set_launch(bd, win)
-            sec = median_wall(lambda: batch.trace(items, graph=True, stream=stream, sync=False), torch, reps,
-                              args.warmup)
-            out[f"{bd}x{win}"] = round(total / sec / 1e6, 2)
+        for ex in (False, True):
+            for bd, win in BATCH_SHAPES:
+                batch.set_launch(bd, win)
+                sec = median_wall(lambda: batch.trace(items, graph=True, stream=stream, sync=False, exact_boxes=ex),
+                                  torch, reps, args.warmup)
+                out[f"{'x' if ex else 'q'}{bd}x{win}"] = round(total / sec / 1e6, 2)
         return out
 
     by_shape = by_shape_of(tuned_items)
@@ -259,11 +286,12 @@ def batched_autotuned(rt, torch, scenes, ref_outs, stream, total, args, log):
     items = tuned_items if use_tuned else uniform_items
     shapes_of = by_shape if use_tuned else uniform_by_shape
     best_shape = max(shapes_of, key=shapes_of.get)
-    bd, win = (int(v) for v in best_shape.split("x"))
+    exact = best_shape[0] == "x"
+    bd, win = (int(v) for v in best_shape[1:].split("x"))
     batch.set_launch(bd, win)
     # the kept configuration timed again, with three times the repetitions
-    sec = median_wall(lambda: batch.trace(items, graph=True, stream=stream, sync=False), torch, 3 * reps,
-                      args.warmup)
+    sec = median_wall(lambda: batch.trace(items, graph=True, stream=stream, sync=False, exact_boxes=exact), torch,
+                      3 * reps, args.warmup)
     mrays = total / sec / 1e6
     # the hit triangles are the same whatever the tree, up to exact ties
     same_tri = sum(int((o[:, 0] == r[:, 0]).sum().item()) for (b, s, d, o), r in zip(tuned, ref_outs)) / total
@@ -550,6 +578,7 @@ def main():
     scene.intersect_device(d_rays, d_hits, counts=d_cnt, stream=stream)
     counts = d_cnt.cpu().numpy().view(rt.COUNTS_DTYPE).reshape(-1)
     alg_bytes = algorithmic_bytes(counts, bvh.node_memory)
+    q_bytes = layout_bytes(counts, bvh.node_memory)
     hits_frac = float((d_hits[:, 2] >= 0).float().mean().item())
 
     for _ in range(args.warmup):
@@ -685,6 +714,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "algorithmic_bytes": int(alg_bytes),
                      "useful_bytes": int(useful), "useful_frac": round(useful / (k_ms / 1000.0) / 1e9 / HBM_PEAK_GBPS, 4),
+                     # what the quantized records move for the same visits
+                     "layout_bytes": int(q_bytes), "layout_frac": round(q_bytes / (k_ms / 1000.0) / 1e9 / HBM_PEAK_GBPS, 4),
                      "hbm_frac": hbm_frac(traffic, k_ms),
                      "limiter": limiter_text(MEASURED_COUNTERS.get((N, L, n))),
                      "counters": MEASURED_COUNTERS.get((N, L, n)),

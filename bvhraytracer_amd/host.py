@@ -172,6 +172,9 @@ class Scene:
 
     def __init__(self, bvh):
         self.bvh = bvh
+        # node format of the per-ray closest / any-hit calls that do not name
+        # one: the quantized records (False) or the exact-box records (True)
+        self.exact_boxes = False
         h = ctypes.c_void_p()
         check(lib().rt_upload_bvh(ctypes.byref(bvh.desc), ctypes.byref(h)))
         self._h = h
@@ -183,14 +186,15 @@ class Scene:
         """Wave schedule of the per-ray kernel (rt_set_schedule)."""
         check(lib().rt_set_schedule(self._h, int(leaf_ratio16)))
 
-    def intersect(self, rays, mode="closest", counts=False, stream=False, coop=False, exact_boxes=False):
+    def intersect(self, rays, mode="closest", counts=False, stream=False, coop=False, exact_boxes=None):
         """Host-array form: rays (n,) RAY_DTYPE -> hits (n,) HIT_DTYPE, or
         occluded (n,) uint8 for mode='any'; plus counts when asked.
-        exact_boxes: the exact-box node records (RT_EXACT_BOXES)."""
+        exact_boxes: the exact-box node records (RT_EXACT_BOXES); None = the
+        scene's choice (Scene.exact_boxes)."""
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
         n = len(rays)
         flags = MODES[mode] | (_lib.RT_STREAM if stream else 0) | (_lib.RT_COOP if coop else 0)
-        if exact_boxes:
+        if self.exact_boxes if exact_boxes is None else exact_boxes:
             flags |= _lib.RT_EXACT_BOXES
         out = np.empty(n, np.uint8) if mode == "any" else np.empty(n, HIT_DTYPE)
         cnt = np.zeros(n, COUNTS_DTYPE) if counts else None
@@ -200,12 +204,12 @@ class Scene:
         return (out, cnt) if counts else out
 
     def intersect_device(self, rays, out, mode="closest", counts=None, stream=None, sync=True,
-                         stream_sched=False, coop=False, exact_boxes=False):
+                         stream_sched=False, coop=False, exact_boxes=None):
         """Device-buffer form: rays/out/counts are torch CUDA tensors (or
         device addresses); stream is a torch.cuda.Stream or None."""
         n = rays.shape[0] if hasattr(rays, "shape") else int(rays[1])
         flags = MODES[mode] | _lib.RT_DEVICE_PTRS
-        if exact_boxes:
+        if self.exact_boxes if exact_boxes is None else exact_boxes:
             flags |= _lib.RT_EXACT_BOXES
         if counts is not None:
             flags |= _lib.RT_COUNTS

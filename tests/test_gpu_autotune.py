@@ -21,6 +21,7 @@ def test_autotune_small_scene(rt, tmp_path):
     assert (best["N"], best["L"]) in ((4, 4), (8, 8))
     assert len(best["tried"]) == 2 and best["mrays"] > 0
     assert best["block"] in (128, 256) and best["window"] == 16 and best["schedule"] in (0, 16)
+    assert best["exact_boxes"] in (False, True)
     at.save()
     again = Autotuner(cache_path=cache).tune_scene(tris, rays)
     assert again["cached"] and (again["N"], again["L"], again["schedule"]) == (best["N"], best["L"], best["schedule"])
@@ -29,6 +30,7 @@ def test_autotune_small_scene(rt, tmp_path):
     scene = rt.Scene(bvh)
     scene.set_launch(best["block"], best["window"])
     scene.set_schedule(best["schedule"])
+    scene.exact_boxes = best["exact_boxes"]
     g, gc = scene.intersect(rays, counts=True)
     ref = oracle_for(bvh).trace(rays)
     assert_closest_equal(g, ref["hits"], bvh, gc, ref["counts"], "tuned")
