@@ -313,7 +313,8 @@ def batched_autotuned(rt, torch, scenes, ref_outs, stream, total, args, log):
 
 def bench_frame(rt, torch, scene, bvh, n_bounce, log, reps=3):
     """One bounce frame: 1920x1080 primaries -> closest hit -> hit surfaces ->
-    n_bounce diffuse rays -> closest hit.  device: every stage on the GPU on
+    n_bounce diffuse rays -> closest hit (camera rays on the exact-box
+    records, the bounces on the quantized ones).  device: every stage on the GPU on
     one stream (rt_camera_rays_device, rt_hit_surface_device,
     rt_spawn_diffuse_device, device-resident rt_trace); host-assisted: the
     host makes the rays and surfaces and each trace copies rays in and hits
@@ -327,7 +328,7 @@ def bench_frame(rt, torch, scene, bvh, n_bounce, log, reps=3):
     def device_frame():
         with torch.cuda.stream(stream):
             d_r = rt.camera_rays_device(pos, tgt, W, H, 16, stream=stream)
-            scene.intersect_device(d_r, d_h, stream=stream, sync=False)
+            scene.intersect_device(d_r, d_h, stream=stream, sync=False, exact_boxes=True)
             d_n, d_p = scene.surface_device(d_r, d_h, stream=stream)
             d_b = rt.spawn_diffuse_device(d_r, d_n, d_p, d_h, n_bounce, seed=3, stream=stream)
             scene.intersect_device(d_b, d_bh, stream=stream, sync=False)
@@ -335,7 +336,7 @@ def bench_frame(rt, torch, scene, bvh, n_bounce, log, reps=3):
 
     def host_frame():
         prim = rt.camera_rays(pos, tgt, W, H, 16)
-        hits = scene.intersect(prim)
+        hits = scene.intersect(prim, exact_boxes=True)
         nrm, p = bvh.surface(prim, hits)
         bounce = rt.spawn_diffuse(prim, nrm, p, hits, n_bounce, seed=3)
         return scene.intersect(bounce)
@@ -604,15 +605,22 @@ def main():
     _, hist = scene.histograms(rays)
     useful = useful_bytes(n, hist, bvh.node_memory)
 
-    # primary rays of the same scene (coherent), for reference
+    # primary rays of the same scene (coherent), for reference: with the
+    # exact-box records (the camera-ray configuration, as rt_render_ambient
+    # traces them) and with the quantized records
     d_prim = torch.from_numpy(prim.view(np.float32).reshape(-1, 10)).to(dev)
     d_phit = torch.empty((len(prim), 6), dtype=torch.int32, device=dev)
-    scene.intersect_device(d_prim, d_phit, stream=stream)
-    pms = []
-    for _ in range(3):
-        scene.intersect_device(d_prim, d_phit, stream=stream)
-        pms.append(scene.stats()["last_kernel_ms"])
-    primary_mrays = len(prim) / (float(np.median(pms)) / 1000.0) / 1e6
+
+    def primary_rate(exact):
+        scene.intersect_device(d_prim, d_phit, stream=stream, exact_boxes=exact)
+        pms = []
+        for _ in range(3):
+            scene.intersect_device(d_prim, d_phit, stream=stream, exact_boxes=exact)
+            pms.append(scene.stats()["last_kernel_ms"])
+        return len(prim) / (float(np.median(pms)) / 1000.0) / 1e6
+
+    primary_mrays = primary_rate(True)
+    primary_q_mrays = primary_rate(False)
 
     # every traversal of the boundary on the same scene (kernel time by HIP
     # events, median of 3 after a warm-up): save distance, ray stream and
@@ -709,6 +717,7 @@ def main():
                    "rays": n, "triangles": int(len(tris)), "branching": N, "leaf_size": L,
                    "hit_fraction": round(hits_frac, 4),
                    "primary_mrays_1920x1080": round(primary_mrays, 2),
+                   "primary_quantized_mrays_1920x1080": round(primary_q_mrays, 2),
                    "host_pointer_mrays": round(host_ptr_mrays, 2), "kernel_ms": round(k_ms, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,

@@ -294,7 +294,10 @@ extern "C" int rt_render_ambient(rt_scene_t s, const rt_camera* cam, uint32_t wi
     if (rc == RT_OK) step(hipEventRecord(ev[0], st), "hipEventRecord");
     if (rc == RT_OK) rc = rt_camera_rays_device(cam, width, height, wg, rays, st);
     const uint32_t dev = RT_DEVICE_PTRS | RT_NO_SYNC;
-    if (rc == RT_OK) rc = rt_trace(s, rays, n, flags | dev, hits, nullptr, st);
+    // camera rays are coherent: the exact-box records issue fewer instructions
+    // for them (the quantized records, which move fewer bytes, serve the
+    // ambient rays below)
+    if (rc == RT_OK) rc = rt_trace(s, rays, n, flags | dev | RT_EXACT_BOXES, hits, nullptr, st);
     if (rc == RT_OK) step(hipEventRecord(ev[1], st), "hipEventRecord");
     if (rc == RT_OK && samples) {
         rc = rt_hit_surface_device(s, rays, hits, n, nrm, pos, st);
