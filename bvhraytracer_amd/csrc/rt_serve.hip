@@ -401,7 +401,7 @@ extern "C" int rt_server_create(const rt_server_config* cfg, rt_server_t* out) {
     if (!sv->cfg.hbm_cap_bytes) sv->cfg.hbm_cap_bytes = 4ull << 30;
     if (!sv->cfg.slots) sv->cfg.slots = 2;
     if (!sv->cfg.max_items) sv->cfg.max_items = 512;
-    if (!sv->cfg.copy_threads) sv->cfg.copy_threads = 4;
+    if (!sv->cfg.copy_threads) sv->cfg.copy_threads = 8;
     if (sv->cfg.copy_threads > 64) sv->cfg.copy_threads = 64;
     sv->cfg.hbm_cap_bytes = align_up(sv->cfg.hbm_cap_bytes);
     (void)hipGetDevice(&sv->device);

@@ -341,7 +341,7 @@ typedef struct rt_server_config {
     uint32_t stack_window;
     uint32_t use_graph;     /* replay captured hipGraphs (1) or launch directly (0)  */
     uint32_t copy_threads;  /* host threads that copy request rays into the pinned
-                               staging pool and results out of it (0 = 4)       */
+                               staging pool and results out of it (0 = 8)       */
 } rt_server_config;
 
 typedef struct rt_server_stats {
