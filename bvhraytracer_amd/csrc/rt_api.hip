@@ -101,9 +101,7 @@ int quant_exponent(float F, float hi) {
     int eF = 0;
     (void)std::frexp(F, &eF);
     int e = std::max(-149, (F == 0.0f ? -149 : eF - 30));
-    const long double ext = (long double)hi - (long double)F;
     while ((long double)F + 255.0L * std::ldexp(1.0L, e) < (long double)hi) ++e;
-    (void)ext;
     return e;
 }
 
@@ -382,12 +380,16 @@ int convert(const rt_bvh_desc* d, std::vector<uint32_t>& nodes, std::vector<uint
     const uint32_t LS = leaf_stride(Lm);
     tris.assign((size_t)nLeafBlocks * LS, 0.0f);
     std::vector<uint32_t> blockK(nLeafBlocks, 0);
+    std::vector<uint8_t> blockSeen(nLeafBlocks, 0);
     for (uint32_t i = 0; i < n; ++i)
         if (isLeaf[i]) {
             blockK[index[i]] = kOf[i];
+            // a block shared by two leaves has no single own box for its head
+            if (blockSeen[index[i]]++) info.quantized = false;
             float* head = &tris[(size_t)index[i] * LS];
             for (uint32_t p = 0; p < 6; ++p) head[p] = own[(size_t)i * 6 + p];
         }
+    if (!info.quantized) qnodes.clear();
     leafLc.assign(nLeafBlocks, 0);
     for (uint64_t l = 0; l < nLeafBlocks; ++l) {
         float* dst = &tris[l * LS];
