@@ -1092,6 +1092,7 @@ struct rt_batch {
     hipStream_t graphStream = nullptr;
     uint32_t graphGrid = 0, graphBd = 0, graphS = 0, graphSpillCap = 0, graphM = 0;
     bool graphQ = false;
+    size_t graphTable = 0;  // LDS bytes the captured launch reserves for the staged tables
     uint32_t* graphSpill = nullptr;
     float lastMs = 0;
     uint64_t raysTraced = 0;
@@ -1129,11 +1130,11 @@ void free_batch(rt_batch* b) {
 }
 
 int capture_graph(rt_batch* b, hipStream_t st, const KParams& p, uint32_t grid, uint32_t bd, uint32_t S,
-                  uint32_t M, bool q) {
+                  uint32_t M, bool q, size_t table) {
     drop_graph(b);
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipError_t e1 = launch_reset(p.work, st);
-    hipError_t e2 = launch_trace(M, kMixed, S, bd, grid, p, st, q);
+    hipError_t e2 = launch_trace(M, kMixed, S, bd, grid, p, st, q, table);
     hipGraph_t g = nullptr;
     hipError_t e3 = hipStreamEndCapture(st, &g);
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
@@ -1165,6 +1166,7 @@ int capture_graph(rt_batch* b, hipStream_t st, const KParams& p, uint32_t grid, 
     b->graphSpillCap = p.spillCap;
     b->graphM = M;
     b->graphQ = q;
+    b->graphTable = table;
     return RT_OK;
 }
 
@@ -1313,7 +1315,17 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
             groupEnd[nGroups - 1] = off;
         }
     }
-    const int occ = std::max(1, trace_occupancy(batchM, kMixed, S, bd, batchQ));
+    // the scene and item tables go to LDS; a captured graph keeps the larger
+    // reservation it was made with, so later batches with smaller tables
+    // replay it unchanged
+    size_t table = batch_table_bytes(nScenes, used);
+    if (graph && b->exec && b->graphTable >= table) table = b->graphTable;
+    const int occ = trace_occupancy(batchM, kMixed, S, bd, batchQ, table);
+    if (occ <= 0)
+        return set_error(RT_ERR_ARG, "rt_batch_trace: the tables of " + std::to_string(nScenes) + " scenes and " +
+                                         std::to_string(used) + " items (" + std::to_string(table) +
+                                         " bytes) do not fit the kernel's LDS beside a " + std::to_string(S) +
+                                         "-entry stack window of " + std::to_string(bd) + " threads");
     const uint64_t full = (uint64_t)b->cus * (uint64_t)occ;
     const uint32_t grid = graph ? (uint32_t)full
                                 : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(full, (total + bd - 1) / bd));
@@ -1333,6 +1345,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     p.items = b->dItems;
     p.scenes = b->dScenes;
     p.nItems = used;
+    p.nScenes = nScenes;
     p.nGroups = nGroups;
     for (uint32_t g = 0; g < nGroups; ++g) {
         p.groupM[g] = groupM[g];
@@ -1343,8 +1356,8 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     if (graph) {
         if (!b->exec || b->graphStream != st || b->graphGrid != grid || b->graphBd != bd || b->graphS != S ||
             b->graphSpill != b->spill || b->graphSpillCap != spillCap || b->graphM != batchM ||
-            b->graphQ != batchQ) {
-            if ((rc = capture_graph(b, st, p, grid, bd, S, batchM, batchQ)) != RT_OK) return rc;
+            b->graphQ != batchQ || b->graphTable != table) {
+            if ((rc = capture_graph(b, st, p, grid, bd, S, batchM, batchQ, table)) != RT_OK) return rc;
         } else {
             // re-arm: same graph, new kernel arguments (ray count, item count)
             hipKernelNodeParams kp = b->traceNodeParams;
@@ -1361,7 +1374,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     } else {
         HIP_TRY(launch_reset(b->work, st), "reset kernel");
         HIP_TRY(hipEventRecord(b->ev0, st), "hipEventRecord");
-        HIP_TRY(launch_trace(batchM, kMixed, S, bd, grid, p, st, batchQ), "batched trace kernel");
+        HIP_TRY(launch_trace(batchM, kMixed, S, bd, grid, p, st, batchQ, table), "batched trace kernel");
         HIP_TRY(hipEventRecord(b->ev1, st), "hipEventRecord");
     }
     b->pending = true;

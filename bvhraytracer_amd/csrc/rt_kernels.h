@@ -93,6 +93,7 @@ struct KParams {
     const BatchItem* items;
     const BatchScene* scenes;
     uint32_t nItems;
+    uint32_t nScenes;
     // batched launch over several node memories (launch_trace with M = 0):
     // items ordered by node memory into nGroups groups, group g holding
     // global rays [groupEnd[g-1], groupEnd[g]) of node memory groupM[g]
@@ -106,11 +107,18 @@ struct KParams {
 // quantized: p.nodes (every scene's nodes, batched) are the quantized
 // records and p.spill has a second region of spillCap floats per slot
 // (closest, any and batched launches).
+// Batched launches (mode kMixed) stage their scene and item tables in LDS:
+// tableBytes >= batch_table_bytes(p.nScenes, p.nItems) more dynamic LDS.
 hipError_t launch_trace(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize,
-                        uint32_t grid, const KParams& p, hipStream_t stream, bool quantized = false);
+                        uint32_t grid, const KParams& p, hipStream_t stream, bool quantized = false,
+                        size_t tableBytes = 0);
 
 // Blocks per CU the per-ray kernel reaches with this configuration.
-int trace_occupancy(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize, bool quantized = false);
+int trace_occupancy(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize, bool quantized = false,
+                    size_t tableBytes = 0);
+
+// LDS bytes of a batched launch's staged tables.
+size_t batch_table_bytes(uint32_t nScenes, uint32_t nItems);
 
 // Dynamic LDS bytes of the per-ray kernel (stack window, and for the
 // save-distance mode the stack distances and aabbDistances, for the

@@ -1103,6 +1103,76 @@ struct WaveQueue {
 // reads the node memory from the scene; MODE = kMixed picks closest/any per
 // item).
 // Body of the per-ray kernel; HIST adds the visit statistics.
+// Batched launch tables staged in LDS after the stack arrays (batch_table_word):
+// per scene the pointers and constants a step reads, per item its rays, first
+// global ray and scene | mode << 31 (the result pointers are read from the
+// item in device memory when the ray finishes).  A lane keeps only its scene
+// and item indices; the step reads the scene's fields from LDS, so the
+// per-lane scene state of a batched launch costs two registers instead of
+// sixteen.
+struct LdsScene {
+    const uint32_t* nodes;
+    const float* tris;
+    const int32_t* triIds;
+    const uint32_t* nodeEntry;
+    uint32_t rootEntry, nodeCount, lmg, pad;  // lmg: Lm | G << 8 | boxesBounded << 16
+};
+struct LdsItem {
+    const rt_ray* rays;
+    uint32_t start, meta;  // meta: scene | (mode == kAny) << 31
+};
+static_assert(sizeof(LdsScene) == 48 && sizeof(LdsItem) == 16, "table sizes the batch limits are stated in");
+
+__host__ __device__ constexpr uint32_t batch_table_word(uint32_t S, uint32_t bd, bool q) {
+    return (q ? 2u : 1u) * S * bd;
+}
+
+// Copies the batch's scene and item tables into LDS; every thread of the
+// block calls it once, before its first ray.
+__device__ __forceinline__ void stage_batch_tables(const KParams& p, uint32_t* smem, uint32_t tableWord) {
+    LdsScene* ts = reinterpret_cast<LdsScene*>(smem + tableWord);
+    LdsItem* ti = reinterpret_cast<LdsItem*>(ts + p.nScenes);
+    for (uint32_t i = threadIdx.x; i < p.nScenes; i += blockDim.x) {
+        const BatchScene& bs = p.scenes[i];
+        LdsScene v;
+        v.nodes = bs.nodes;
+        v.tris = bs.tris;
+        v.triIds = bs.triIds;
+        v.nodeEntry = bs.nodeEntry;
+        v.rootEntry = bs.rootEntry;
+        v.nodeCount = bs.nodeCount;
+        v.lmg = bs.Lm | bs.gang << 8 | (bs.boxesBounded ? 1u : 0u) << 16;
+        v.pad = 0;
+        ts[i] = v;
+    }
+    for (uint32_t i = threadIdx.x; i < p.nItems; i += blockDim.x) {
+        const BatchItem& it = p.items[i];
+        LdsItem v;
+        v.rays = it.rays;
+        v.start = it.start;
+        v.meta = it.scene | (it.mode == kAny ? 0x80000000u : 0u);
+        ti[i] = v;
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ SceneCtx lds_scene(const LdsScene& ls, uint32_t MT) {
+    SceneCtx sc;
+    sc.nodes = ls.nodes;
+    sc.tris = ls.tris;
+    sc.triIds = ls.triIds;
+    sc.nodeEntry = ls.nodeEntry;
+    sc.leafPrims = nullptr;
+    sc.rootEntry = ls.rootEntry;
+    sc.nodeCount = ls.nodeCount;
+    sc.Lm = ls.lmg & 0xFFu;
+    sc.G = (ls.lmg >> 8) & 0xFFu;
+    sc.boxesBounded = ((ls.lmg >> 16) & 1u) != 0;
+    sc.M = MT;
+    sc.recw = record_words(MT);
+    return sc;
+}
+
 // Rays come from the counter `counter`, which hands out indices 0 .. n-1;
 // local index i is global ray base + i (the batched launch runs its groups
 // of scenes through one counter each).
@@ -1137,10 +1207,10 @@ __device__ __forceinline__ void trace_body(const KParams& p, uint32_t* counter, 
     }
 
     SceneCtx sc = plain_scene(p, MT);
-    const rt_ray* rays = p.rays;
-    rt_hit* hits = p.hits;
-    uint8_t* occluded = p.occluded;
-    rt_counts* counts = p.counts;
+    // batched launch: the staged tables, and the lane's scene and item
+    const LdsScene* tabS = reinterpret_cast<const LdsScene*>(smem + batch_table_word(S, bd, Q));
+    const LdsItem* tabI = reinterpret_cast<const LdsItem*>(tabS + p.nScenes);
+    uint32_t sidx = 0, item = 0;
 
     bool has = false;
     WaveQueue q;
@@ -1154,36 +1224,23 @@ __device__ __forceinline__ void trace_body(const KParams& p, uint32_t* counter, 
             if (!has && local < n) {
                 const uint32_t rr = base + local;
                 r = rr;
-                bool anyMode = (MODE == kAny);
                 if constexpr (BATCH) {
                     // item of global ray rr: last item with start <= rr
                     uint32_t lo = 0, hi = p.nItems - 1;
                     while (lo < hi) {
                         const uint32_t mid = (lo + hi + 1) >> 1;
-                        if (p.items[mid].start <= rr) lo = mid;
+                        if (tabI[mid].start <= rr) lo = mid;
                         else hi = mid - 1;
                     }
-                    const BatchItem& it = p.items[lo];
-                    const BatchScene& bs = p.scenes[it.scene];
-                    sc.nodes = bs.nodes;
-                    sc.tris = bs.tris;
-                    sc.triIds = bs.triIds;
-                    sc.nodeEntry = bs.nodeEntry;
-                    sc.rootEntry = bs.rootEntry;
-                    sc.G = bs.gang;
-                    sc.Lm = bs.Lm;
-                    sc.nodeCount = bs.nodeCount;
-                    sc.boxesBounded = bs.boxesBounded != 0;
-                    sc.M = bs.M;
-                    sc.recw = record_words(bs.M);
-                    rays = it.rays;
-                    hits = static_cast<rt_hit*>(it.out);
-                    occluded = static_cast<uint8_t*>(it.out);
-                    counts = it.counts;
-                    anyMode = it.mode == kAny;
+                    const LdsItem it = tabI[lo];
+                    item = lo;
+                    sidx = it.meta & 0x7FFFFFFFu;
                     r = rr - it.start;
+                    const SceneCtx scs = lds_scene(tabS[sidx], MT);
+                    trav_init<Q>(t, it.rays[r], scs.rootEntry, (it.meta >> 31) != 0, scs.boxesBounded);
+                } else {
+                    trav_init<Q>(t, p.rays[r], sc.rootEntry, MODE == kAny, sc.boxesBounded);
                 }
-                trav_init<Q>(t, rays[r], sc.rootEntry, anyMode, sc.boxesBounded);
                 if (MODE == kSaveDist)
                     for (int s = 0; s < MT; ++s) sk.aabbD[s * bd] = __builtin_nanf("");
                 has = true;
@@ -1199,10 +1256,20 @@ __device__ __forceinline__ void trace_body(const KParams& p, uint32_t* counter, 
             const uint32_t nn = (uint32_t)__popcll(__ballot(!leafNext));
             if (leafNext != (16u * nl >= p.sched * nn)) continue;
         }
-        if constexpr (Q) trav_step_q<MT, S>(sc, sk, t);
-        else trav_step<MT, MODE, S, HIST, !HIST>(sc, sk, t);
-        if (!trav_done(t)) continue;
-        trav_finish<!HIST>(sc, t, hits, occluded, counts, r, p.work + kErrWord);
+        if constexpr (BATCH) {
+            const SceneCtx scs = lds_scene(tabS[sidx], MT);
+            if constexpr (Q) trav_step_q<MT, S>(scs, sk, t);
+            else trav_step<MT, MODE, S, HIST, !HIST>(scs, sk, t);
+            if (!trav_done(t)) continue;
+            const BatchItem& it = p.items[item];
+            trav_finish<!HIST>(scs, t, static_cast<rt_hit*>(it.out), static_cast<uint8_t*>(it.out), it.counts, r,
+                               p.work + kErrWord);
+        } else {
+            if constexpr (Q) trav_step_q<MT, S>(sc, sk, t);
+            else trav_step<MT, MODE, S, HIST, !HIST>(sc, sk, t);
+            if (!trav_done(t)) continue;
+            trav_finish<!HIST>(sc, t, p.hits, p.occluded, p.counts, r, p.work + kErrWord);
+        }
         has = false;
     }
     if constexpr (HIST) {
@@ -1231,6 +1298,10 @@ __device__ __forceinline__ void trace_body(const KParams& p, uint32_t* counter, 
 template <int MT, int MODE, int S, bool BATCH, bool Q = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q && !BATCH && MT <= 8 ? 6 : 5)))
 k_trace(KParams p) {
+    if constexpr (BATCH) {
+        extern __shared__ uint32_t smem[];
+        stage_batch_tables(p, smem, batch_table_word(S, blockDim.x, Q));
+    }
     trace_body<MT, MODE, S, BATCH, false, Q>(p, p.work + kRayCounter, 0, p.n);
 }
 
@@ -1248,6 +1319,8 @@ k_trace(KParams p) {
 template <int S, bool Q>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 k_trace_groups(KParams p) {
+    extern __shared__ uint32_t smem[];
+    stage_batch_tables(p, smem, batch_table_word(S, blockDim.x, Q));
     uint32_t base = 0;
     for (uint32_t g = 0; g < p.nGroups; ++g) {
         const uint32_t end = p.groupEnd[g];
@@ -1900,16 +1973,22 @@ size_t lds_bytes(uint32_t M, int mode, uint32_t S, uint32_t bd, bool q) {
 }  // namespace
 
 hipError_t launch_trace(uint32_t M, int mode, uint32_t S, uint32_t bd, uint32_t grid, const KParams& p,
-                        hipStream_t stream, bool quantized) {
+                        hipStream_t stream, bool quantized, size_t tableBytes) {
     void* fn = pick(M, mode, S, quantized);
     if (!fn) return hipErrorInvalidValue;
+    if (mode == kMixed && tableBytes < batch_table_bytes(p.nScenes, p.nItems)) return hipErrorInvalidValue;
     KParams copy = p;
     void* args[] = {&copy};
-    return hipLaunchKernel(fn, dim3(grid), dim3(bd), args, lds_bytes(M, mode, S, bd, quantized), stream);
+    return hipLaunchKernel(fn, dim3(grid), dim3(bd), args, lds_bytes(M, mode, S, bd, quantized) + tableBytes,
+                           stream);
 }
 
 size_t trace_lds_bytes(uint32_t M, int mode, uint32_t S, uint32_t bd, bool quantized) {
     return lds_bytes(M, mode, S, bd, quantized);
+}
+
+size_t batch_table_bytes(uint32_t nScenes, uint32_t nItems) {
+    return ((size_t)nScenes * sizeof(LdsScene) + (size_t)nItems * sizeof(LdsItem) + 255) / 256 * 256;
 }
 
 namespace {
@@ -2011,12 +2090,12 @@ hipError_t launch_trace_hist(uint32_t M, int mode, uint32_t grid, const KParams&
     return hipLaunchKernel(fn, dim3(grid), dim3(bd), args, trace_hist_lds_bytes(M, mode, bd), stream);
 }
 
-int trace_occupancy(uint32_t M, int mode, uint32_t S, uint32_t bd, bool quantized) {
+int trace_occupancy(uint32_t M, int mode, uint32_t S, uint32_t bd, bool quantized, size_t tableBytes) {
     void* fn = pick(M, mode, S, quantized);
     if (!fn) return 0;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, reinterpret_cast<const void*>(fn), (int)bd,
-                                                     lds_bytes(M, mode, S, bd, quantized)) != hipSuccess)
+                                                     lds_bytes(M, mode, S, bd, quantized) + tableBytes) != hipSuccess)
         return 0;
     return blocks;
 }

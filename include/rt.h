@@ -314,7 +314,10 @@ int rt_batch_create(uint32_t max_items, rt_batch_t* out);
 /* Items must all live on the batch's device; total rays < 2^31.  A NULL
  * hip_stream keeps the null stream's order, as for rt_trace.  flags takes
  * RT_GRAPH and RT_NO_SYNC (other bits: RT_ERR_ARG); items with n == 0 are
- * skipped. */
+ * skipped.  The kernel stages the batch's tables in LDS beside its stack
+ * window (48 bytes per distinct scene, 16 per item, rounded up to 256): a
+ * batch whose tables do not fit with the launch shape returns RT_ERR_ARG
+ * (thousands of items fit; larger tables lower the kernel's occupancy). */
 int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t n_items, uint32_t flags,
                    void* hip_stream);
 int rt_batch_set_launch(rt_batch_t b, uint32_t block_size, uint32_t stack_window);

@@ -116,3 +116,52 @@ def test_batch_uniform_node_memory(rt, scenes):
             cc = c.cpu().numpy().view(rt.COUNTS_DTYPE).reshape(-1)
             assert np.array_equal(cc["nodes"], rc["nodes"]) and np.array_equal(cc["leaves"], rc["leaves"])
     batch.close()
+
+
+def test_batch_many_small_items(rt, scenes):
+    """3000 items of a few rays over five scenes (the kernel stages the item
+    table in LDS, many rows per thread): every ray equals the plain launch,
+    direct and replayed, and a later batch of 50 items replays the graph
+    captured with the larger table.  A batch whose tables cannot fit the
+    kernel's LDS is refused with RT_ERR_ARG and leaves the batch usable."""
+    import torch
+    refs, items, spans = [], [], []
+    for i, (bvh, sc, rays, _) in enumerate(scenes):
+        r = dev(rt, rays)
+        ref = torch.empty((len(rays), 6), dtype=torch.int32, device="cuda")
+        sc.intersect_device(r, ref)
+        out = torch.full_like(ref, -7)
+        refs.append((ref, out))
+        cuts = np.linspace(0, len(rays), 601).astype(int)
+        items += [(sc, r[a:b], out[a:b], "closest") for a, b in zip(cuts[:-1], cuts[1:])]
+        spans += [(i, a, b) for a, b in zip(cuts[:-1], cuts[1:])]
+    few = [(it, sp) for it, sp in zip(items, spans) if sp[0] == 2][:50]
+    rng = np.random.default_rng(5)
+    items = [items[j] for j in rng.permutation(len(items))]
+    batch = rt.Batch(30_000)
+    for graph in (False, True):
+        for _, out in refs:
+            out.fill_(-7)
+        print(f"many-item batch graph={graph}: {len(items)} items", flush=True)
+        batch.trace(items, graph=graph)
+        torch.cuda.synchronize()
+        for k, (ref, out) in enumerate(refs):
+            assert torch.equal(out, ref), f"scene {k} graph={graph}"
+    refs[2][1].fill_(-7)
+    batch.trace([it for it, _ in few], graph=True)
+    torch.cuda.synchronize()
+    last = few[-1][1][2]
+    assert torch.equal(refs[2][1][:last], refs[2][0][:last]) and bool((refs[2][1][last:] == -7).all())
+    # 25000 one-ray items: 400 KB of item table, more than a CU's LDS
+    sc, r = scenes[1][1], dev(rt, scenes[1][2])
+    one = torch.empty((1, 6), dtype=torch.int32, device="cuda")
+    huge = [(sc, r[j % len(r):j % len(r) + 1], one, "closest") for j in range(25_000)]
+    with pytest.raises(rt.RtError) as e:
+        batch.trace(huge, graph=True)
+    assert e.value.code == -1 and "LDS" in str(e.value)
+    for _, out in refs:
+        out.fill_(-7)
+    batch.trace(items, graph=True)
+    torch.cuda.synchronize()
+    assert all(torch.equal(out, ref) for ref, out in refs)
+    batch.close()
