@@ -1134,7 +1134,7 @@ int capture_graph(rt_batch* b, hipStream_t st, const KParams& p, uint32_t grid, 
     drop_graph(b);
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipError_t e1 = launch_reset(p.work, st);
-    hipError_t e2 = launch_trace(M, kMixed, S, bd, grid, p, st, q, table);
+    hipError_t e2 = launch_trace(M, kBatch, S, bd, grid, p, st, q, table);
     hipGraph_t g = nullptr;
     hipError_t e3 = hipStreamEndCapture(st, &g);
     if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
@@ -1283,10 +1283,11 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     }
     if (used == 0) return RT_OK;
     const bool graph = (flags & RT_GRAPH) != 0;
-    // One node memory for the whole batch: the kernel specialised on it.
-    // Several: the items are ordered by node memory (stable, ascending) into
-    // groups that the grouped kernel runs one after another.  Item starts
-    // are the running ray offsets in that order.
+    // The items are ordered (stable) by node memory, then closest before
+    // any hit, into groups that the kernel runs one after another, each with
+    // the body specialised on its node memory and mode.  One node memory for
+    // the whole batch: the kernel specialised on it; several: the grouped
+    // kernel.  Item starts are the running ray offsets in that order.
     uint32_t batchM = b->hScenes[0].M;
     for (uint32_t i = 1; i < nScenes; ++i)
         if (b->hScenes[i].M != batchM) batchM = 0;
@@ -1297,18 +1298,19 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
         for (uint32_t i = 0; i < nScenes; ++i) b->hScenes[i].nodes = scenes[i]->qnodes;
     const bool qShape = batchQ && !b->launchSet;
     const uint32_t S = qShape ? kQWindow : b->stackWindow, bd = qShape ? kQBlock : b->blockSize;
-    uint32_t nGroups = 0, groupM[kMaxGroups] = {}, groupEnd[kMaxGroups] = {};
+    uint32_t nGroups = 0, groupKey[kMaxGroups] = {}, groupEnd[kMaxGroups] = {};
     {
-        std::stable_sort(b->hItems, b->hItems + used, [&](const BatchItem& x, const BatchItem& y) {
-            return b->hScenes[x.scene].M < b->hScenes[y.scene].M;
-        });
+        auto key = [&](const BatchItem& x) { return batch_group_key(b->hScenes[x.scene].M, x.mode); };
+        std::stable_sort(b->hItems, b->hItems + used,
+                         [&](const BatchItem& x, const BatchItem& y) { return key(x) < key(y); });
         uint32_t off = 0;
         for (uint32_t i = 0; i < used; ++i) {
             BatchItem& bi = b->hItems[i];
-            const uint32_t m = b->hScenes[bi.scene].M;
-            if (nGroups == 0 || groupM[nGroups - 1] != m) {
-                if (nGroups == kMaxGroups) return set_error(RT_ERR_LAYOUT, "rt_batch_trace: more than 4 node memories");
-                groupM[nGroups++] = m;
+            const uint32_t k = key(bi);
+            if (nGroups == 0 || groupKey[nGroups - 1] != k) {
+                if (nGroups == kMaxGroups)
+                    return set_error(RT_ERR_LAYOUT, "rt_batch_trace: more than 8 (node memory, mode) groups");
+                groupKey[nGroups++] = k;
             }
             bi.start = off;
             off += bi.n;
@@ -1320,7 +1322,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     // replay it unchanged
     size_t table = batch_table_bytes(nScenes, used);
     if (graph && b->exec && b->graphTable >= table) table = b->graphTable;
-    const int occ = trace_occupancy(batchM, kMixed, S, bd, batchQ, table);
+    const int occ = trace_occupancy(batchM, kBatch, S, bd, batchQ, table);
     if (occ <= 0)
         return set_error(RT_ERR_ARG, "rt_batch_trace: the tables of " + std::to_string(nScenes) + " scenes and " +
                                          std::to_string(used) + " items (" + std::to_string(table) +
@@ -1348,7 +1350,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     p.nScenes = nScenes;
     p.nGroups = nGroups;
     for (uint32_t g = 0; g < nGroups; ++g) {
-        p.groupM[g] = groupM[g];
+        p.groupKey[g] = groupKey[g];
         p.groupEnd[g] = groupEnd[g];
     }
     p.chunk = ray_chunk(total, grid, bd);
@@ -1374,7 +1376,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     } else {
         HIP_TRY(launch_reset(b->work, st), "reset kernel");
         HIP_TRY(hipEventRecord(b->ev0, st), "hipEventRecord");
-        HIP_TRY(launch_trace(batchM, kMixed, S, bd, grid, p, st, batchQ, table), "batched trace kernel");
+        HIP_TRY(launch_trace(batchM, kBatch, S, bd, grid, p, st, batchQ, table), "batched trace kernel");
         HIP_TRY(hipEventRecord(b->ev1, st), "hipEventRecord");
     }
     b->pending = true;

@@ -19,7 +19,7 @@
 
 namespace rt {
 
-enum TraceMode { kClosest = 0, kAny = 1, kSaveDist = 2, kMixed = 3 };
+enum TraceMode { kClosest = 0, kAny = 1, kSaveDist = 2, kBatch = 3 };
 
 // Work counters: the ray counter and the guarded-condition counter, each on
 // its own 128-byte line.  launch_reset zeroes the ray counter's line before
@@ -28,13 +28,16 @@ enum TraceMode { kClosest = 0, kAny = 1, kSaveDist = 2, kMixed = 3 };
 constexpr uint32_t kRayCounter = 0;
 constexpr uint32_t kErrWord = 32;  // rays stopped by a guard
 constexpr uint32_t kWorkWords = 64;
-// Batched launch grouped by node memory: group g's ray counter is word
-// kGroupCounter + kGroupCounterStride * g (inside the line launch_reset
-// zeroes; group 0 shares the plain counter's word).
-constexpr uint32_t kMaxGroups = 4;
+// Batched launch grouped by node memory and mode (closest, any hit): group
+// g's ray counter is word kGroupCounter + kGroupCounterStride * g (inside the
+// line launch_reset zeroes; group 0 shares the plain counter's word).
+constexpr uint32_t kMaxGroups = 8;
 constexpr uint32_t kGroupCounter = kRayCounter;
-constexpr uint32_t kGroupCounterStride = 8;
+constexpr uint32_t kGroupCounterStride = 4;
 static_assert(kGroupCounter + kGroupCounterStride * (kMaxGroups - 1) < kErrWord, "group counters overlap");
+
+// Group key: node memory and mode; ascending keys order the groups.
+constexpr uint32_t batch_group_key(uint32_t M, uint32_t mode) { return M << 1 | (mode == kAny ? 1u : 0u); }
 
 // Batched launch tables (device memory).  A batch is a list of items, each a
 // contiguous ray range of one scene in one mode; the global ray index space
@@ -94,20 +97,20 @@ struct KParams {
     const BatchScene* scenes;
     uint32_t nItems;
     uint32_t nScenes;
-    // batched launch over several node memories (launch_trace with M = 0):
-    // items ordered by node memory into nGroups groups, group g holding
-    // global rays [groupEnd[g-1], groupEnd[g]) of node memory groupM[g]
+    // batched launch: items ordered by node memory and mode into nGroups
+    // groups, group g holding global rays [groupEnd[g-1], groupEnd[g]) of
+    // group key groupKey[g] (batch_group_key)
     uint32_t nGroups;
-    uint32_t groupM[kMaxGroups];
+    uint32_t groupKey[kMaxGroups];
     uint32_t groupEnd[kMaxGroups];
 };
 
-// Launches the per-ray kernel for node memory M on `stream`; mode kMixed
+// Launches the per-ray kernel for node memory M on `stream`; mode kBatch
 // launches the batched kernel (M = 0: the grouped one, p.nGroups set).
 // quantized: p.nodes (every scene's nodes, batched) are the quantized
 // records and p.spill has a second region of spillCap floats per slot
 // (closest, any and batched launches).
-// Batched launches (mode kMixed) stage their scene and item tables in LDS:
+// Batched launches (mode kBatch) stage their scene and item tables in LDS:
 // tableBytes >= batch_table_bytes(p.nScenes, p.nItems) more dynamic LDS.
 hipError_t launch_trace(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSize,
                         uint32_t grid, const KParams& p, hipStream_t stream, bool quantized = false,

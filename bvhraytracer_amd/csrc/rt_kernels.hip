@@ -1097,19 +1097,16 @@ struct WaveQueue {
     __device__ __forceinline__ bool drained() const { return exhausted && next == end; }
 };
 
-// Per-ray kernel.  One template for the plain launch (one scene, uniform
-// parameters, node memory MT known at compile time) and the batched launch
-// (BATCH: every lane carries the scene and item of its current ray; MT = 0
-// reads the node memory from the scene; MODE = kMixed picks closest/any per
-// item).
-// Body of the per-ray kernel; HIST adds the visit statistics.
+// Per-ray kernel.  One template body for the plain launch (one scene,
+// uniform parameters) and the batched launch (BATCH: every lane carries the
+// scene and item of its current ray); both are specialised on the node
+// memory MT and the mode.
 // Batched launch tables staged in LDS after the stack arrays (batch_table_word):
 // per scene the pointers and constants a step reads, per item its rays, first
-// global ray and scene | mode << 31 (the result pointers are read from the
-// item in device memory when the ray finishes).  A lane keeps only its scene
-// and item indices; the step reads the scene's fields from LDS, so the
-// per-lane scene state of a batched launch costs two registers instead of
-// sixteen.
+// global ray and scene (the result pointers are read from the item in device
+// memory when the ray finishes).  A lane keeps only its scene and item
+// indices; the step reads the scene's fields from LDS, so the per-lane scene
+// state of a batched launch costs two registers instead of sixteen.
 struct LdsScene {
     const uint32_t* nodes;
     const float* tris;
@@ -1119,7 +1116,7 @@ struct LdsScene {
 };
 struct LdsItem {
     const rt_ray* rays;
-    uint32_t start, meta;  // meta: scene | (mode == kAny) << 31
+    uint32_t start, scene;
 };
 static_assert(sizeof(LdsScene) == 48 && sizeof(LdsItem) == 16, "table sizes the batch limits are stated in");
 
@@ -1150,7 +1147,7 @@ __device__ __forceinline__ void stage_batch_tables(const KParams& p, uint32_t* s
         LdsItem v;
         v.rays = it.rays;
         v.start = it.start;
-        v.meta = it.scene | (it.mode == kAny ? 0x80000000u : 0u);
+        v.scene = it.scene;
         ti[i] = v;
     }
     __syncthreads();
@@ -1173,9 +1170,10 @@ __device__ __forceinline__ SceneCtx lds_scene(const LdsScene& ls, uint32_t MT) {
     return sc;
 }
 
-// Rays come from the counter `counter`, which hands out indices 0 .. n-1;
-// local index i is global ray base + i (the batched launch runs its groups
-// of scenes through one counter each).
+// Body of the per-ray kernel.  Rays come from the counter `counter`, which
+// hands out indices 0 .. n-1; local index i is global ray base + i (the
+// batched launch runs its groups of items through one counter each).  HIST
+// adds the visit statistics.
 // Q: the scene's node array holds the quantized records (trav_step_q;
 // closest and any hit, the tmax of each stack entry in the distance arrays).
 template <int MT, int MODE, int S, bool BATCH, bool HIST, bool Q = false>
@@ -1234,10 +1232,10 @@ __device__ __forceinline__ void trace_body(const KParams& p, uint32_t* counter, 
                     }
                     const LdsItem it = tabI[lo];
                     item = lo;
-                    sidx = it.meta & 0x7FFFFFFFu;
+                    sidx = it.scene;
                     r = rr - it.start;
                     const SceneCtx scs = lds_scene(tabS[sidx], MT);
-                    trav_init<Q>(t, it.rays[r], scs.rootEntry, (it.meta >> 31) != 0, scs.boxesBounded);
+                    trav_init<Q>(t, it.rays[r], scs.rootEntry, MODE == kAny, scs.boxesBounded);
                 } else {
                     trav_init<Q>(t, p.rays[r], sc.rootEntry, MODE == kAny, sc.boxesBounded);
                 }
@@ -1291,31 +1289,50 @@ __device__ __forceinline__ void trace_body(const KParams& p, uint32_t* counter, 
 
 // Register budget: at most 96 VGPRs, 5 waves per SIMD (measured faster than
 // the unconstrained builds at 4 waves: 100 VGPRs for the plain launch, 108
-// for the batched one, whose lanes keep their scene's pointers; the batched
-// kernel spills a little at 5 waves and still gains 7%; 6 waves spills more
-// and is slower).  The plain kernel over quantized records fits 80 VGPRs
-// without spilling for node memories up to 8: 6 waves per SIMD.
-template <int MT, int MODE, int S, bool BATCH, bool Q = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q && !BATCH && MT <= 8 ? 6 : 5)))
+// for the batched one; 6 waves spills more and is slower).  The plain kernel
+// over quantized records fits 80 VGPRs without spilling for node memories up
+// to 8: 6 waves per SIMD.  The batched kernels stay at 5 waves: at 6 their
+// closest-hit body spills 8 words and runs 2% slower.
+template <int MT, int MODE, int S, bool Q = false>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(Q && MT <= 8 ? 6 : 5)))
 k_trace(KParams p) {
-    if constexpr (BATCH) {
-        extern __shared__ uint32_t smem[];
-        stage_batch_tables(p, smem, batch_table_word(S, blockDim.x, Q));
-    }
-    trace_body<MT, MODE, S, BATCH, false, Q>(p, p.work + kRayCounter, 0, p.n);
+    trace_body<MT, MODE, S, false, false, Q>(p, p.work + kRayCounter, 0, p.n);
 }
 
-// Batched launch over scenes of different node memories.  The host orders
-// the items by node memory into groups (p.groupM[g], global rays
-// [p.groupEnd[g-1], p.groupEnd[g])), each with its own ray counter.  Every
-// wave runs the groups in turn, each with the body specialised on that
-// group's node memory: it takes rays from group g until the group's counter
-// is exhausted and its own rays are finished, then moves to group g + 1.
-// Lanes of one wave therefore always run the same node memory's code (a
-// body that reads M per lane keeps every memory's registers live and
-// spilled 59 words at 5 waves, a third slower).  Only the last group's end
-// is the launch's tail; a wave that reaches a drained group leaves it after
-// one atomic.
+// Batched launch.  The host orders the items by node memory and mode into
+// groups (key p.groupKey[g], global rays [p.groupEnd[g-1], p.groupEnd[g])),
+// each with its own ray counter.  Every wave runs the groups in turn, each
+// with the body specialised on that group's node memory and mode: it takes
+// rays from group g until the group's counter is exhausted and its own rays
+// are finished, then moves to group g + 1.  Lanes of one wave therefore
+// always run one node memory's and one mode's code (a body that reads M per
+// lane keeps every memory's registers live and spilled 59 words at 5 waves,
+// a third slower; one that picks closest or any hit per lane runs a batch of
+// closest-hit items 5% slower than the closest-hit body).  Only the last
+// group's end is the launch's tail; a wave that reaches a drained group
+// leaves it after one atomic.
+template <int MT, int S, bool Q>
+__device__ __forceinline__ void batch_group(const KParams& p, uint32_t key, uint32_t* counter, uint32_t base,
+                                            uint32_t n) {
+    if (key & 1u) trace_body<MT, kAny, S, true, false, Q>(p, counter, base, n);
+    else trace_body<MT, kClosest, S, true, false, Q>(p, counter, base, n);
+}
+
+// Every scene of the batch has node memory MT.
+template <int MT, int S, bool Q>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+k_trace_batch(KParams p) {
+    extern __shared__ uint32_t smem[];
+    stage_batch_tables(p, smem, batch_table_word(S, blockDim.x, Q));
+    uint32_t base = 0;
+    for (uint32_t g = 0; g < p.nGroups; ++g) {
+        const uint32_t end = p.groupEnd[g];
+        batch_group<MT, S, Q>(p, p.groupKey[g], p.work + kGroupCounter + kGroupCounterStride * g, base, end - base);
+        base = end;
+    }
+}
+
+// Scenes of several node memories.
 template <int S, bool Q>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 k_trace_groups(KParams p) {
@@ -1323,13 +1340,13 @@ k_trace_groups(KParams p) {
     stage_batch_tables(p, smem, batch_table_word(S, blockDim.x, Q));
     uint32_t base = 0;
     for (uint32_t g = 0; g < p.nGroups; ++g) {
-        const uint32_t end = p.groupEnd[g];
+        const uint32_t end = p.groupEnd[g], key = p.groupKey[g];
         uint32_t* counter = p.work + kGroupCounter + kGroupCounterStride * g;
-        switch (p.groupM[g]) {
-            case 4: trace_body<4, kMixed, S, true, false, Q>(p, counter, base, end - base); break;
-            case 8: trace_body<8, kMixed, S, true, false, Q>(p, counter, base, end - base); break;
-            case 12: trace_body<12, kMixed, S, true, false, Q>(p, counter, base, end - base); break;
-            case 16: trace_body<16, kMixed, S, true, false, Q>(p, counter, base, end - base); break;
+        switch (key >> 1) {
+            case 4: batch_group<4, S, Q>(p, key, counter, base, end - base); break;
+            case 8: batch_group<8, S, Q>(p, key, counter, base, end - base); break;
+            case 12: batch_group<12, S, Q>(p, key, counter, base, end - base); break;
+            case 16: batch_group<16, S, Q>(p, key, counter, base, end - base); break;
             default: break;
         }
         base = end;
@@ -1862,7 +1879,7 @@ __global__ void k_zero(uint32_t* p, uint32_t words) {
 
 template <int M, int MODE, int S, bool Q>
 void* kernel_ptr() {
-    return reinterpret_cast<void*>(&k_trace<M, MODE, S, false, Q>);
+    return reinterpret_cast<void*>(&k_trace<M, MODE, S, Q>);
 }
 
 template <int M, int MODE>
@@ -1901,9 +1918,9 @@ size_t stream_lds(uint32_t S) { return (size_t)(10 + 11 + S + 2) * kStreamSlots 
 template <int M, bool Q>
 void* pick_batch_s(uint32_t S) {
     switch (S) {
-        case 8: return reinterpret_cast<void*>(&k_trace<M, kMixed, 8, true, Q>);
-        case 16: return reinterpret_cast<void*>(&k_trace<M, kMixed, 16, true, Q>);
-        case 32: return reinterpret_cast<void*>(&k_trace<M, kMixed, 32, true, Q>);
+        case 8: return reinterpret_cast<void*>(&k_trace_batch<M, 8, Q>);
+        case 16: return reinterpret_cast<void*>(&k_trace_batch<M, 16, Q>);
+        case 32: return reinterpret_cast<void*>(&k_trace_batch<M, 32, Q>);
         default: return nullptr;
     }
 }
@@ -1951,7 +1968,7 @@ void* pick_mode(int mode, uint32_t S, bool q) {
 }
 
 void* pick(uint32_t M, int mode, uint32_t S, bool q) {
-    if (mode == kMixed) return pick_batch(S, M, q);
+    if (mode == kBatch) return pick_batch(S, M, q);
     switch (M) {
         case 4: return pick_mode<4>(mode, S, q);
         case 8: return pick_mode<8>(mode, S, q);
@@ -1976,7 +1993,7 @@ hipError_t launch_trace(uint32_t M, int mode, uint32_t S, uint32_t bd, uint32_t 
                         hipStream_t stream, bool quantized, size_t tableBytes) {
     void* fn = pick(M, mode, S, quantized);
     if (!fn) return hipErrorInvalidValue;
-    if (mode == kMixed && tableBytes < batch_table_bytes(p.nScenes, p.nItems)) return hipErrorInvalidValue;
+    if (mode == kBatch && tableBytes < batch_table_bytes(p.nScenes, p.nItems)) return hipErrorInvalidValue;
     KParams copy = p;
     void* args[] = {&copy};
     return hipLaunchKernel(fn, dim3(grid), dim3(bd), args, lds_bytes(M, mode, S, bd, quantized) + tableBytes,
@@ -2127,7 +2144,7 @@ uint32_t coop_width(uint32_t M, uint32_t Lm) {
 }
 
 hipError_t launch_coop(uint32_t W, int mode, uint32_t grid, const KParams& p, hipStream_t stream) {
-    if (mode == kSaveDist || mode == kMixed) return hipErrorInvalidValue;
+    if (mode == kSaveDist || mode == kBatch) return hipErrorInvalidValue;
     void* fn = pick_coop(W, mode);
     if (!fn) return hipErrorInvalidValue;
     KParams copy = p;
