@@ -53,10 +53,30 @@ def limiter_text(c):
     """What binds the kernel, from the counter runs."""
     if not c or "td_busy" not in c:
         return None
-    return (f"VALU issue ({c['valu_insts_per_ray']:.0f} instructions per ray at {c['active_lanes']:.0%} active lanes; "
-            f"the VALUBusy metric reads {c['valu_busy_metric']:.0%}) and the latency of each node's dependent "
-            f"loads (TD {c['td_busy']:.0%} busy, {c['td_stalled']:.0%} stalled on the L1; TA {c['ta_busy']:.0%}); "
+    return (f"the load pipeline: one line per lane for every record load, whose rate the gather probe "
+            f"bounds (gather.frac; TD {c['td_busy']:.0%} busy, {c['td_stalled']:.0%} stalled on the L1; "
+            f"TA {c['ta_busy']:.0%}), beside VALU issue ({c['valu_insts_per_ray']:.0f} instructions per ray at "
+            f"{c['active_lanes']:.0%} active lanes; the VALUBusy metric reads {c['valu_busy_metric']:.0%}); "
             f"HBM carries a fraction of its peak (hbm_frac)")
+
+
+def gather_roofline(c, rays, k_ms):
+    """The kernel's record loads against the load pipeline's ceiling: its
+    wave-level loads per second (the counter runs' loads per ray x rays over
+    this run's kernel time) over rt_measure_gather's rate when every active
+    lane loads from its own 128-byte line (L1-resident, 6 waves per SIMD), at
+    the kernel's VALU active-lane fraction and at a full wave."""
+    from bvhraytracer_amd.host import measure_gather
+    if not c or "loads_per_ray" not in c:
+        return None
+    lanes = max(1, min(64, int(round(c["active_lanes"] * 64))))
+    peak = measure_gather(lanes)
+    full = measure_gather(64)
+    achieved = c["loads_per_ray"] * rays / (k_ms / 1000.0)
+    return {"unit": "wave loads/s", "achieved": float(f"{achieved:.4g}"), "peak": float(f"{peak:.4g}"),
+            "active_lanes": lanes, "frac": round(achieved / peak, 4),
+            "peak_64_lanes": float(f"{full:.4g}"), "frac_64_lanes": round(achieved / full, 4),
+            "probe": "rt_measure_gather: 16-byte loads, one 128-byte line per lane, 16 KB buffer"}
 
 
 def measured_traffic(key):
@@ -727,6 +747,7 @@ def main():
                      "layout_bytes": int(q_bytes), "layout_frac": round(q_bytes / (k_ms / 1000.0) / 1e9 / HBM_PEAK_GBPS, 4),
                      "hbm_frac": hbm_frac(traffic, k_ms),
                      "limiter": limiter_text(MEASURED_COUNTERS.get((N, L, n))),
+                     "gather": gather_roofline(MEASURED_COUNTERS.get((N, L, n)), n, k_ms),
                      "counters": MEASURED_COUNTERS.get((N, L, n)),
                      # bytes the L1s read from L2 per step (the counters' pass) over this run's kernel time
                      "l2_read_gbps": (round(MEASURED_COUNTERS[(N, L, n)]["l2_read_bytes"] / (k_ms / 1000.0) / 1e9, 1)

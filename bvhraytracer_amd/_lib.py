@@ -133,6 +133,7 @@ SIGNATURES = [
     ("rt_trace_steps", ctypes.c_int, [_P, _P, _U32, _U32, _U32, _U32, _P, _P, _P, _P]),
     ("rt_check_recip", ctypes.c_int, [_U32, ctypes.c_uint64, _P, ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_uint64), _P]),
+    ("rt_measure_gather", ctypes.c_int, [_U32, _U32, ctypes.POINTER(ctypes.c_double)]),
     ("rt_set_launch", ctypes.c_int, [_P, _U32, _U32]),
     ("rt_set_schedule", ctypes.c_int, [_P, _U32]),
     ("rt_release", ctypes.c_int, [_P]),

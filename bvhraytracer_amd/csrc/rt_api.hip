@@ -1019,6 +1019,50 @@ extern "C" int rt_check_recip(uint32_t first_bits, uint64_t count, float* values
     return RT_OK;
 }
 
+extern "C" int rt_measure_gather(uint32_t active_lanes, uint32_t buffer_bytes, double* wave_loads_per_s) {
+    if (!wave_loads_per_s) return set_error(RT_ERR_ARG, "rt_measure_gather: null result");
+    if (active_lanes == 0 || active_lanes > 64) return set_error(RT_ERR_ARG, "rt_measure_gather: active lanes 1..64");
+    if (buffer_bytes < 8192 || buffer_bytes > (1u << 30) || (buffer_bytes & (buffer_bytes - 1)))
+        return set_error(RT_ERR_ARG, "rt_measure_gather: buffer bytes a power of two in [8192, 2^30]");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    // 6 blocks of 4 waves per CU: 6 waves per SIMD, the per-ray kernel's
+    // occupancy.  A warm-up launch of about 10 ms brings the clocks up; the
+    // rate is the best of three timed launches of about 40 ms.
+    const uint32_t blocks = (uint32_t)std::max(1, prop.multiProcessorCount) * 6u, iters = 65536;
+    void* buf = nullptr;
+    uint32_t* sink = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms = 0;
+    hipError_t e = hipMalloc(&buf, buffer_bytes);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sink), 64 * 4);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMemsetAsync(buf, 0, buffer_bytes, st);
+    if (e == hipSuccess) e = launch_gather_probe(buf, buffer_bytes / 128, active_lanes, iters / 4, blocks, sink, st);
+    for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
+        float t = 0;
+        e = hipEventRecord(e0, st);
+        if (e == hipSuccess) e = launch_gather_probe(buf, buffer_bytes / 128, active_lanes, iters, blocks, sink, st);
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e == hipSuccess && (ms == 0 || t < ms)) ms = t;
+    }
+    if (e1) (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(sink);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return hip_fail(e, "rt_measure_gather");
+    *wave_loads_per_s = ms > 0 ? (double)blocks * 4.0 * iters / (ms * 1e-3) : 0.0;
+    return RT_OK;
+}
+
 extern "C" int rt_stats(rt_scene_t s, rt_stats_t* o) {
     if (!s || !o) return set_error(RT_ERR_ARG, "rt_stats: null argument");
     std::lock_guard<std::mutex> lk(s->mu);

@@ -148,6 +148,12 @@ hipError_t launch_reset(uint32_t* work, hipStream_t stream);
 // pattern); values (optional) receives the reciprocals.
 hipError_t launch_check_recip(uint32_t first, uint64_t count, float* values, unsigned long long* counter,
                               hipStream_t stream);
+// Load-pipeline calibration: `blocks` blocks of 256 threads; in every wave
+// the first `active` lanes each load 16 bytes from their own 128-byte line of
+// buf (lines * 128 bytes, lines a power of two >= 64, zero-filled), `iters`
+// times; sink: 64 words.
+hipError_t launch_gather_probe(const void* buf, uint32_t lines, uint32_t active, uint32_t iters, uint32_t blocks,
+                               uint32_t* sink, hipStream_t stream);
 // Zeroes `words` 32-bit words with a kernel (no memset node if captured).
 hipError_t launch_zero(uint32_t* p, uint32_t words, hipStream_t stream);
 

@@ -1846,6 +1846,28 @@ __global__ void k_reset(uint32_t* work) {
 // the lowest mismatching pattern is kept in counter[1] (atomicMin).  values,
 // if not null, receives recip(x) for each pattern.  Every thread's loop is
 // bounded by count.
+// Load-pipeline calibration.  Each of the first `active` lanes of every wave
+// loads 16 bytes from its own 128-byte line, `iters` independent loads per
+// lane; lineMask + 1 lines (a power of two, at least 64, so the lanes of one
+// load touch as many different lines).  This is the shape of the per-ray
+// kernel's node and leaf loads: one record per lane, a different line for
+// every lane.  The buffer is zero, so the final store is never taken; it
+// keeps the loads.
+__global__ void __launch_bounds__(256) k_gather_probe(const uint4* __restrict__ buf, uint32_t lineMask,
+                                                      uint32_t active, uint32_t iters, uint32_t* sink) {
+    const uint32_t lane = __lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    uint32_t acc = 0;
+    if (lane < active) {
+        for (uint32_t it = 0; it < iters; ++it) {
+            const uint32_t line = ((wave * 97u + it * 131u) * 64u + lane) & lineMask;
+            const uint4 v = buf[(size_t)line * 8u];
+            acc += v.x ^ v.y ^ v.z ^ v.w;
+        }
+    }
+    if (acc == 0x9E3779B9u) sink[lane] = acc;
+}
+
 __global__ void __launch_bounds__(256) k_check_recip(uint32_t first, uint64_t count, float* values,
                                                      unsigned long long* counter) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -2170,6 +2192,14 @@ hipError_t launch_zero(uint32_t* p, uint32_t words, hipStream_t stream) {
 
 hipError_t launch_reset(uint32_t* work, hipStream_t stream) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, stream, work);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_probe(const void* buf, uint32_t lines, uint32_t active, uint32_t iters, uint32_t blocks,
+                               uint32_t* sink, hipStream_t stream) {
+    if (lines < 64 || (lines & (lines - 1)) || active == 0 || active > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gather_probe, dim3(blocks), dim3(256), 0, stream, static_cast<const uint4*>(buf), lines - 1,
+                       active, iters, sink);
     return hipGetLastError();
 }
 

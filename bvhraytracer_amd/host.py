@@ -520,3 +520,12 @@ def make_rays(origins, targets, tmax=222222.0):
     r["ix"], r["iy"], r["iz"] = inv[:, 0], inv[:, 1], inv[:, 2]
     r["dx"], r["dy"], r["dz"] = d[:, 0], d[:, 1], d[:, 2]
     return r
+
+
+def measure_gather(active_lanes=64, buffer_bytes=16384):
+    """Wave-level 16-byte loads per second on the current GPU when each of the
+    first `active_lanes` lanes of every wave loads from its own 128-byte line
+    (rt_measure_gather): the ceiling of the traversal kernels' record loads."""
+    out = ctypes.c_double()
+    check(lib().rt_measure_gather(int(active_lanes), int(buffer_bytes), ctypes.byref(out)))
+    return out.value

@@ -267,6 +267,17 @@ int rt_trace_steps(rt_scene_t s, const rt_ray* rays, uint32_t n, uint32_t flags,
 int rt_check_recip(uint32_t first_bits, uint64_t count, float* values, uint64_t* mismatches,
                    uint64_t* first_mismatch, void* hip_stream);
 
+/* Load-pipeline calibration on the current device: wave-level 16-byte loads
+ * per second when each of the first active_lanes (1..64) lanes of every wave
+ * loads from its own 128-byte line of a buffer_bytes buffer (a power of two
+ * in [8192, 2^30]; 16 KB stays in the vector L1), 6 waves per SIMD.  This is
+ * the shape of the traversal kernels' node and leaf loads, one record per
+ * lane, and the rate is the ceiling their load count is measured against.
+ * No reference counterpart; the call allocates, runs a warm-up launch and
+ * three timed ones (best of three, about 0.15 s in all) on a stream of its
+ * own and waits. */
+int rt_measure_gather(uint32_t active_lanes, uint32_t buffer_bytes, double* wave_loads_per_s);
+
 /* Overrides the launch configuration (block size 64, 128, 192 or 256,
  * LDS stack window 8/16/32 entries).  0 keeps the current value. */
 int rt_set_launch(rt_scene_t s, uint32_t block_size, uint32_t stack_window);
