@@ -253,23 +253,25 @@ def batched_autotuned(rt, torch, scenes, ref_outs, stream, total, args, log):
     2^20 per launch, so the GPU is full), traced in one batched graph launch
     that mixes the chosen node and leaf memories.  Block size and stack
     window are properties of a launch (its blocks and its LDS allocation),
-    and so is the node format (one kernel per launch), so the batch takes one
-    format and shape: the fastest of the quantized and exact-box records at
-    BATCH_SHAPES for the whole launch.  The tuner's last step compares that
-    batch with N8L8 for every scene, at each format and shape, and keeps the
-    faster (per-scene timings alone do not see how the scenes share the GPU
-    in one launch).  The per-scene
-    shape choices are reported with what they gain over the default shape
-    on the scene alone.  Tuning and the rebuilds are ingest cost, outside
-    the timing."""
+    and so is the node format (one kernel per launch).  Two ways to honour
+    them: one format and shape for the whole launch (the fastest of the
+    quantized and exact-box records at BATCH_SHAPES), or every scene with its
+    own tuned format, shape and schedule (RT_SCENE_SHAPES: one persistent
+    launch per distinct shape, in one graph).  The tuner's last step compares
+    these with N8L8 for every scene, at each format and shape, and keeps the
+    fastest (per-scene timings alone do not see how the scenes share the GPU
+    in one launch).  The per-scene shape choices are reported with what they
+    gain over the default shape on the scene alone.  Tuning and the rebuilds
+    are ingest cost, outside the timing."""
     from bvhraytracer_amd.autotune import Autotuner
 
     t0 = time.time()
     tuner = Autotuner(nl=((4, 4), (4, 8), (8, 4), (8, 8)), blocks=(128, 256), windows=(8, 16),
                       schedules=(16,), sample=8192, reps=2, fill=1 << 20)
-    tuned, choice, shapes, gains = [], {}, {}, []
+    tuned, choice, shapes, gains, bests = [], {}, {}, [], []
     for tris, bvh0, sc0, rays, d_r, d_h in scenes:
         best = tuner.tune_scene(tris, rays)
+        bests.append(best)
         key = f"N{best['N']}L{best['L']}"
         choice[key] = choice.get(key, 0) + 1
         shape = f"{'x' if best['exact_boxes'] else 'q'}{best['block']}x{best['window']}"
@@ -301,23 +303,44 @@ def batched_autotuned(rt, torch, scenes, ref_outs, stream, total, args, log):
 
     by_shape = by_shape_of(tuned_items)
     uniform_by_shape = by_shape_of(uniform_items)
+    # every scene with its own tuned format, shape and schedule
+    # (RT_SCENE_SHAPES): one persistent launch per distinct shape, one graph
+    for (bvh, sc, d_r, o), best in zip(tuned, bests):
+        sc.set_launch(best["block"], best["window"])
+        sc.set_schedule(best["schedule"])
+        sc.exact_boxes = best["exact_boxes"]
+    segments = len({(b["exact_boxes"], b["block"], b["window"], b["schedule"]) for b in bests})
+    sec = median_wall(lambda: batch.trace(tuned_items, graph=True, stream=stream, sync=False, scene_shapes=True),
+                      torch, reps, args.warmup)
+    scene_shapes_mrays = total / sec / 1e6
+    shaped_outs = [o.clone() for (b, s_, d, o) in tuned]
     per_scene_best = max(by_shape.values())
-    use_tuned = per_scene_best >= max(uniform_by_shape.values())
-    items = tuned_items if use_tuned else uniform_items
-    shapes_of = by_shape if use_tuned else uniform_by_shape
-    best_shape = max(shapes_of, key=shapes_of.get)
-    exact = best_shape[0] == "x"
-    bd, win = (int(v) for v in best_shape[1:].split("x"))
-    batch.set_launch(bd, win)
+    candidates = {"per-scene N/L": per_scene_best, "N8L8 for every scene": max(uniform_by_shape.values()),
+                  "per-scene N/L and shapes": scene_shapes_mrays}
+    kept = max(candidates, key=candidates.get)
+    if kept == "per-scene N/L and shapes":
+        best_shape = f"per scene ({segments} launches)"
+        run = lambda: batch.trace(tuned_items, graph=True, stream=stream, sync=False, scene_shapes=True)  # noqa: E731
+    else:
+        items = tuned_items if kept == "per-scene N/L" else uniform_items
+        shapes_of = by_shape if kept == "per-scene N/L" else uniform_by_shape
+        best_shape = max(shapes_of, key=shapes_of.get)
+        exact = best_shape[0] == "x"
+        bd, win = (int(v) for v in best_shape[1:].split("x"))
+        batch.set_launch(bd, win)
+        run = lambda: batch.trace(items, graph=True, stream=stream, sync=False, exact_boxes=exact)  # noqa: E731
     # the kept configuration timed again, with three times the repetitions
-    sec = median_wall(lambda: batch.trace(items, graph=True, stream=stream, sync=False, exact_boxes=exact), torch,
-                      3 * reps, args.warmup)
+    sec = median_wall(run, torch, 3 * reps, args.warmup)
     mrays = total / sec / 1e6
+    # the per-scene-shape launches give the same results as one launch shape
+    batch.set_launch(128, 8)
+    batch.trace(tuned_items, graph=True, stream=stream)
+    shapes_identical = all(torch.equal(a, o) for a, (b, s_, d, o) in zip(shaped_outs, tuned))
     # the hit triangles are the same whatever the tree, up to exact ties
     same_tri = sum(int((o[:, 0] == r[:, 0]).sum().item()) for (b, s, d, o), r in zip(tuned, ref_outs)) / total
-    kept = "per-scene N/L" if use_tuned else "N8L8 for every scene"
     log(f"batched autotuned {mrays:.1f} Mrays/s ({kept}, launch shape {best_shape}; per-scene N/L by shape "
-        f"{by_shape}, N8L8 by shape {uniform_by_shape}), choices {choice}, per-scene shapes {shapes}, "
+        f"{by_shape}, N8L8 by shape {uniform_by_shape}, per-scene N/L and shapes {scene_shapes_mrays:.1f} in "
+        f"{segments} launches, identical {shapes_identical}), choices {choice}, per-scene shapes {shapes}, "
         f"tuning {tune_s:.1f}s")
     batch.close()
     for bvh, sc, d_r, o in tuned:
@@ -325,6 +348,8 @@ def batched_autotuned(rt, torch, scenes, ref_outs, stream, total, args, log):
             sc.close()
     return {"batched_mrays": round(mrays, 2), "kept": kept, "launch_shape": best_shape,
             "per_scene_nl_mrays_by_shape": by_shape, "uniform_n8l8_mrays_by_shape": uniform_by_shape,
+            "per_scene_shapes_mrays": round(scene_shapes_mrays, 2), "per_scene_shape_launches": segments,
+            "per_scene_shapes_identical": shapes_identical,
             "choices": choice, "per_scene_shapes": shapes,
             "per_scene_shape_gain_median": round(float(np.median(gains)), 4) if gains else None,
             "per_scene_shape_gain_max": round(float(np.max(gains)), 4) if gains else None,

@@ -25,6 +25,7 @@ RT_COOP = 64
 RT_EXACT_BOXES = 256
 
 RT_GRAPH = 128
+RT_SCENE_SHAPES = 512
 
 RT_SCENE_SPHERES = 1
 RT_SCENE_INTERIOR = 2
@@ -135,6 +136,7 @@ SIGNATURES = [
                                       ctypes.POINTER(ctypes.c_uint64), _P]),
     ("rt_measure_gather", ctypes.c_int, [_U32, _U32, ctypes.POINTER(ctypes.c_double)]),
     ("rt_set_launch", ctypes.c_int, [_P, _U32, _U32]),
+    ("rt_set_node_format", ctypes.c_int, [_P, _U32]),
     ("rt_set_schedule", ctypes.c_int, [_P, _U32]),
     ("rt_release", ctypes.c_int, [_P]),
     ("rt_last_error", ctypes.c_char_p, []),

@@ -165,6 +165,7 @@ struct rt_scene {
     uint64_t bytes = 0;
     uint32_t blockSize = 256, stackWindow = 16;
     bool launchSet = false;  // rt_set_launch was called: its shape serves both node formats
+    bool exactBoxes = false;  // rt_set_node_format: items of RT_SCENE_SHAPES batches on the exact-box records
     uint32_t cus = 0;
     std::unordered_map<hipStream_t, Workspace> ws;
     uint64_t useClock = 0;
@@ -611,6 +612,14 @@ extern "C" int rt_set_launch(rt_scene_t s, uint32_t block_size, uint32_t stack_w
     if (block_size) s->blockSize = block_size;
     if (stack_window) s->stackWindow = stack_window;
     if (block_size || stack_window) s->launchSet = true;
+    return RT_OK;
+}
+
+extern "C" int rt_set_node_format(rt_scene_t s, uint32_t flags) {
+    if (!s) return set_error(RT_ERR_ARG, "rt_set_node_format: null scene");
+    if (flags & ~RT_EXACT_BOXES) return set_error(RT_ERR_ARG, "rt_set_node_format: flags take RT_EXACT_BOXES only");
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->exactBoxes = (flags & RT_EXACT_BOXES) != 0;
     return RT_OK;
 }
 
@@ -1111,6 +1120,26 @@ extern "C" int rt_release(rt_scene_t s) {
 // ---------------------------------------------------------------------------
 // Batched launch: many scenes and ray batches in one persistent launch.
 
+// One persistent launch of a batched call: the items [i0, i1) of the sorted
+// table, which share a node format and a launch shape.  A call without
+// RT_SCENE_SHAPES is one segment; with it, one segment per distinct
+// (format, block size, stack window, schedule) of the items' scenes, run one
+// after another on the stream.
+struct BatchSeg {
+    bool q = false;
+    uint32_t bd = 0, S = 0, sched = 0;
+    uint32_t M = 0;  // node memory of every scene of the segment, 0 when they differ
+    uint32_t i0 = 0, i1 = 0;
+    uint32_t nGroups = 0, groupKey[kMaxGroups] = {}, groupEnd[kMaxGroups] = {};
+    uint64_t total = 0;
+    uint32_t grid = 0, spillCap = 0;
+    size_t table = 0;  // LDS bytes reserved for the staged tables
+    bool same_launch(const BatchSeg& o) const {
+        return q == o.q && bd == o.bd && S == o.S && M == o.M && grid == o.grid && table == o.table &&
+               spillCap == o.spillCap;
+    }
+};
+
 struct rt_batch {
     std::mutex mu;
     int device = 0;
@@ -1128,16 +1157,15 @@ struct rt_batch {
     bool pending = false;
     hipStream_t ownStream = nullptr;
     hipEvent_t joinEv = nullptr;  // orders ownStream with the null stream
-    // captured graph: [reset kernel] -> [batched trace kernel]
+    // captured graph: per launch segment [reset kernel] -> [batched trace kernel]
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipGraphNode_t traceNode = nullptr;
-    hipKernelNodeParams traceNodeParams{};
+    std::vector<hipGraphNode_t> traceNodes;  // in launch order
+    std::vector<hipKernelNodeParams> traceNodeParams;
     hipStream_t graphStream = nullptr;
-    uint32_t graphGrid = 0, graphBd = 0, graphS = 0, graphSpillCap = 0, graphM = 0;
-    bool graphQ = false;
-    size_t graphTable = 0;  // LDS bytes the captured launch reserves for the staged tables
+    std::vector<BatchSeg> graphSegs;  // the captured segments (shape, grid, LDS tables, spill)
     uint32_t* graphSpill = nullptr;
+    uint32_t maxStack = 0;  // stack entries per ray of the last launch
     float lastMs = 0;
     uint64_t raysTraced = 0;
     uint64_t graphLaunches = 0;
@@ -1150,7 +1178,9 @@ void drop_graph(rt_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
-    b->traceNode = nullptr;
+    b->traceNodes.clear();
+    b->traceNodeParams.clear();
+    b->graphSegs.clear();
 }
 
 void free_batch(rt_batch* b) {
@@ -1173,44 +1203,54 @@ void free_batch(rt_batch* b) {
     delete b;
 }
 
-int capture_graph(rt_batch* b, hipStream_t st, const KParams& p, uint32_t grid, uint32_t bd, uint32_t S,
-                  uint32_t M, bool q, size_t table) {
+// Captures the segments' launches (reset kernel, trace kernel, one pair per
+// segment, in order) as one graph and finds the trace kernel nodes by
+// walking the captured chain from its root.
+int capture_graph(rt_batch* b, hipStream_t st, const std::vector<BatchSeg>& segs, const std::vector<KParams>& ps) {
     drop_graph(b);
     HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    hipError_t e1 = launch_reset(p.work, st);
-    hipError_t e2 = launch_trace(M, kBatch, S, bd, grid, p, st, q, table);
-    hipGraph_t g = nullptr;
-    hipError_t e3 = hipStreamEndCapture(st, &g);
-    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        return hip_fail(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3), "graph capture");
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < segs.size() && e == hipSuccess; ++i) {
+        const BatchSeg& g = segs[i];
+        e = launch_reset(ps[i].work, st);
+        if (e == hipSuccess) e = launch_trace(g.M, kBatch, g.S, g.bd, g.grid, ps[i], st, g.q, g.table);
     }
-    b->graph = g;
-    size_t nn = 0;
-    HIP_TRY(hipGraphGetNodes(g, nullptr, &nn), "hipGraphGetNodes");
-    std::vector<hipGraphNode_t> nodes(nn);
-    HIP_TRY(hipGraphGetNodes(g, nodes.data(), &nn), "hipGraphGetNodes");
-    for (auto nd : nodes) {
+    hipGraph_t gr = nullptr;
+    const hipError_t e3 = hipStreamEndCapture(st, &gr);
+    if (e != hipSuccess || e3 != hipSuccess) {
+        if (gr) (void)hipGraphDestroy(gr);
+        return hip_fail(e != hipSuccess ? e : e3, "graph capture");
+    }
+    b->graph = gr;
+    size_t nr = 1;
+    hipGraphNode_t nd = nullptr;
+    HIP_TRY(hipGraphGetRootNodes(gr, &nd, &nr), "hipGraphGetRootNodes");
+    if (nr != 1) return set_error(RT_ERR_HIP, "graph capture: expected one chain");
+    while (nd) {
         hipGraphNodeType t;
-        if (hipGraphNodeGetType(nd, &t) != hipSuccess || t != hipGraphNodeTypeKernel) continue;
-        hipKernelNodeParams kp{};
-        HIP_TRY(hipGraphKernelNodeGetParams(nd, &kp), "hipGraphKernelNodeGetParams");
-        if (kp.gridDim.x == grid && kp.blockDim.x == bd && kp.sharedMemBytes > 0) {
-            b->traceNode = nd;
-            b->traceNodeParams = kp;
+        HIP_TRY(hipGraphNodeGetType(nd, &t), "hipGraphNodeGetType");
+        if (t == hipGraphNodeTypeKernel) {
+            hipKernelNodeParams kp{};
+            HIP_TRY(hipGraphKernelNodeGetParams(nd, &kp), "hipGraphKernelNodeGetParams");
+            // the trace kernels are the nodes with LDS (stack window and tables)
+            if (kp.sharedMemBytes > 0) {
+                b->traceNodes.push_back(nd);
+                b->traceNodeParams.push_back(kp);
+            }
         }
+        size_t nd2 = 0;
+        HIP_TRY(hipGraphNodeGetDependentNodes(nd, nullptr, &nd2), "hipGraphNodeGetDependentNodes");
+        if (nd2 > 1) return set_error(RT_ERR_HIP, "graph capture: expected one chain");
+        hipGraphNode_t next = nullptr;
+        if (nd2 == 1) HIP_TRY(hipGraphNodeGetDependentNodes(nd, &next, &nd2), "hipGraphNodeGetDependentNodes");
+        nd = next;
     }
-    if (!b->traceNode) return set_error(RT_ERR_HIP, "graph capture: trace kernel node not found");
-    HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (b->traceNodes.size() != segs.size())
+        return set_error(RT_ERR_HIP, "graph capture: trace kernel nodes not found");
+    HIP_TRY(hipGraphInstantiate(&b->exec, gr, nullptr, nullptr, 0), "hipGraphInstantiate");
     b->graphStream = st;
-    b->graphGrid = grid;
-    b->graphBd = bd;
-    b->graphS = S;
     b->graphSpill = b->spill;
-    b->graphSpillCap = p.spillCap;
-    b->graphM = M;
-    b->graphQ = q;
-    b->graphTable = table;
+    b->graphSegs = segs;
     return RT_OK;
 }
 
@@ -1257,7 +1297,8 @@ extern "C" int rt_batch_set_launch(rt_batch_t b, uint32_t block_size, uint32_t s
 extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t n_items, uint32_t flags,
                               void* hip_stream) {
     if (!b) return set_error(RT_ERR_ARG, "rt_batch_trace: null batch");
-    if (flags & ~(RT_GRAPH | RT_NO_SYNC | RT_EXACT_BOXES)) return set_error(RT_ERR_ARG, "rt_batch_trace: unknown flag bits");
+    if (flags & ~(RT_GRAPH | RT_NO_SYNC | RT_EXACT_BOXES | RT_SCENE_SHAPES))
+        return set_error(RT_ERR_ARG, "rt_batch_trace: unknown flag bits");
     if (n_items == 0) return RT_OK;
     if (!items) return set_error(RT_ERR_ARG, "rt_batch_trace: null items");
     if (n_items > b->maxItems) return set_error(RT_ERR_ARG, "rt_batch_trace: more items than the batch was created for");
@@ -1327,100 +1368,156 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     }
     if (used == 0) return RT_OK;
     const bool graph = (flags & RT_GRAPH) != 0;
-    // The items are ordered (stable) by node memory, then closest before
-    // any hit, into groups that the kernel runs one after another, each with
-    // the body specialised on its node memory and mode.  One node memory for
-    // the whole batch: the kernel specialised on it; several: the grouped
-    // kernel.  Item starts are the running ray offsets in that order.
-    uint32_t batchM = b->hScenes[0].M;
-    for (uint32_t i = 1; i < nScenes; ++i)
-        if (b->hScenes[i].M != batchM) batchM = 0;
-    // the quantized records when every scene of the batch has them
-    bool batchQ = !(flags & RT_EXACT_BOXES);
-    for (uint32_t i = 0; i < nScenes; ++i) batchQ = batchQ && scenes[i]->qnodes;
-    if (batchQ)
-        for (uint32_t i = 0; i < nScenes; ++i) b->hScenes[i].nodes = scenes[i]->qnodes;
-    const bool qShape = batchQ && !b->launchSet;
-    const uint32_t S = qShape ? kQWindow : b->stackWindow, bd = qShape ? kQBlock : b->blockSize;
-    uint32_t nGroups = 0, groupKey[kMaxGroups] = {}, groupEnd[kMaxGroups] = {};
+    const bool sceneShapes = (flags & RT_SCENE_SHAPES) != 0;
+    // Node format and launch shape of every scene.  Without RT_SCENE_SHAPES
+    // the batch has one: the quantized records when every scene has them,
+    // and the batch's launch shape.  With it, each scene its own: its node
+    // format (rt_set_node_format, RT_EXACT_BOXES), its launch shape
+    // (rt_set_launch) and its wave schedule.
+    std::vector<uint64_t> sceneKey(nScenes);
     {
-        auto key = [&](const BatchItem& x) { return batch_group_key(b->hScenes[x.scene].M, x.mode); };
-        std::stable_sort(b->hItems, b->hItems + used,
-                         [&](const BatchItem& x, const BatchItem& y) { return key(x) < key(y); });
-        uint32_t off = 0;
-        for (uint32_t i = 0; i < used; ++i) {
-            BatchItem& bi = b->hItems[i];
-            const uint32_t k = key(bi);
-            if (nGroups == 0 || groupKey[nGroups - 1] != k) {
-                if (nGroups == kMaxGroups)
-                    return set_error(RT_ERR_LAYOUT, "rt_batch_trace: more than 8 (node memory, mode) groups");
-                groupKey[nGroups++] = k;
+        bool allQ = !(flags & RT_EXACT_BOXES);
+        for (uint32_t i = 0; i < nScenes; ++i) allQ = allQ && scenes[i]->qnodes;
+        for (uint32_t i = 0; i < nScenes; ++i) {
+            const rt_scene* sc = scenes[i];
+            bool q;
+            uint32_t bd, S, sched;
+            if (sceneShapes) {
+                q = use_quantized(sc, kClosest, flags) && !sc->exactBoxes;
+                const bool qShape = q && !sc->launchSet;
+                bd = qShape ? kQBlock : sc->blockSize;
+                S = qShape ? kQWindow : sc->stackWindow;
+                sched = sc->sched;
+            } else {
+                q = allQ;
+                const bool qShape = q && !b->launchSet;
+                bd = qShape ? kQBlock : b->blockSize;
+                S = qShape ? kQWindow : b->stackWindow;
+                sched = kDefaultSched;
             }
-            bi.start = off;
-            off += bi.n;
-            groupEnd[nGroups - 1] = off;
+            if (q) b->hScenes[i].nodes = sc->qnodes;
+            sceneKey[i] = (uint64_t)(q ? 1u : 0u) << 56 | (uint64_t)bd << 40 | (uint64_t)S << 32 | sched;
         }
     }
-    // the scene and item tables go to LDS; a captured graph keeps the larger
-    // reservation it was made with, so later batches with smaller tables
-    // replay it unchanged
-    size_t table = batch_table_bytes(nScenes, used);
-    if (graph && b->exec && b->graphTable >= table) table = b->graphTable;
-    const int occ = trace_occupancy(batchM, kBatch, S, bd, batchQ, table);
-    if (occ <= 0)
-        return set_error(RT_ERR_ARG, "rt_batch_trace: the tables of " + std::to_string(nScenes) + " scenes and " +
-                                         std::to_string(used) + " items (" + std::to_string(table) +
-                                         " bytes) do not fit the kernel's LDS beside a " + std::to_string(S) +
-                                         "-entry stack window of " + std::to_string(bd) + " threads");
-    const uint64_t full = (uint64_t)b->cus * (uint64_t)occ;
-    const uint32_t grid = graph ? (uint32_t)full
-                                : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(full, (total + bd - 1) / bd));
-    uint32_t spillCap = cap > S ? cap - S : 0;
-    if (graph && b->exec && b->graphSpillCap > spillCap && b->graphGrid == grid) spillCap = b->graphSpillCap;
-    const size_t spillNeed = std::max<size_t>(64, (size_t)grid * bd * spillCap * 4 * (batchQ ? 2 : 1));
+    // The items are ordered (stable) by segment, then by node memory, then
+    // closest before any hit, into groups that the segment's kernel runs one
+    // after another, each with the body specialised on its node memory and
+    // mode.  A segment whose scenes share one node memory runs the kernel
+    // specialised on it; otherwise the grouped kernel.  Item starts are the
+    // running ray offsets within the segment.
+    auto gkey = [&](const BatchItem& x) { return batch_group_key(b->hScenes[x.scene].M, x.mode); };
+    std::stable_sort(b->hItems, b->hItems + used, [&](const BatchItem& x, const BatchItem& y) {
+        const uint64_t kx = sceneKey[x.scene], ky = sceneKey[y.scene];
+        return kx != ky ? kx < ky : gkey(x) < gkey(y);
+    });
+    std::vector<BatchSeg> segs;
+    for (uint32_t i = 0; i < used; ++i) {
+        BatchItem& bi = b->hItems[i];
+        const uint64_t sk = sceneKey[bi.scene];
+        if (segs.empty() || sceneKey[b->hItems[segs.back().i0].scene] != sk) {
+            BatchSeg g;
+            g.q = (sk >> 56) != 0;
+            g.bd = (uint32_t)(sk >> 40) & 0xFFFFu;
+            g.S = (uint32_t)(sk >> 32) & 0xFFu;
+            g.sched = (uint32_t)sk;
+            g.M = b->hScenes[bi.scene].M;
+            g.i0 = i;
+            segs.push_back(g);
+        }
+        BatchSeg& g = segs.back();
+        g.i1 = i + 1;
+        if (b->hScenes[bi.scene].M != g.M) g.M = 0;
+        const uint32_t k = gkey(bi);
+        if (g.nGroups == 0 || g.groupKey[g.nGroups - 1] != k) {
+            if (g.nGroups == kMaxGroups)
+                return set_error(RT_ERR_LAYOUT, "rt_batch_trace: more than 8 (node memory, mode) groups");
+            g.groupKey[g.nGroups++] = k;
+        }
+        bi.start = (uint32_t)g.total;
+        g.total += bi.n;
+        g.groupEnd[g.nGroups - 1] = (uint32_t)g.total;
+    }
+    // A captured graph with the same segment shapes is re-armed; it keeps the
+    // larger LDS table reservation and spill capacity it was made with.
+    const bool reuse = graph && b->exec && b->graphStream == st && b->graphSegs.size() == segs.size();
+    size_t spillNeed = 64;
+    uint32_t maxStack = 0;
+    for (size_t i = 0; i < segs.size(); ++i) {
+        BatchSeg& g = segs[i];
+        const BatchSeg* old = reuse ? &b->graphSegs[i] : nullptr;
+        if (old && (old->q != g.q || old->bd != g.bd || old->S != g.S || old->M != g.M)) old = nullptr;
+        // the scene and item tables go to LDS
+        g.table = batch_table_bytes(nScenes, g.i1 - g.i0);
+        if (old && old->table >= g.table) g.table = old->table;
+        const int occ = trace_occupancy(g.M, kBatch, g.S, g.bd, g.q, g.table);
+        if (occ <= 0)
+            return set_error(RT_ERR_ARG, "rt_batch_trace: the tables of " + std::to_string(nScenes) + " scenes and " +
+                                             std::to_string(g.i1 - g.i0) + " items (" + std::to_string(g.table) +
+                                             " bytes) do not fit the kernel's LDS beside a " + std::to_string(g.S) +
+                                             "-entry stack window of " + std::to_string(g.bd) + " threads");
+        const uint64_t full = (uint64_t)b->cus * (uint64_t)occ;
+        g.grid = graph ? (uint32_t)full
+                       : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(full, (g.total + g.bd - 1) / g.bd));
+        g.spillCap = cap > g.S ? cap - g.S : 0;
+        if (old && old->spillCap > g.spillCap && old->grid == g.grid) g.spillCap = old->spillCap;
+        spillNeed = std::max<size_t>(spillNeed, (size_t)g.grid * g.bd * g.spillCap * 4 * (g.q ? 2 : 1));
+        maxStack = std::max(maxStack, g.spillCap + g.S);
+    }
     int rc;
     if ((rc = ensure(reinterpret_cast<void**>(&b->spill), b->spillBytes, spillNeed, "hipMalloc(spill)")) != RT_OK)
         return rc;
     HIP_TRY(hipMemcpyAsync(b->dItems, b->hItems, sizeof(BatchItem) * used, hipMemcpyHostToDevice, st), "items");
     HIP_TRY(hipMemcpyAsync(b->dScenes, b->hScenes, sizeof(BatchScene) * nScenes, hipMemcpyHostToDevice, st), "scenes");
-    KParams p{};
-    p.n = (uint32_t)total;
-    p.spill = b->spill;
-    p.spillCap = spillCap;
-    p.work = b->work;
-    p.items = b->dItems;
-    p.scenes = b->dScenes;
-    p.nItems = used;
-    p.nScenes = nScenes;
-    p.nGroups = nGroups;
-    for (uint32_t g = 0; g < nGroups; ++g) {
-        p.groupKey[g] = groupKey[g];
-        p.groupEnd[g] = groupEnd[g];
+    std::vector<KParams> ps(segs.size());
+    for (size_t i = 0; i < segs.size(); ++i) {
+        const BatchSeg& g = segs[i];
+        KParams& p = ps[i];
+        p = KParams{};
+        p.n = (uint32_t)g.total;
+        p.spill = b->spill;
+        p.spillCap = g.spillCap;
+        p.work = b->work;
+        p.items = b->dItems + g.i0;
+        p.scenes = b->dScenes;
+        p.nItems = g.i1 - g.i0;
+        p.nScenes = nScenes;
+        p.nGroups = g.nGroups;
+        for (uint32_t k = 0; k < g.nGroups; ++k) {
+            p.groupKey[k] = g.groupKey[k];
+            p.groupEnd[k] = g.groupEnd[k];
+        }
+        p.chunk = ray_chunk(g.total, g.grid, g.bd);
+        p.sched = g.sched;
     }
-    p.chunk = ray_chunk(total, grid, bd);
-    p.sched = kDefaultSched;
+    b->maxStack = maxStack;
     if (graph) {
-        if (!b->exec || b->graphStream != st || b->graphGrid != grid || b->graphBd != bd || b->graphS != S ||
-            b->graphSpill != b->spill || b->graphSpillCap != spillCap || b->graphM != batchM ||
-            b->graphQ != batchQ || b->graphTable != table) {
-            if ((rc = capture_graph(b, st, p, grid, bd, S, batchM, batchQ, table)) != RT_OK) return rc;
+        bool same = reuse && b->graphSpill == b->spill;
+        for (size_t i = 0; same && i < segs.size(); ++i) same = segs[i].same_launch(b->graphSegs[i]);
+        if (!same) {
+            if ((rc = capture_graph(b, st, segs, ps)) != RT_OK) return rc;
         } else {
-            // re-arm: same graph, new kernel arguments (ray count, item count)
-            hipKernelNodeParams kp = b->traceNodeParams;
-            KParams copy = p;
-            void* args[] = {&copy};
-            kp.kernelParams = args;
-            kp.extra = nullptr;
-            HIP_TRY(hipGraphExecKernelNodeSetParams(b->exec, b->traceNode, &kp), "hipGraphExecKernelNodeSetParams");
+            // re-arm: same graph, new kernel arguments (ray counts, item tables)
+            for (size_t i = 0; i < segs.size(); ++i) {
+                hipKernelNodeParams kp = b->traceNodeParams[i];
+                KParams copy = ps[i];
+                void* args[] = {&copy};
+                kp.kernelParams = args;
+                kp.extra = nullptr;
+                HIP_TRY(hipGraphExecKernelNodeSetParams(b->exec, b->traceNodes[i], &kp),
+                        "hipGraphExecKernelNodeSetParams");
+            }
         }
         HIP_TRY(hipEventRecord(b->ev0, st), "hipEventRecord");
         HIP_TRY(hipGraphLaunch(b->exec, st), "hipGraphLaunch");
         HIP_TRY(hipEventRecord(b->ev1, st), "hipEventRecord");
         ++b->graphLaunches;
     } else {
-        HIP_TRY(launch_reset(b->work, st), "reset kernel");
         HIP_TRY(hipEventRecord(b->ev0, st), "hipEventRecord");
-        HIP_TRY(launch_trace(batchM, kBatch, S, bd, grid, p, st, batchQ, table), "batched trace kernel");
+        for (size_t i = 0; i < segs.size(); ++i) {
+            const BatchSeg& g = segs[i];
+            HIP_TRY(launch_reset(b->work, st), "reset kernel");
+            HIP_TRY(launch_trace(g.M, kBatch, g.S, g.bd, g.grid, ps[i], st, g.q, g.table), "batched trace kernel");
+        }
         HIP_TRY(hipEventRecord(b->ev1, st), "hipEventRecord");
     }
     b->pending = true;
@@ -1451,7 +1548,7 @@ extern "C" int rt_batch_stats(rt_batch_t b, rt_stats_t* o) {
     o->rays_traced = b->raysTraced;
     o->block_size = b->blockSize;
     o->stack_window = b->stackWindow;
-    o->max_stack = b->graphSpillCap + b->stackWindow;
+    o->max_stack = b->maxStack;
     // the guard counter accumulates on the device over the batch's launches
     if (b->pending) HIP_TRY(hipEventSynchronize(b->ev1), "hipEventSynchronize(stats)");
     uint32_t errs = 0;

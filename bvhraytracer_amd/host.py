@@ -172,12 +172,22 @@ class Scene:
 
     def __init__(self, bvh):
         self.bvh = bvh
-        # node format of the per-ray closest / any-hit calls that do not name
-        # one: the quantized records (False) or the exact-box records (True)
-        self.exact_boxes = False
         h = ctypes.c_void_p()
         check(lib().rt_upload_bvh(ctypes.byref(bvh.desc), ctypes.byref(h)))
         self._h = h
+        self._exact_boxes = False
+
+    @property
+    def exact_boxes(self):
+        """Node format of the per-ray closest / any-hit calls that do not name
+        one and of this scene's items in batches traced with scene_shapes:
+        the quantized records (False) or the exact-box records (True)."""
+        return self._exact_boxes
+
+    @exact_boxes.setter
+    def exact_boxes(self, value):
+        check(lib().rt_set_node_format(self._h, _lib.RT_EXACT_BOXES if value else 0))
+        self._exact_boxes = bool(value)
 
     def set_launch(self, block_size=0, stack_window=0):
         check(lib().rt_set_launch(self._h, int(block_size), int(stack_window)))
@@ -324,7 +334,11 @@ class Batch:
     def set_launch(self, block_size=0, stack_window=0):
         check(lib().rt_batch_set_launch(self._h, int(block_size), int(stack_window)))
 
-    def trace(self, items, graph=True, stream=None, sync=True, exact_boxes=False):
+    def trace(self, items, graph=True, stream=None, sync=True, exact_boxes=False, scene_shapes=False):
+        """exact_boxes: every item on the exact-box records.  scene_shapes:
+        every scene's items with that scene's own launch shape, schedule and
+        node format (RT_SCENE_SHAPES), one persistent launch per distinct
+        shape."""
         arr = (RtBatchItem * len(items))()
         for i, it in enumerate(items):
             scene, rays, out, mode = it[:4]
@@ -334,6 +348,8 @@ class Batch:
         flags = (_lib.RT_GRAPH if graph else 0) | (0 if sync else _lib.RT_NO_SYNC)
         if exact_boxes:
             flags |= _lib.RT_EXACT_BOXES
+        if scene_shapes:
+            flags |= _lib.RT_SCENE_SHAPES
         s = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
         check(lib().rt_batch_trace(self._h, arr, len(items), flags, s))
 

@@ -282,6 +282,13 @@ int rt_measure_gather(uint32_t active_lanes, uint32_t buffer_bytes, double* wave
  * LDS stack window 8/16/32 entries).  0 keeps the current value. */
 int rt_set_launch(rt_scene_t s, uint32_t block_size, uint32_t stack_window);
 
+/* Node format of this scene's items in batched launches with
+ * RT_SCENE_SHAPES: 0 = the quantized records when the scene has them (the
+ * default), RT_EXACT_BOXES = the exact-box records.  Single-scene calls
+ * choose theirs per call (RT_EXACT_BOXES).  Results are the same either way;
+ * the autotuner sets the faster one.  No reference counterpart. */
+int rt_set_node_format(rt_scene_t s, uint32_t flags);
+
 /* Wave schedule of the per-ray kernel.  0: every lane takes one step per
  * iteration, so a wave runs the node and the leaf code whenever its lanes
  * disagree.  r > 0: each iteration runs one phase only, the leaf phase when
@@ -320,6 +327,12 @@ typedef struct rt_batch_item {
 
 #define RT_GRAPH 128u /* rt_batch_trace: replay the captured hipGraph (re-armed
                          with the new item table) instead of a direct launch  */
+#define RT_SCENE_SHAPES 512u /* rt_batch_trace: every scene's items run with that
+                         scene's own launch shape (rt_set_launch), wave schedule
+                         (rt_set_schedule) and node format (rt_set_node_format);
+                         the items are grouped into one persistent launch per
+                         distinct (format, block size, window, schedule), run
+                         one after another on the stream (one graph)          */
 
 int rt_batch_create(uint32_t max_items, rt_batch_t* out);
 /* Items must all live on the batch's device; total rays < 2^31.  A NULL

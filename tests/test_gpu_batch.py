@@ -165,3 +165,52 @@ def test_batch_many_small_items(rt, scenes):
     torch.cuda.synchronize()
     assert all(torch.equal(out, ref) for ref, out in refs)
     batch.close()
+
+
+def test_batch_scene_shapes(rt, scenes):
+    """RT_SCENE_SHAPES: every scene's items run with that scene's own launch
+    shape, schedule and node format, one persistent launch per distinct shape
+    in one graph.  Every result and count equals the plain launch's, direct
+    and replayed, and after a call with other segments."""
+    import torch
+    shapes = [(64, 8, 16, True), (256, 16, 16, False), (128, 32, 0, False), (128, 8, 16, False), (192, 16, 8, True)]
+    for (bvh, sc, _, _), (bd, S, sched, ex) in zip(scenes, shapes):
+        sc.set_launch(bd, S)
+        sc.set_schedule(sched)
+        sc.exact_boxes = ex
+    batch = rt.Batch(64)
+    order = [(0, "closest"), (1, "any"), (2, "closest"), (3, "closest"), (1, "closest"), (4, "any"), (0, "any"),
+             (4, "closest")]
+    try:
+        for rep, (ordr, graph) in enumerate(((order, False), (order, True), (order, True), (order[:3], True),
+                                            (order, True))):
+            print(f"scene-shape batch call {rep}: {len(ordr)} items, graph={graph}", flush=True)
+            items, outs = [], []
+            for i, kind in ordr:
+                bvh, sc, rays, shadow = scenes[i]
+                rr = rays if kind == "closest" else shadow
+                r = dev(rt, rr)
+                o = (torch.empty((len(rr), 6), dtype=torch.int32, device="cuda") if kind == "closest"
+                     else torch.empty(len(rr), dtype=torch.uint8, device="cuda"))
+                c = torch.empty((len(rr), 4), dtype=torch.int32, device="cuda")
+                items.append((sc, r, o, kind, c))
+                outs.append((i, kind, rr, o, c))
+            batch.trace(items, graph=graph, scene_shapes=True)
+            torch.cuda.synchronize()
+            for i, kind, rr, o, c in outs:
+                sc = scenes[i][1]
+                ref, rc = sc.intersect(rr, mode=kind, counts=True)
+                got = o.cpu().numpy()
+                if kind == "closest":
+                    got = np.ascontiguousarray(got).view(rt.HIT_DTYPE).reshape(-1)
+                assert np.array_equal(got, ref), f"scene {i} {kind} call {rep}"
+                cc = c.cpu().numpy().view(rt.COUNTS_DTYPE).reshape(-1)
+                assert np.array_equal(cc, rc), f"scene {i} {kind} counts call {rep}"
+        st = batch.stats()
+        assert st["error_rays"] == 0 and st["max_stack"] > 0
+    finally:
+        for _, sc, _, _ in scenes:
+            sc.set_launch(256, 16)
+            sc.set_schedule(16)
+            sc.exact_boxes = False
+        batch.close()
