@@ -33,7 +33,7 @@ HBM_PEAK_GBPS = 8000.0
 # and profiles/k_trace_batch_quantized_64scenes.md (the calibration in
 # profiles/k_trace_exact_N8L8_diffuse8M_hbm_traffic.md).
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
-MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 6777475, "write": 742866},
+MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 6776918, "write": 742906},
                         ("batch", 8, 8, 9_872_996): {"fetch": 2318751, "write": 501051}}
 
 # What the timed kernel spends, from rocprofv3 counter runs of this benchmark
@@ -42,9 +42,9 @@ MEASURED_TRAFFIC_KIB = {(8, 8, 8_000_000): {"fetch": 6777475, "write": 742866},
 # texture address unit 70% busy and the data unit 95% (55% of it stalled on
 # the L1), 3.44e8 L1-to-L2 requests (22 GB) per step, 68% L2 hits, and HBM
 # carrying about a quarter of its peak.
-MEASURED_COUNTERS = {(8, 8, 8_000_000): {"td_busy": 0.951, "td_stalled": 0.550, "ta_busy": 0.704,
-                                         "valu_busy_metric": 1.154, "active_lanes": 0.414, "l2_hit": 0.678,
-                                         "l2_read_bytes": 22_015_520_000,
+MEASURED_COUNTERS = {(8, 8, 8_000_000): {"td_busy": 0.955, "td_stalled": 0.551, "ta_busy": 0.696,
+                                         "valu_busy_metric": 1.155, "active_lanes": 0.414, "l2_hit": 0.678,
+                                         "l2_read_bytes": 22_007_040_000,
                                          "valu_insts_per_ray": 573.2, "loads_per_ray": 12.46,
                                          "source": "profiles/k_trace_quantized_N8L8_diffuse8M_counters.md"}}
 
