@@ -6,7 +6,9 @@ equals the plain per-scene launch (hits and counts), and a 2,000-ray sample
 per scene equals the oracle (bit-exact, counts included).  Two batches: all
 scenes at N8L8 (the kernel specialised on node memory 8), and N/L varying
 per scene over sixteen combinations (the generic kernel that reads the node
-memory per scene, as an autotuned batch runs)."""
+memory per scene, as an autotuned batch runs).  A third batch gives every
+scene its own node format, launch shape and schedule (RT_SCENE_SHAPES: one
+persistent launch per distinct shape in the graph)."""
 import numpy as np
 import pytest
 
@@ -29,7 +31,7 @@ def _world(rt, nl_of):
     return out
 
 
-def _check_batch(rt, world, label):
+def _check_batch(rt, world, label, scene_shapes=False):
     import torch
 
     dev = []
@@ -42,7 +44,7 @@ def _check_batch(rt, world, label):
     try:
         items = [(w[2], d_r, o, "closest", c) for w, (d_r, o, c) in zip(world, dev)]
         print(f"{label}: one graph launch over {len(items)} items", flush=True)
-        batch.trace(items, graph=True)
+        batch.trace(items, graph=True, scene_shapes=scene_shapes)
         torch.cuda.synchronize()
         assert batch.stats()["error_rays"] == 0
         g = np.random.default_rng(44)
@@ -81,5 +83,22 @@ def test_config4_mixed_node_and_leaf_sizes(rt):
     world = _world(rt, lambda i: MIXED_NL[i % len(MIXED_NL)])
     try:
         _check_batch(rt, world, "mixed N/L batch")
+    finally:
+        _close(world)
+
+
+SHAPES = [(False, 128, 8, 16), (True, 256, 16, 16), (False, 256, 16, 0), (True, 128, 32, 8), (False, 64, 8, 32),
+          (True, 192, 16, 16)]
+
+
+def test_config4_per_scene_shapes(rt):
+    world = _world(rt, lambda i: MIXED_NL[(3 * i) % len(MIXED_NL)])
+    try:
+        for i, (tris, bvh, sc, rays) in enumerate(world):
+            ex, bd, S, sched = SHAPES[i % len(SHAPES)]
+            sc.exact_boxes = ex
+            sc.set_launch(bd, S)
+            sc.set_schedule(sched)
+        _check_batch(rt, world, "per-scene shape batch", scene_shapes=True)
     finally:
         _close(world)
