@@ -1123,8 +1123,12 @@ extern "C" int rt_release(rt_scene_t s) {
 // One persistent launch of a batched call: the items [i0, i1) of the sorted
 // table, which share a node format and a launch shape.  A call without
 // RT_SCENE_SHAPES is one segment; with it, one segment per distinct
-// (format, block size, stack window, schedule) of the items' scenes, run one
-// after another on the stream.
+// (format, block size, stack window, schedule) of the items' scenes.
+// Segments are independent (each has its own work words and spill range)
+// and run concurrently: independent branches of the graph, or launches on
+// forked streams, so one segment's tail overlaps the others' work.
+constexpr uint32_t kMaxSegs = 16;
+
 struct BatchSeg {
     bool q = false;
     uint32_t bd = 0, S = 0, sched = 0;
@@ -1134,9 +1138,10 @@ struct BatchSeg {
     uint64_t total = 0;
     uint32_t grid = 0, spillCap = 0;
     size_t table = 0;  // LDS bytes reserved for the staged tables
+    size_t spillOff = 0;  // first word of the segment's spill range
     bool same_launch(const BatchSeg& o) const {
         return q == o.q && bd == o.bd && S == o.S && M == o.M && grid == o.grid && table == o.table &&
-               spillCap == o.spillCap;
+               spillCap == o.spillCap && spillOff == o.spillOff;
     }
 };
 
@@ -1148,12 +1153,16 @@ struct rt_batch {
     BatchScene* dScenes = nullptr;
     BatchItem* hItems = nullptr;  // pinned staging of the tables
     BatchScene* hScenes = nullptr;
-    uint32_t* work = nullptr;
+    uint32_t* work = nullptr;  // kWorkWords words per segment, kMaxSegs segments
     uint32_t* spill = nullptr;
     size_t spillBytes = 0;
     uint32_t blockSize = 256, stackWindow = 16, cus = 0;
     bool launchSet = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // direct launches of several segments: forked streams and their events
+    hipEvent_t forkEv = nullptr;
+    std::vector<hipStream_t> aux;
+    std::vector<hipEvent_t> auxEv;
     bool pending = false;
     hipStream_t ownStream = nullptr;
     hipEvent_t joinEv = nullptr;  // orders ownStream with the null stream
@@ -1197,60 +1206,80 @@ void free_batch(rt_batch* b) {
     if (b->spill) (void)hipFree(b->spill);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->forkEv) (void)hipEventDestroy(b->forkEv);
+    for (auto e : b->auxEv) (void)hipEventDestroy(e);
+    for (auto x : b->aux) (void)hipStreamDestroy(x);
     if (b->joinEv) (void)hipEventDestroy(b->joinEv);
     if (b->ownStream) (void)hipStreamDestroy(b->ownStream);
     (void)hipSetDevice(cur);
     delete b;
 }
 
-// Captures the segments' launches (reset kernel, trace kernel, one pair per
-// segment, in order) as one graph and finds the trace kernel nodes by
-// walking the captured chain from its root.
-int capture_graph(rt_batch* b, hipStream_t st, const std::vector<BatchSeg>& segs, const std::vector<KParams>& ps) {
+// Builds the graph node by node: per segment a reset kernel node and the
+// batched trace kernel node after it; segments are independent branches.
+int build_graph(rt_batch* b, hipStream_t st, const std::vector<BatchSeg>& segs, const std::vector<KParams>& ps) {
     drop_graph(b);
-    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    hipError_t e = hipSuccess;
-    for (size_t i = 0; i < segs.size() && e == hipSuccess; ++i) {
-        const BatchSeg& g = segs[i];
-        e = launch_reset(ps[i].work, st);
-        if (e == hipSuccess) e = launch_trace(g.M, kBatch, g.S, g.bd, g.grid, ps[i], st, g.q, g.table);
-    }
     hipGraph_t gr = nullptr;
-    const hipError_t e3 = hipStreamEndCapture(st, &gr);
-    if (e != hipSuccess || e3 != hipSuccess) {
-        if (gr) (void)hipGraphDestroy(gr);
-        return hip_fail(e != hipSuccess ? e : e3, "graph capture");
-    }
+    HIP_TRY(hipGraphCreate(&gr, 0), "hipGraphCreate");
     b->graph = gr;
-    size_t nr = 1;
-    hipGraphNode_t nd = nullptr;
-    HIP_TRY(hipGraphGetRootNodes(gr, &nd, &nr), "hipGraphGetRootNodes");
-    if (nr != 1) return set_error(RT_ERR_HIP, "graph capture: expected one chain");
-    while (nd) {
-        hipGraphNodeType t;
-        HIP_TRY(hipGraphNodeGetType(nd, &t), "hipGraphNodeGetType");
-        if (t == hipGraphNodeTypeKernel) {
-            hipKernelNodeParams kp{};
-            HIP_TRY(hipGraphKernelNodeGetParams(nd, &kp), "hipGraphKernelNodeGetParams");
-            // the trace kernels are the nodes with LDS (stack window and tables)
-            if (kp.sharedMemBytes > 0) {
-                b->traceNodes.push_back(nd);
-                b->traceNodeParams.push_back(kp);
-            }
-        }
-        size_t nd2 = 0;
-        HIP_TRY(hipGraphNodeGetDependentNodes(nd, nullptr, &nd2), "hipGraphNodeGetDependentNodes");
-        if (nd2 > 1) return set_error(RT_ERR_HIP, "graph capture: expected one chain");
-        hipGraphNode_t next = nullptr;
-        if (nd2 == 1) HIP_TRY(hipGraphNodeGetDependentNodes(nd, &next, &nd2), "hipGraphNodeGetDependentNodes");
-        nd = next;
+    for (size_t i = 0; i < segs.size(); ++i) {
+        const BatchSeg& g = segs[i];
+        uint32_t* work = ps[i].work;
+        void* rargs[] = {&work};
+        hipKernelNodeParams rp{};
+        rp.func = const_cast<void*>(reset_kernel());
+        rp.gridDim = dim3(1);
+        rp.blockDim = dim3(64);
+        rp.sharedMemBytes = 0;
+        rp.kernelParams = rargs;
+        rp.extra = nullptr;
+        hipGraphNode_t rn = nullptr, tn = nullptr;
+        HIP_TRY(hipGraphAddKernelNode(&rn, gr, nullptr, 0, &rp), "hipGraphAddKernelNode(reset)");
+        KParams copy = ps[i];
+        void* targs[] = {&copy};
+        hipKernelNodeParams tp{};
+        tp.func = const_cast<void*>(batch_kernel(g.M, g.S, g.q));
+        if (!tp.func) return set_error(RT_ERR_UNSUPPORTED, "rt_batch_trace: no batched kernel for this shape");
+        tp.gridDim = dim3(g.grid);
+        tp.blockDim = dim3(g.bd);
+        tp.sharedMemBytes = (unsigned)(trace_lds_bytes(g.M, kBatch, g.S, g.bd, g.q) + g.table);
+        tp.kernelParams = targs;
+        tp.extra = nullptr;
+        HIP_TRY(hipGraphAddKernelNode(&tn, gr, &rn, 1, &tp), "hipGraphAddKernelNode(trace)");
+        b->traceNodes.push_back(tn);
+        b->traceNodeParams.push_back(tp);
     }
-    if (b->traceNodes.size() != segs.size())
-        return set_error(RT_ERR_HIP, "graph capture: trace kernel nodes not found");
     HIP_TRY(hipGraphInstantiate(&b->exec, gr, nullptr, nullptr, 0), "hipGraphInstantiate");
     b->graphStream = st;
     b->graphSpill = b->spill;
     b->graphSegs = segs;
+    return RT_OK;
+}
+
+// Direct launches of the segments: the first on `st`, the others on forked
+// streams that join `st` again.
+int launch_segments(rt_batch* b, hipStream_t st, const std::vector<BatchSeg>& segs, const std::vector<KParams>& ps) {
+    if (segs.size() > 1) {
+        if (!b->forkEv) HIP_TRY(hipEventCreateWithFlags(&b->forkEv, hipEventDisableTiming), "hipEventCreate");
+        while (b->aux.size() + 1 < segs.size()) {
+            hipStream_t x = nullptr;
+            hipEvent_t e = nullptr;
+            HIP_TRY(hipStreamCreateWithFlags(&x, hipStreamNonBlocking), "hipStreamCreate");
+            b->aux.push_back(x);
+            HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            b->auxEv.push_back(e);
+        }
+        HIP_TRY(hipEventRecord(b->forkEv, st), "hipEventRecord");
+    }
+    for (size_t i = 0; i < segs.size(); ++i) {
+        const BatchSeg& g = segs[i];
+        hipStream_t x = i == 0 ? st : b->aux[i - 1];
+        if (i > 0) HIP_TRY(hipStreamWaitEvent(x, b->forkEv, 0), "hipStreamWaitEvent");
+        HIP_TRY(launch_reset(ps[i].work, x), "reset kernel");
+        HIP_TRY(launch_trace(g.M, kBatch, g.S, g.bd, g.grid, ps[i], x, g.q, g.table), "batched trace kernel");
+        if (i > 0) HIP_TRY(hipEventRecord(b->auxEv[i - 1], x), "hipEventRecord");
+    }
+    for (size_t i = 1; i < segs.size(); ++i) HIP_TRY(hipStreamWaitEvent(st, b->auxEv[i - 1], 0), "hipStreamWaitEvent");
     return RT_OK;
 }
 
@@ -1270,10 +1299,11 @@ extern "C" int rt_batch_create(uint32_t max_items, rt_batch_t* out) {
         (e = hipMalloc(reinterpret_cast<void**>(&b->dScenes), sizeof(BatchScene) * max_items)) != hipSuccess ||
         (e = hipHostMalloc(reinterpret_cast<void**>(&b->hItems), sizeof(BatchItem) * max_items, hipHostMallocDefault)) != hipSuccess ||
         (e = hipHostMalloc(reinterpret_cast<void**>(&b->hScenes), sizeof(BatchScene) * max_items, hipHostMallocDefault)) != hipSuccess ||
-        (e = hipMalloc(reinterpret_cast<void**>(&b->work), kWorkWords * 4)) != hipSuccess ||
+        (e = hipMalloc(reinterpret_cast<void**>(&b->work), kWorkWords * kMaxSegs * 4)) != hipSuccess ||
         (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess ||
         // the guard counter accumulates from here
-        (e = launch_zero(b->work, kWorkWords, nullptr)) != hipSuccess || (e = hipStreamSynchronize(nullptr)) != hipSuccess) {
+        (e = launch_zero(b->work, kWorkWords * kMaxSegs, nullptr)) != hipSuccess ||
+        (e = hipStreamSynchronize(nullptr)) != hipSuccess) {
         free_batch(b);
         return hip_fail(e, "rt_batch_create");
     }
@@ -1439,8 +1469,11 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     }
     // A captured graph with the same segment shapes is re-armed; it keeps the
     // larger LDS table reservation and spill capacity it was made with.
+    if (segs.size() > kMaxSegs)
+        return set_error(RT_ERR_ARG, "rt_batch_trace: more than " + std::to_string(kMaxSegs) +
+                                         " distinct scene launch shapes");
     const bool reuse = graph && b->exec && b->graphStream == st && b->graphSegs.size() == segs.size();
-    size_t spillNeed = 64;
+    size_t spillWords = 0;
     uint32_t maxStack = 0;
     for (size_t i = 0; i < segs.size(); ++i) {
         BatchSeg& g = segs[i];
@@ -1460,11 +1493,14 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
                        : (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(full, (g.total + g.bd - 1) / g.bd));
         g.spillCap = cap > g.S ? cap - g.S : 0;
         if (old && old->spillCap > g.spillCap && old->grid == g.grid) g.spillCap = old->spillCap;
-        spillNeed = std::max<size_t>(spillNeed, (size_t)g.grid * g.bd * g.spillCap * 4 * (g.q ? 2 : 1));
+        // the segments run at once: each has a spill range of its own
+        g.spillOff = spillWords;
+        spillWords += (size_t)g.grid * g.bd * g.spillCap * (g.q ? 2 : 1);
         maxStack = std::max(maxStack, g.spillCap + g.S);
     }
     int rc;
-    if ((rc = ensure(reinterpret_cast<void**>(&b->spill), b->spillBytes, spillNeed, "hipMalloc(spill)")) != RT_OK)
+    if ((rc = ensure(reinterpret_cast<void**>(&b->spill), b->spillBytes, std::max<size_t>(64, spillWords * 4),
+                     "hipMalloc(spill)")) != RT_OK)
         return rc;
     HIP_TRY(hipMemcpyAsync(b->dItems, b->hItems, sizeof(BatchItem) * used, hipMemcpyHostToDevice, st), "items");
     HIP_TRY(hipMemcpyAsync(b->dScenes, b->hScenes, sizeof(BatchScene) * nScenes, hipMemcpyHostToDevice, st), "scenes");
@@ -1474,9 +1510,9 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
         KParams& p = ps[i];
         p = KParams{};
         p.n = (uint32_t)g.total;
-        p.spill = b->spill;
+        p.spill = b->spill + g.spillOff;
         p.spillCap = g.spillCap;
-        p.work = b->work;
+        p.work = b->work + (size_t)kWorkWords * i;
         p.items = b->dItems + g.i0;
         p.scenes = b->dScenes;
         p.nItems = g.i1 - g.i0;
@@ -1494,7 +1530,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
         bool same = reuse && b->graphSpill == b->spill;
         for (size_t i = 0; same && i < segs.size(); ++i) same = segs[i].same_launch(b->graphSegs[i]);
         if (!same) {
-            if ((rc = capture_graph(b, st, segs, ps)) != RT_OK) return rc;
+            if ((rc = build_graph(b, st, segs, ps)) != RT_OK) return rc;
         } else {
             // re-arm: same graph, new kernel arguments (ray counts, item tables)
             for (size_t i = 0; i < segs.size(); ++i) {
@@ -1513,11 +1549,7 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
         ++b->graphLaunches;
     } else {
         HIP_TRY(hipEventRecord(b->ev0, st), "hipEventRecord");
-        for (size_t i = 0; i < segs.size(); ++i) {
-            const BatchSeg& g = segs[i];
-            HIP_TRY(launch_reset(b->work, st), "reset kernel");
-            HIP_TRY(launch_trace(g.M, kBatch, g.S, g.bd, g.grid, ps[i], st, g.q, g.table), "batched trace kernel");
-        }
+        if ((rc = launch_segments(b, st, segs, ps)) != RT_OK) return rc;
         HIP_TRY(hipEventRecord(b->ev1, st), "hipEventRecord");
     }
     b->pending = true;
@@ -1551,9 +1583,11 @@ extern "C" int rt_batch_stats(rt_batch_t b, rt_stats_t* o) {
     o->max_stack = b->maxStack;
     // the guard counter accumulates on the device over the batch's launches
     if (b->pending) HIP_TRY(hipEventSynchronize(b->ev1), "hipEventSynchronize(stats)");
-    uint32_t errs = 0;
-    HIP_TRY(hipMemcpy(&errs, b->work + kErrWord, 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
-    o->error_rays = errs;
+    std::vector<uint32_t> words((size_t)kWorkWords * kMaxSegs);
+    HIP_TRY(hipMemcpy(words.data(), b->work, words.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy(errors)");
+    uint64_t errs = 0;
+    for (uint32_t i = 0; i < kMaxSegs; ++i) errs += words[(size_t)kWorkWords * i + kErrWord];
+    o->error_rays = (uint32_t)std::min<uint64_t>(errs, 0xFFFFFFFFu);
     return RT_OK;
 }
 

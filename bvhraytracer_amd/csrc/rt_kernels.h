@@ -123,6 +123,14 @@ int trace_occupancy(uint32_t M, int mode, uint32_t stackWindow, uint32_t blockSi
 // LDS bytes of a batched launch's staged tables.
 size_t batch_table_bytes(uint32_t nScenes, uint32_t nItems);
 
+// Kernel functions for graphs built node by node: the batched kernel for
+// node memory M (0: the grouped one), window S and node format (grid and
+// block as launch_trace, dynamic LDS trace_lds_bytes(M, kBatch, ...) plus the
+// tables; one KParams argument), and the reset kernel (1 block of 64
+// threads, one uint32_t* argument: the work words).
+const void* batch_kernel(uint32_t M, uint32_t S, bool quantized);
+const void* reset_kernel();
+
 // Dynamic LDS bytes of the per-ray kernel (stack window, and for the
 // save-distance mode the stack distances and aabbDistances, for the
 // quantized records the entries' tmax).

@@ -2022,6 +2022,10 @@ hipError_t launch_trace(uint32_t M, int mode, uint32_t S, uint32_t bd, uint32_t 
                            stream);
 }
 
+const void* batch_kernel(uint32_t M, uint32_t S, bool quantized) { return pick(M, kBatch, S, quantized); }
+
+const void* reset_kernel() { return reinterpret_cast<const void*>(&k_reset); }
+
 size_t trace_lds_bytes(uint32_t M, int mode, uint32_t S, uint32_t bd, bool quantized) {
     return lds_bytes(M, mode, S, bd, quantized);
 }

@@ -331,8 +331,9 @@ typedef struct rt_batch_item {
                          scene's own launch shape (rt_set_launch), wave schedule
                          (rt_set_schedule) and node format (rt_set_node_format);
                          the items are grouped into one persistent launch per
-                         distinct (format, block size, window, schedule), run
-                         one after another on the stream (one graph)          */
+                         distinct (format, block size, window, schedule), at
+                         most 16, which run concurrently (independent branches
+                         of one graph, or forked streams joined again)        */
 
 int rt_batch_create(uint32_t max_items, rt_batch_t* out);
 /* Items must all live on the batch's device; total rays < 2^31.  A NULL
