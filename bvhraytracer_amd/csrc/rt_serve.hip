@@ -40,6 +40,7 @@ struct Request {
     uint32_t scene = 0, n = 0, flags = 0;
     const rt_ray* rays = nullptr;
     void* out = nullptr;
+    bool device = false;  // rays and out in device memory: no pool region, no host copies
     size_t rayBytes = 0, outBytes = 0, bytes = 0;
     size_t offset = 0;  // pool offset (rays at offset, results at offset + rayBytes)
     int state = kQueued;
@@ -235,8 +236,10 @@ void complete(rt_server* sv, std::vector<Retired>& done) {
         r->tDone = now;
         r->state = d.err == RT_OK ? kDone : kFailed;
         r->error = d.err;
-        sv->pool.release(r->offset, r->bytes);
-        sv->admittedBytes -= r->bytes;
+        if (!r->device) {
+            sv->pool.release(r->offset, r->bytes);
+            sv->admittedBytes -= r->bytes;
+        }
         if (d.err == RT_OK) {
             ++sv->completed;
             sv->raysDone += r->n;
@@ -254,13 +257,18 @@ int launch(rt_server* sv, Slot& sl, std::vector<Request*>& batch) {
     std::vector<rt_batch_item> items;
     items.reserve(batch.size());
     for (Request* r : batch) {
-        char* d = sv->dPool + r->offset;
-        if (hipMemcpyAsync(d, sv->hPool + r->offset, r->rayBytes, hipMemcpyHostToDevice, sl.stream) != hipSuccess)
-            return set_error(RT_ERR_HIP, "rt_server: ray upload failed");
         rt_batch_item it{};
         it.scene = sv->scenes[r->scene];
-        it.rays = reinterpret_cast<const rt_ray*>(d);
-        it.out = d + r->rayBytes;
+        if (r->device) {
+            it.rays = r->rays;
+            it.out = r->out;
+        } else {
+            char* d = sv->dPool + r->offset;
+            if (hipMemcpyAsync(d, sv->hPool + r->offset, r->rayBytes, hipMemcpyHostToDevice, sl.stream) != hipSuccess)
+                return set_error(RT_ERR_HIP, "rt_server: ray upload failed");
+            it.rays = reinterpret_cast<const rt_ray*>(d);
+            it.out = d + r->rayBytes;
+        }
         it.counts = nullptr;
         it.n = r->n;
         it.flags = r->flags & RT_ANY;
@@ -270,6 +278,7 @@ int launch(rt_server* sv, Slot& sl, std::vector<Request*>& batch) {
     int rc = rt_batch_trace(sl.batch, items.data(), (uint32_t)items.size(), fl, sl.stream);
     if (rc != RT_OK) return rc;
     for (Request* r : batch) {
+        if (r->device) continue;
         if (hipMemcpyAsync(sv->hPool + r->offset + r->rayBytes, sv->dPool + r->offset + r->rayBytes, r->outBytes,
                            hipMemcpyDeviceToHost, sl.stream) != hipSuccess)
             return set_error(RT_ERR_HIP, "rt_server: result download failed");
@@ -297,6 +306,14 @@ void dispatcher(rt_server* sv) {
         // 2. admit in arrival order while the footprint fits
         while (!sv->queue.empty()) {
             Request* r = sv->queue.front();
+            if (r->device) {
+                // the caller's device buffers: nothing to allocate or copy
+                sv->queue.pop_front();
+                r->state = kStaging;
+                staging.push_back(r);
+                progress = true;
+                continue;
+            }
             size_t off = 0;
             if (!sv->pool.alloc(r->bytes, off)) break;
             sv->queue.pop_front();
@@ -315,8 +332,10 @@ void dispatcher(rt_server* sv) {
             lk.unlock();
             jobs.clear();
             for (const Retired& d : finished)
-                if (d.err == RT_OK) jobs.push_back({d.r->out, sv->hPool + d.r->offset + d.r->rayBytes, d.r->outBytes});
-            for (Request* r : staging) jobs.push_back({sv->hPool + r->offset, r->rays, (size_t)r->n * sizeof(rt_ray)});
+                if (d.err == RT_OK && !d.r->device)
+                    jobs.push_back({d.r->out, sv->hPool + d.r->offset + d.r->rayBytes, d.r->outBytes});
+            for (Request* r : staging)
+                if (!r->device) jobs.push_back({sv->hPool + r->offset, r->rays, (size_t)r->n * sizeof(rt_ray)});
             sv->copies.run(jobs);
             lk.lock();
             if (!finished.empty()) {
@@ -463,7 +482,8 @@ extern "C" int rt_server_submit(rt_server_t sv, uint32_t scene_id, const rt_ray*
                                 void* out, uint64_t* request_id) {
     if (!sv || !request_id) return set_error(RT_ERR_ARG, "rt_server_submit: null argument");
     if (n == 0 || !rays || !out) return set_error(RT_ERR_ARG, "rt_server_submit: empty request");
-    if (flags & ~RT_ANY) return set_error(RT_ERR_UNSUPPORTED, "rt_server_submit: RT_CLOSEST or RT_ANY only");
+    if (flags & ~(RT_ANY | RT_DEVICE_PTRS))
+        return set_error(RT_ERR_UNSUPPORTED, "rt_server_submit: RT_CLOSEST or RT_ANY, optionally RT_DEVICE_PTRS");
     Request* r = new (std::nothrow) Request;
     if (!r) return set_error(RT_ERR_NOMEM, "rt_server_submit: out of host memory");
     r->scene = scene_id;
@@ -471,9 +491,10 @@ extern "C" int rt_server_submit(rt_server_t sv, uint32_t scene_id, const rt_ray*
     r->flags = flags;
     r->rays = rays;
     r->out = out;
+    r->device = (flags & RT_DEVICE_PTRS) != 0;
     r->rayBytes = align_up((size_t)n * sizeof(rt_ray));
     r->outBytes = (flags & RT_ANY) ? (size_t)n : (size_t)n * sizeof(rt_hit);
-    r->bytes = r->rayBytes + align_up(r->outBytes);
+    r->bytes = r->device ? 0 : r->rayBytes + align_up(r->outBytes);
     std::lock_guard<std::mutex> lk(sv->mu);
     if (scene_id >= sv->scenes.size()) {
         delete r;
@@ -532,8 +553,10 @@ extern "C" int rt_server_cancel(rt_server_t sv, uint64_t id) {
         auto a = std::find(sv->admitted.begin(), sv->admitted.end(), r);
         if (a == sv->admitted.end()) return set_error(RT_ERR_BUSY, "rt_server_cancel: request already being staged or launched");
         sv->admitted.erase(a);
-        sv->pool.release(r->offset, r->bytes);
-        sv->admittedBytes -= r->bytes;
+        if (!r->device) {
+            sv->pool.release(r->offset, r->bytes);
+            sv->admittedBytes -= r->bytes;
+        }
     }
     r->state = kCancelled;
     ++sv->cancelled;

@@ -392,7 +392,19 @@ class Server:
 
     def submit(self, scene_id, rays, mode="closest", out=None):
         """Returns (request id, output array filled when the request completes).
-        out: optional caller-owned result array (HIT_DTYPE or uint8, len(rays))."""
+        out: optional caller-owned result array (HIT_DTYPE or uint8, len(rays)).
+        Device requests: rays a CUDA tensor of (n, 10) float32 ray records and
+        out a CUDA tensor of (n, 6) int32 (closest) or n uint8 (any); the
+        server reads and writes them in place (RT_DEVICE_PTRS)."""
+        if hasattr(rays, "is_cuda") and rays.is_cuda:
+            n = rays.shape[0]
+            if out is None or not getattr(out, "is_cuda", False) or out.shape[0] != n or not out.is_contiguous():
+                raise ValueError("submit: a device request needs a contiguous CUDA `out` of len(rays) results")
+            rid = ctypes.c_uint64()
+            check(lib().rt_server_submit(self._h, int(scene_id), ptr(rays), n, MODES[mode] | _lib.RT_DEVICE_PTRS,
+                                         ptr(out), ctypes.byref(rid)))
+            self._live[rid.value] = (rays, out)
+            return rid.value, out
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
         want = np.dtype(np.uint8) if mode == "any" else HIT_DTYPE
         if out is None:

@@ -387,7 +387,12 @@ int rt_server_create(const rt_server_config* cfg, rt_server_t* out);
 int rt_server_add_scene(rt_server_t sv, rt_scene_t scene, uint32_t* scene_id);
 /* Queues a request.  rays/out are host buffers that must stay valid until the
  * request completes (out: rt_hit[n] for RT_CLOSEST, uint8_t[n] for RT_ANY).
- * RT_ERR_BUSY: the request alone exceeds the admission cap. */
+ * RT_ERR_BUSY: the request alone exceeds the admission cap.  With
+ * RT_DEVICE_PTRS, rays/out are device buffers on the server's device (rays
+ * made by rt_spawn_*_device, results consumed on the device): the launch reads
+ * and writes them in place, with no host copies and no bytes of the pool (the
+ * admission cap bounds host requests only); they are admitted in arrival
+ * order like the others, and rt_server_wait returns once `out` is written. */
 int rt_server_submit(rt_server_t sv, uint32_t scene_id, const rt_ray* rays, uint32_t n,
                      uint32_t flags, void* out, uint64_t* request_id);
 /* RT_OK when the request completed, RT_ERR_BUSY on timeout, the launch's error

@@ -56,6 +56,49 @@ def test_mix_equals_alone(rt, world):
     srv.close()
 
 
+def test_device_requests_mixed_with_host_requests(rt, world):
+    """Requests whose rays and results stay in device memory (RT_DEVICE_PTRS)
+    join the same launches as host requests; each result equals the request
+    traced alone.  A small admission cap holds host requests back (only they
+    take pool bytes) while device requests pass."""
+    import torch
+    srv = rt.Server(hbm_cap_bytes=4 << 20, slots=2)
+    ids = [srv.add_scene(sc) for _, sc, _, _ in world]
+    g = np.random.default_rng(6)
+    reqs = []
+    for i in range(80):
+        s = int(g.integers(0, len(world)))
+        mode = "any" if g.random() < 0.5 else "closest"
+        src = world[s][3] if mode == "any" else world[s][2]
+        n = int(min(len(src), g.integers(500, 30_000)))
+        off = int(g.integers(0, len(src) - n + 1))
+        rays = src[off:off + n]
+        if i % 2:
+            d_r = torch.from_numpy(np.ascontiguousarray(rays).view(np.float32).reshape(-1, 10)).cuda()
+            d_o = (torch.full((n,), 7, dtype=torch.uint8, device="cuda") if mode == "any"
+                   else torch.full((n, 6), -7, dtype=torch.int32, device="cuda"))
+            rid, out = srv.submit(ids[s], d_r, mode, out=d_o)
+        else:
+            rid, out = srv.submit(ids[s], rays, mode)
+        reqs.append((rid, out, s, mode, rays))
+    print(f"submitted {len(reqs)} requests, half of them device-resident", flush=True)
+    for rid, *_ in reqs:
+        srv.wait(rid, timeout=60)
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    for rid, out, s, mode, rays in reqs:
+        sc = world[s][1]
+        alone = sc.intersect(rays, mode=mode)
+        got = out.cpu().numpy() if hasattr(out, "is_cuda") else out
+        if mode == "any":
+            assert np.array_equal(np.asarray(got).reshape(-1), alone)
+        else:
+            assert np.array_equal(np.ascontiguousarray(got).view(np.uint8).reshape(-1), alone.view(np.uint8))
+    with pytest.raises(ValueError):
+        srv.submit(ids[0], torch.zeros((4, 10), dtype=torch.float32, device="cuda"), "closest")
+    srv.close()
+
+
 def test_admission_cancel_and_errors(rt, world):
     bvh, sc, rays, _ = world[0]
     # 1 MiB cap: a 100k-ray request (6.4 MB) can never be admitted
