@@ -450,11 +450,13 @@ def bench_render_frame(rt, scene, bvh, tris, log, reps=3):
     return out
 
 
-def serve_run(rt, srv, sids, world, rate, seconds, seed):
+def serve_run(rt, srv, sids, world, rate, seconds, seed, device=None):
     """Poisson arrivals at `rate` req/s for `seconds` (workloads.serving_requests)
     submitted at their arrival times from this thread while a second thread
     waits for them in order.  Returns the server's statistics of the window
-    plus the arrival rate the submitting thread actually reached."""
+    plus the arrival rate the submitting thread actually reached.  device:
+    per scene the (closest-hit rays, shadow rays) as CUDA tensors; the
+    requests then stay in device memory (RT_DEVICE_PTRS), results too."""
     import queue
     import threading
 
@@ -465,8 +467,14 @@ def serve_run(rt, srv, sids, world, rate, seconds, seed):
     # result buffers allocated and touched before the window, as a client
     # that reuses its buffers has them (first-touch page faults of fresh
     # allocations would otherwise be timed as part of the server's copies)
-    outs = [np.zeros(int(sizes[k]), np.uint8) if shadow[k] else np.zeros(int(sizes[k]), rt.HIT_DTYPE)
-            for k in range(n_req)]
+    if device is None:
+        outs = [np.zeros(int(sizes[k]), np.uint8) if shadow[k] else np.zeros(int(sizes[k]), rt.HIT_DTYPE)
+                for k in range(n_req)]
+    else:
+        import torch
+        outs = [torch.zeros(int(sizes[k]), dtype=torch.uint8, device="cuda") if shadow[k]
+                else torch.zeros((int(sizes[k]), 6), dtype=torch.int32, device="cuda") for k in range(n_req)]
+        torch.cuda.synchronize()
     srv.stats(reset=True)
     done_q = queue.SimpleQueue()
 
@@ -487,8 +495,8 @@ def serve_run(rt, srv, sids, world, rate, seconds, seed):
         dt = at - time.perf_counter()
         if dt > 0:
             time.sleep(dt)
-        w = world[scene_of[i]]
-        src = w[4] if shadow[i] else w[3]
+        w = world[scene_of[i]] if device is None else device[scene_of[i]]
+        src = (w[4] if shadow[i] else w[3]) if device is None else (w[1] if shadow[i] else w[0])
         rid, _ = srv.submit(sids[scene_of[i]], src[offs[i]:offs[i] + sizes[i]], "any" if shadow[i] else "closest",
                             out=outs[i])
         done_q.put(rid)
@@ -505,6 +513,7 @@ def serve_run(rt, srv, sids, world, rate, seconds, seed):
 
 SWEEP_RATES = (2000, 10000, 20000, 40000)
 SWEEP_REQUESTS = 3000
+DEVICE_SWEEP_RATES = (40000, 80000)
 
 
 def bench_serving(rt, args, log):
@@ -514,7 +523,10 @@ def bench_serving(rt, args, log):
     1.3), 1024, 2^19), half shadow (any-hit), half closest-hit; latency from
     submit to results in host memory.  Then a sweep of higher arrival rates,
     3000 requests each (seed 6), stopping at the first failed request: where
-    served Mrays/s stops following the offered load is the saturation knee."""
+    served Mrays/s stops following the offered load is the saturation knee.
+    Last, the same requests at DEVICE_SWEEP_RATES with rays and results in
+    device memory (RT_DEVICE_PTRS), which removes the host copies that bound
+    the host sweep."""
     from bvhraytracer_amd import workloads as wl
 
     t0 = time.time()
@@ -539,10 +551,28 @@ def bench_serving(rt, args, log):
                 f"{r['requests_per_launch']} requests/launch, failed {r['failed']}")
             if r["failed"]:
                 break
+    device_sweep = []
+    if not args.no_serve_sweep:
+        # the same requests with rays and results in device memory (a renderer
+        # that spawns rays on the GPU): no host copies on the request path
+        import torch
+        dev = [(torch.from_numpy(np.ascontiguousarray(w[3]).view(np.float32).reshape(-1, 10)).cuda(),
+                torch.from_numpy(np.ascontiguousarray(w[4]).view(np.float32).reshape(-1, 10)).cuda()) for w in world]
+        for rate in DEVICE_SWEEP_RATES:
+            r = serve_run(rt, srv, sids, world, rate, SWEEP_REQUESTS / rate, 6, device=dev)
+            device_sweep.append(r)
+            log(f"serving device requests at {rate} req/s (submitted {r['submitted_rate']}): offered "
+                f"{r['offered_mrays_per_s']} Mrays/s, served {r['mrays_per_s']} Mrays/s, p50 {r['p50_ms']} ms, "
+                f"p99 {r['p99_ms']} ms, {r['requests_per_launch']} requests/launch, failed {r['failed']}")
+            if r["failed"]:
+                break
+        del dev
     srv.close()
     for w in world:
         w[2].close()
-    out = {"scenes": 16, "seconds": args.serve_seconds, **base, "sweep": sweep}
+    out = {"scenes": 16, "seconds": args.serve_seconds, **base, "sweep": sweep, "device_sweep": device_sweep}
+    if device_sweep:
+        out["peak_served_device_mrays_per_s"] = max(r["mrays_per_s"] for r in device_sweep)
     if sweep:
         # knee: the first rate whose served throughput falls below 90% of what was offered
         knee = next((r["arrival_rate"] for r in sweep if r["mrays_per_s"] < 0.9 * r["offered_mrays_per_s"]), None)
