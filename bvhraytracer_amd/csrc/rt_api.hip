@@ -1467,11 +1467,11 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
         g.total += bi.n;
         g.groupEnd[g.nGroups - 1] = (uint32_t)g.total;
     }
-    // A captured graph with the same segment shapes is re-armed; it keeps the
-    // larger LDS table reservation and spill capacity it was made with.
     if (segs.size() > kMaxSegs)
         return set_error(RT_ERR_ARG, "rt_batch_trace: more than " + std::to_string(kMaxSegs) +
                                          " distinct scene launch shapes");
+    // A captured graph with the same segment shapes is re-armed; it keeps the
+    // larger LDS table reservation and spill capacity it was made with.
     const bool reuse = graph && b->exec && b->graphStream == st && b->graphSegs.size() == segs.size();
     size_t spillWords = 0;
     uint32_t maxStack = 0;
