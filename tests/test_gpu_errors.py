@@ -106,3 +106,37 @@ def test_batch_argument_errors(rt, small):
         assert_closest_equal(got, ref["hits"], bvh, None, None, "batch after refused calls")
     finally:
         b.close()
+
+
+def test_batch_scene_shape_limits(rt, small):
+    """RT_SCENE_SHAPES runs one launch per distinct scene shape, at most 16:
+    17 scenes with 17 schedules are refused with RT_ERR_ARG, and the same
+    batch with 16 of them traces every item as the plain launch does.
+    rt_set_node_format takes RT_EXACT_BOXES only."""
+    import torch
+
+    bvh, _, rays = small
+    n = len(rays)
+    d_rays = torch.from_numpy(rays.view(np.float32).reshape(n, 10)).cuda()
+    scenes = [rt.Scene(bvh) for _ in range(17)]
+    outs = [torch.zeros((n, 6), dtype=torch.int32, device="cuda") for _ in scenes]
+    b = rt.Batch(max_items=32)
+    try:
+        for i, sc in enumerate(scenes):
+            sc.set_schedule(i + 1)
+        items = [(sc, d_rays, o, "closest") for sc, o in zip(scenes, outs)]
+        print("17 distinct scene shapes", flush=True)
+        with pytest.raises(rt.RtError, match="distinct scene launch shapes"):
+            b.trace(items, scene_shapes=True)
+        b.trace(items[:16], scene_shapes=True)
+        ref = scenes[0].intersect(rays)
+        for o in outs[:16]:
+            got = o.cpu().numpy().view(rt.HIT_DTYPE).reshape(-1)
+            assert np.array_equal(got, ref)
+        lib = rt.lib()
+        assert lib.rt_set_node_format(scenes[0]._h, _lib.RT_COUNTS) == -1
+        assert b"RT_EXACT_BOXES only" in lib.rt_last_error()
+    finally:
+        b.close()
+        for sc in scenes:
+            sc.close()
