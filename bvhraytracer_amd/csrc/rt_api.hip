@@ -1132,7 +1132,8 @@ constexpr uint32_t kMaxSegs = 16;
 struct BatchSeg {
     bool q = false;
     uint32_t bd = 0, S = 0, sched = 0;
-    uint32_t M = 0;  // node memory of every scene of the segment, 0 when they differ
+    uint32_t M = 0;     // node memory of every scene of the segment, or kBatchMixed(Small)
+    uint32_t maxM = 0;  // largest node memory of the segment's scenes
     uint32_t i0 = 0, i1 = 0;
     uint32_t nGroups = 0, groupKey[kMaxGroups] = {}, groupEnd[kMaxGroups] = {};
     uint64_t total = 0;
@@ -1456,7 +1457,8 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
         }
         BatchSeg& g = segs.back();
         g.i1 = i + 1;
-        if (b->hScenes[bi.scene].M != g.M) g.M = 0;
+        if (b->hScenes[bi.scene].M != g.M) g.M = kBatchMixed;
+        g.maxM = std::max(g.maxM, b->hScenes[bi.scene].M);
         const uint32_t k = gkey(bi);
         if (g.nGroups == 0 || g.groupKey[g.nGroups - 1] != k) {
             if (g.nGroups == kMaxGroups)
@@ -1470,6 +1472,9 @@ extern "C" int rt_batch_trace(rt_batch_t b, const rt_batch_item* items, uint32_t
     if (segs.size() > kMaxSegs)
         return set_error(RT_ERR_ARG, "rt_batch_trace: more than " + std::to_string(kMaxSegs) +
                                          " distinct scene launch shapes");
+    // mixed node memories all up to 8: the grouped kernel without the larger bodies
+    for (BatchSeg& g : segs)
+        if (g.M == kBatchMixed && g.maxM <= 8) g.M = kBatchMixedSmall;
     // A captured graph with the same segment shapes is re-armed; it keeps the
     // larger LDS table reservation and spill capacity it was made with.
     const bool reuse = graph && b->exec && b->graphStream == st && b->graphSegs.size() == segs.size();

@@ -105,8 +105,14 @@ struct KParams {
     uint32_t groupEnd[kMaxGroups];
 };
 
+// Node memory argument of a batched launch over scenes of several node
+// memories (the grouped kernel): all of them up to 16, or all up to 8.
+constexpr uint32_t kBatchMixed = 0;
+constexpr uint32_t kBatchMixedSmall = 1;
+
 // Launches the per-ray kernel for node memory M on `stream`; mode kBatch
-// launches the batched kernel (M = 0: the grouped one, p.nGroups set).
+// launches the batched kernel (M = kBatchMixed or kBatchMixedSmall: the
+// grouped one).
 // quantized: p.nodes (every scene's nodes, batched) are the quantized
 // records and p.spill has a second region of spillCap floats per slot
 // (closest, any and batched launches).

@@ -1332,8 +1332,11 @@ k_trace_batch(KParams p) {
     }
 }
 
-// Scenes of several node memories.
-template <int S, bool Q>
+// Scenes of several node memories, each at most MMAX (8 or 16): the kernel
+// holds one body per node memory up to MMAX, and its registers are those of
+// the largest (without the 12- and 16-slot bodies the quantized form does
+// not spill).
+template <int S, bool Q, int MMAX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 k_trace_groups(KParams p) {
     extern __shared__ uint32_t smem[];
@@ -1345,8 +1348,12 @@ k_trace_groups(KParams p) {
         switch (key >> 1) {
             case 4: batch_group<4, S, Q>(p, key, counter, base, end - base); break;
             case 8: batch_group<8, S, Q>(p, key, counter, base, end - base); break;
-            case 12: batch_group<12, S, Q>(p, key, counter, base, end - base); break;
-            case 16: batch_group<16, S, Q>(p, key, counter, base, end - base); break;
+            case 12:
+                if constexpr (MMAX > 8) batch_group<12, S, Q>(p, key, counter, base, end - base);
+                break;
+            case 16:
+                if constexpr (MMAX > 8) batch_group<16, S, Q>(p, key, counter, base, end - base);
+                break;
             default: break;
         }
         base = end;
@@ -1947,9 +1954,20 @@ void* pick_batch_s(uint32_t S) {
     }
 }
 
+template <bool Q, int MMAX>
+void* pick_groups_s(uint32_t S) {
+    switch (S) {
+        case 8: return reinterpret_cast<void*>(&k_trace_groups<8, Q, MMAX>);
+        case 16: return reinterpret_cast<void*>(&k_trace_groups<16, Q, MMAX>);
+        case 32: return reinterpret_cast<void*>(&k_trace_groups<32, Q, MMAX>);
+        default: return nullptr;
+    }
+}
+
 // Batched kernel: specialised on the node memory when every scene of the
-// batch has the same M; otherwise (M = 0) the grouped kernel, which runs
-// each group of scenes with the body specialised on its node memory.
+// batch has the same M; otherwise the grouped kernel (kBatchMixed: node
+// memories up to 16; kBatchMixedSmall: up to 8), which runs each group of
+// scenes with the body specialised on its node memory.
 template <bool Q>
 void* pick_batch_q(uint32_t S, uint32_t M) {
     switch (M) {
@@ -1957,13 +1975,9 @@ void* pick_batch_q(uint32_t S, uint32_t M) {
         case 8: return pick_batch_s<8, Q>(S);
         case 12: return pick_batch_s<12, Q>(S);
         case 16: return pick_batch_s<16, Q>(S);
-        default:
-            switch (S) {
-                case 8: return reinterpret_cast<void*>(&k_trace_groups<8, Q>);
-                case 16: return reinterpret_cast<void*>(&k_trace_groups<16, Q>);
-                case 32: return reinterpret_cast<void*>(&k_trace_groups<32, Q>);
-                default: return nullptr;
-            }
+        case kBatchMixed: return pick_groups_s<Q, 16>(S);
+        case kBatchMixedSmall: return pick_groups_s<Q, 8>(S);
+        default: return nullptr;
     }
 }
 

@@ -95,12 +95,14 @@ def test_batch_equals_plain(rt, scenes):
 def test_batch_uniform_node_memory(rt, scenes):
     """Batches whose scenes share one node memory run the kernel specialised
     on it; a graph captured for one node memory is re-captured when the next
-    batch mixes them.  Every result equals the plain launch's."""
+    batch mixes them (the grouped kernel, in its form for node memories up to
+    8 when no scene has more).  Every result equals the plain launch's."""
     import torch
     batch = rt.Batch(16)
     same_m = [(1, "closest"), (4, "any"), (4, "closest"), (1, "any")]  # N8L8 and N8L4: M = 8
     mixed = [(0, "closest"), (1, "closest"), (2, "any"), (3, "closest")]
-    for rep, order in enumerate((same_m, mixed, same_m)):
+    small_m = [(0, "closest"), (1, "any"), (4, "closest"), (0, "any")]  # M = 4 and 8: the small grouped kernel
+    for rep, order in enumerate((same_m, mixed, same_m, small_m, mixed, small_m)):
         print(f"uniform-M batch call {rep}: {len(order)} items", flush=True)
         outs = run_batch(rt, scenes, order, True, batch)
         for i, kind, o, c in outs:
