@@ -121,10 +121,10 @@ def test_batch_uniform_node_memory(rt, scenes):
 
 
 def test_batch_many_small_items(rt, scenes):
-    """3000 items of a few rays over five scenes (the kernel stages the item
-    table in LDS, many rows per thread): every ray equals the plain launch,
-    direct and replayed, and a later batch of 50 items replays the graph
-    captured with the larger table.  A batch whose tables cannot fit the
+    """9000 items of a few rays over five scenes (the kernel stages the item
+    table in LDS, 144 KB of it, many rows per thread): every ray equals the
+    plain launch, direct and replayed, and a later batch of 50 items replays
+    the graph captured with the larger table.  A batch whose tables cannot fit the
     kernel's LDS is refused with RT_ERR_ARG and leaves the batch usable."""
     import torch
     refs, items, spans = [], [], []
@@ -134,7 +134,7 @@ def test_batch_many_small_items(rt, scenes):
         sc.intersect_device(r, ref)
         out = torch.full_like(ref, -7)
         refs.append((ref, out))
-        cuts = np.linspace(0, len(rays), 601).astype(int)
+        cuts = np.linspace(0, len(rays), 1801).astype(int)
         items += [(sc, r[a:b], out[a:b], "closest") for a, b in zip(cuts[:-1], cuts[1:])]
         spans += [(i, a, b) for a, b in zip(cuts[:-1], cuts[1:])]
     few = [(it, sp) for it, sp in zip(items, spans) if sp[0] == 2][:50]
